@@ -1,0 +1,220 @@
+"""Benchmark of the FFT-VIE hot path on one MI355X.
+
+Workload (the headline metric's configuration): synthetic head-like phantom,
+three nested ellipsoids (semi-axes 0.09/0.08/0.11 m, shells 1/0.85/0.7,
+per-shell eps_r ~ U[20,80], sigma ~ U[0.1,2] S/m, seed 0) scaled by 0.85 so
+the outer shell fits the 128^3 grid at 1.5 mm (0.192 m), 298 MHz, JVIE
+form II, complex128, 256^3 x 6 circulant assembled on the device.
+
+A step is one matvec y = A_N x of the JVIE operator over the compact
+non-air DOFs (mv_AN 'notransp'), x and y resident in HBM. `value` is FP64
+matvecs/s; the same line carries FP32 matvecs/s, GMRES(50) time-to-1e-6
+(true residual) for plain FP64 and for complex64 matvecs with FP64 iterative
+refinement, the refined solution's error against the FP64 solution, the
+per-kernel HIP-event timings with their HBM rates, the roofline of the
+dominant kernel and a CPU baseline (the oracle's numpy/scipy restatement of
+mv_AN on a bounded sample).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128]
+                       [--no-solve] [--no-cpu] [--profile-only]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print('[bench]', *a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(L, M, N, Px, Py, Pz, nidx, s):
+    """Compulsory HBM bytes of each matvec kernel (s = bytes per complex)."""
+    n = L * M * N
+    A = 3 * Px * M * N * s
+    B = 3 * Px * Py * N * s
+    fN = 6 * Px * Py * Pz * s
+    pos = 3 * n * 4
+    return {
+        'fwd_x': pos + nidx * s + A,
+        'fwd_y': A + B,
+        'z_product': 2 * B + fN,
+        'inv_y': B + A,
+        'inv_x_epilogue': A + pos + 2 * nidx * s + 2 * n * s,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--size', type=int, default=128)
+    ap.add_argument('--no-solve', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--profile-only', action='store_true',
+                    help='assemble and run the timed FP64 matvecs only (for rocprofv3 runs)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path (the sharding design is '
+              'described in DESIGN.md); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from marie_amd import geometry, lib, vie
+
+    dev = torch.device('cuda:0')
+    torch.cuda.init()
+    n1 = args.size
+    L = M = N = n1
+    dx = 1.5e-3 * 128 / n1
+    freq = 298e6
+    scale = 0.85
+    r, eps_r, sig, ph = geometry.head_phantom(L, M, N, dx, seed=0, scale=scale)
+    em = vie.EMconstants(freq)
+    e_r = eps_r - 1j * sig / (em['eo'] * em['omega'])
+    Einc, _ = vie.PlaneWave_Excitation(r, [0, 0, em['ko']], em['omega_mu'], [1, 0, 0])
+    Mr, Mc, Gram, idx1, Vrhs = vie.jvie_system(Einc, e_r, r, freq, form=2)
+    nidx = idx1.size
+    log('grid %d^3, dx %.3g mm, non-air DOF %d' % (n1, dx * 1e3, nidx))
+
+    op = lib.Operator(L, M, N, precisions=lib.PREC_F64 | lib.PREC_F32)
+    stream = torch.cuda.current_stream(dev)
+    op.set_stream(stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.assemble_N(dx, em['ko'])
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    log('assembly %.3f s' % t_asm)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx1, one_based=True)
+
+    g = torch.Generator(device='cpu').manual_seed(1234)
+    xr = torch.randn(nidx, generator=g, dtype=torch.float64)
+    xi = torch.randn(nidx, generator=g, dtype=torch.float64)
+    x64 = torch.complex(xr, xi).to(dev)
+    y64 = torch.empty_like(x64)
+    x32 = x64.to(torch.complex64)
+    y32 = torch.empty_like(x32)
+
+    def run(prec, k):
+        xp, yp = (x64, y64) if prec == lib.PREC_F64 else (x32, y32)
+        for _ in range(k):
+            op.apply_device(xp.data_ptr(), yp.data_ptr(), lib.OP_AN, False, prec)
+
+    def timed(prec, k, w):
+        run(prec, w)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        run(prec, k)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    K, W = args.steps, args.warmup
+    T64 = timed(lib.PREC_F64, K, W)
+    mv64 = K / T64
+    log('fp64 %.1f matvecs/s (%.3f ms)' % (mv64, 1e3 * T64 / K))
+    if args.profile_only:
+        print(json.dumps({'metric': 'matvecs_per_s_fp64', 'value': mv64, 'ms_per_step': 1e3 * T64 / K}))
+        return
+    T32 = timed(lib.PREC_F32, K, W)
+    mv32 = K / T32
+    log('fp32 %.1f matvecs/s (%.3f ms)' % (mv32, 1e3 * T32 / K))
+
+    # per-kernel durations from HIP events on the launch stream
+    Px, Py, Pz = op.P
+    kern = {}
+    for prec, s, name in ((lib.PREC_F64, 16, 'fp64'), (lib.PREC_F32, 8, 'fp32')):
+        op.enable_timing(True)
+        run(prec, 20)
+        ms, calls = op.timing()
+        op.enable_timing(False)
+        kb = kernel_bytes(L, M, N, Px, Py, Pz, nidx, s)
+        kern[name] = {k: {'ms': m, 'GBps': kb[k] / (m * 1e-3) / 1e9, 'bytes': kb[k]}
+                      for k, m in zip(kb.keys(), ms)}
+    dom = max(kern['fp64'], key=lambda k: kern['fp64'][k]['ms'])
+    d = kern['fp64'][dom]
+    n = L * M * N
+    nS = nidx // 3
+    bmv = 16 * (8 * nS + 48 * n)
+    roof = {'bound': 'hbm', 'kernel': dom, 'achieved': d['GBps'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+            'frac': d['GBps'] / HBM_PEAK_GBS, 'traffic': None,
+            'matvec_algorithmic_bytes': bmv, 'matvec_achieved_GBps': bmv * mv64 / 1e9,
+            'matvec_frac': bmv * mv64 / 1e9 / HBM_PEAK_GBS}
+
+    out = {}
+    if not args.no_solve:
+        b = torch.from_numpy(np.ascontiguousarray(Vrhs)).to(dev)
+        res = {}
+        for mode, name in ((0, 'fp64'), (1, 'fp32_refined')):
+            xs = torch.zeros_like(b)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            flag, relres, it, rv = op.gmres_device(b.data_ptr(), xs.data_ptr(), restart=50, tol=1e-6, maxit=200,
+                                                   mode=mode, inner_tol=1e-4)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            res[name] = (xs, flag, relres, it, rv, dt)
+            log('%s solve %.3f s flag %d relres %.3g iters %d' % (name, dt, flag, relres, len(rv) - 1))
+        x64s = res['fp64'][0]
+        err = (torch.linalg.norm(res['fp32_refined'][0] - x64s) / torch.linalg.norm(x64s)).item()
+        out = {
+            'time_to_1e-6_fp64_s': res['fp64'][5],
+            'time_to_1e-6_fp32_refined_s': res['fp32_refined'][5],
+            'speedup_fp32_refined': res['fp64'][5] / res['fp32_refined'][5],
+            'fp32_refined_rel_error_vs_fp64': err,
+            'relres_fp64': res['fp64'][2], 'relres_fp32_refined': res['fp32_refined'][2],
+            'flag_fp64': res['fp64'][1], 'flag_fp32_refined': res['fp32_refined'][1],
+            'gmres_iters_fp64': len(res['fp64'][4]) - 1,
+            'gmres_iters_fp32_refined': len(res['fp32_refined'][4]) - 1,
+        }
+
+    cpu = None
+    if not args.no_cpu:
+        import scipy.fft  # noqa: F401
+        from oracle import mvp as omvp   # CPU baseline leg only
+        cores = os.cpu_count() or 1
+        cores = min(cores, 16)
+        fN = op.get_fN()
+        xh = x64.cpu().numpy()
+        idx0 = idx1 - 1
+        omvp.mv_AN(xh, fN, Mr, Mc, Gram, 'notransp', idx0, workers=cores)   # warm-up
+        reps, t = 0, time.perf_counter()
+        while True:
+            omvp.mv_AN(xh, fN, Mr, Mc, Gram, 'notransp', idx0, workers=cores)
+            reps += 1
+            el = time.perf_counter() - t
+            if el > 10.0 or reps >= 20:
+                break
+        cpu = {'value': reps / el, 'unit': 'matvecs/s', 'cores': cores, 'kind': 'port',
+               'sample': '%d FP64 matvecs of the same %d^3 operator (oracle.mvp.mv_AN: numpy + scipy.fft, '
+                         'workers=%d), %.1f s' % (reps, n1, cores, el)}
+        log('cpu %.3f matvecs/s' % cpu['value'])
+
+    line = {
+        'metric': 'VIE matvecs/s + time-to-1e-6 residual at 128^3 voxels, FP64 vs FP32-refined',
+        'value': mv64, 'unit': 'matvecs/s', 'n_gpus': 1, 'steps': K, 'warmup': W,
+        'ms_per_step': 1e3 * T64 / K, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'complex128', 'data': 'synthetic',
+        'config': {'workload': 'jvie_matvec_head_phantom_%dcubed_1p5mm_298MHz' % n1 if n1 == 128 else
+                   'jvie_matvec_head_phantom_%dcubed' % n1,
+                   'grid': [L, M, N], 'circulant': [Px, Py, Pz], 'dx_m': dx, 'freq_Hz': freq,
+                   'phantom_scale': scale, 'phantom': ph, 'non_air_dof': nidx, 'form': 2},
+        'fp32_matvecs_per_s': mv32, 'fp32_ms_per_step': 1e3 * T32 / K,
+        'assembly_s': t_asm,
+        'kernels': kern,
+        'roofline': roof,
+        'cpu_baseline': cpu,
+    }
+    line.update(out)
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,136 @@
+/*
+ * marie_vie.h -- C-ABI of the MI355X (gfx950) FFT-VIE library.
+ *
+ * Drop-in boundary for MARIE's JVIE hot path. The MATLAB-facing functions
+ * keep their signatures (a MEX gateway forwards to these entry points; see
+ * the mex/ sources of the package); every entry point below names the
+ * reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 = OK, otherwise one of
+ *    MARIE_ERR_*; marie_vie_last_error() gives a message (thread-local).
+ *  - Complex arrays are interleaved (re, im) doubles, column-major exactly
+ *    as MATLAB stores them: a voxel field is L x M x N (x fastest), a
+ *    3-vector field L x M x N x 3, fN is Px x Py x Pz x 6.
+ *  - "Host" entry points take caller-owned host buffers that are never
+ *    retained; "_device" entry points take caller-owned device pointers
+ *    (complex128 or complex64 per `prec`) on the handle's device.
+ *  - A handle is reentrant across handles, not across concurrent calls on
+ *    the same handle. All work runs on the handle's stream; host entry
+ *    points synchronise before returning, device entry points do not.
+ */
+#ifndef MARIE_VIE_H
+#define MARIE_VIE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MARIE_OK 0
+#define MARIE_ERR_ARG 1
+#define MARIE_ERR_ALLOC 2
+#define MARIE_ERR_HIP 3
+#define MARIE_ERR_STATE 4
+#define MARIE_ERR_SIZE 5
+#define MARIE_ERR_NONFINITE 6
+
+#define MARIE_PREC_F64 1 /* complex128 */
+#define MARIE_PREC_F32 2 /* complex64  */
+
+#define MARIE_OP_AN 0 /* Gram*Mr*x - Mc*N x   (mv_AN.m)  */
+#define MARIE_OP_N 1  /* N x                  (mv_N.m)   */
+#define MARIE_OP_L 2  /* N x - Gram*x         (mv_L.m)   */
+
+typedef struct marie_vie_op marie_vie_op;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int marie_vie_version(void);
+
+/* Message of the last failing call on this thread. */
+const char* marie_vie_last_error(void);
+
+/* Create an operator for an L x M x N voxel grid whose circulant is
+ * Px x Py x Pz (Px >= L, ...; getOPERATORS returns 2L x 2M x 2N,
+ * src_vie/src_operators/getOPERATORS.m:82-90, circulant_nop.m:5).
+ * `precisions` is a bitmask of MARIE_PREC_*. Replaces the device set-up
+ * of JVIE_Solver.m:253-278 (gpuDevice, gpuArray upload). */
+int marie_vie_create(marie_vie_op** h, int L, int M, int N, int Px, int Py, int Pz,
+                     int precisions, int device);
+
+/* Free every device buffer of the handle (JVIE_Solver.m:509-511 reset(dev)). */
+int marie_vie_destroy(marie_vie_op* h);
+
+/* Run the handle's work on an external stream (hipStream_t, NULL = own). */
+int marie_vie_set_stream(marie_vie_op* h, void* stream);
+
+/* Upload fN (Px x Py x Pz x 6 complex, host), the FFT of the circulant
+ * N operator as returned by getOPERATORS(r,f,'N') (getOPERATORS.m:88-90).
+ * The F32 copy, if enabled, is rounded from it. */
+int marie_vie_set_fN(marie_vie_op* h, const double* fN);
+
+/* Same from a device buffer of complex128. */
+int marie_vie_set_fN_device(marie_vie_op* h, const void* fN_dev);
+
+/* Download fN (complex128) to host. */
+int marie_vie_get_fN(marie_vie_op* h, double* fN);
+
+/* Assemble fN on the device for voxel size dx and wavenumber ko
+ * (getOPERATORS(r,f,'N'), getOPERATORS.m:47-48, :74-96; assembly_nop.m,
+ * circulant_nop.m, fft_operator.m). The circulant must be 2L x 2M x 2N.
+ * `singular` = 10 doubles (re,im of I_ST, I_EAc, I_EAo, I_VAc, I_VAo as
+ * returned by singular_st/ea/va.m) or NULL for the library's own
+ * machine-precision self-terms. */
+int marie_vie_assemble_N(marie_vie_op* h, double dx, double ko, const double* singular);
+
+/* The five self-term integrals the library uses (10 doubles out). */
+int marie_vie_singular_terms(double dx, double ko, double* out10);
+
+/* Material: Mr, Mc (L x M x N complex, host) and the Gram scalar dx^3
+ * (mv_AN.m:1 arguments Mr, Mc, Gram; JVIE_Solver.m:135-140, :242-244). */
+int marie_vie_set_material(marie_vie_op* h, const double* Mr, const double* Mc, double Gram);
+
+/* Compact DOF index into the L x M x N x 3 grid (idxS3, JVIE_Solver.m:147-148).
+ * `one_based` = 1 for MATLAB indices. Duplicates are rejected. */
+int marie_vie_set_index(marie_vie_op* h, const int64_t* idx, int64_t nidx, int one_based);
+
+/* y = op(x) for `nrhs` stacked compact vectors of nidx complex each
+ * (mv_AN.m:1 with transp_flag 'notransp'/'transp', mv_N.m, mv_L.m).
+ * Host buffers, complex128. */
+int marie_vie_apply(marie_vie_op* h, int op, int transp, const double* x, double* y, int nrhs);
+
+/* Same on device buffers of precision `prec` (MARIE_PREC_F64 or _F32). */
+int marie_vie_apply_device(marie_vie_op* h, int op, int transp, int prec, const void* x_dev,
+                           void* y_dev, int nrhs);
+
+/* Restarted GMRES(restart) on A_N with MARIE's pgmres semantics
+ * (src_numeric/src_iterative/pgmres.m:70-181, pgmres_iter.m:22-150):
+ * stop when the least-squares residual < tol*||b||; flag/relres from the
+ * true residual ||b - A x||/||b||; resvec relative to ||b|| (capacity
+ * `resvec_cap`, count in *nres); iter = {inner, outer}. Device buffers
+ * b_dev, x_dev (complex128, nidx each; x_dev holds x0 on entry).
+ * `mode`: 0 = plain FP64; 1 = complex64 matvecs inside FP64 GMRES with
+ * iterative refinement on the FP64 residual (inner tolerance `inner_tol`). */
+int marie_vie_gmres_device(marie_vie_op* h, const void* b_dev, void* x_dev, int restart, double tol,
+                           int maxit, int mode, double inner_tol, int* flag, double* relres,
+                           int* iter, double* resvec, int resvec_cap, int* nres);
+
+/* Same with host buffers b, x (complex128, nidx each; x holds x0 on entry
+ * and the solution on return) -- the form a MEX gateway of pgmres /
+ * JVIE_Solver.m:363-367 forwards to. */
+int marie_vie_gmres(marie_vie_op* h, const double* b, double* x, int restart, double tol, int maxit,
+                    int mode, double inner_tol, int* flag, double* relres, int* iter, double* resvec,
+                    int resvec_cap, int* nres);
+
+/* Per-kernel timing with HIP events on the handle's stream. When enabled,
+ * every apply records an event between its five kernels and accumulates
+ * the durations; marie_vie_get_timing returns the mean ms per kernel
+ * (5 values: fwd_x, fwd_y, z_product, inv_y, inv_x_epilogue) and the count. */
+int marie_vie_enable_timing(marie_vie_op* h, int on);
+int marie_vie_get_timing(marie_vie_op* h, double* mean_ms5, int64_t* calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MARIE_VIE_H */

@@ -1,0 +1,80 @@
+// Operator handle shared by the matvec, assembly and Krylov translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "vie_fft.hpp"
+
+namespace marie {
+
+enum Status {
+    OK = 0,
+    ERR_ARG = 1,
+    ERR_ALLOC = 2,
+    ERR_HIP = 3,
+    ERR_STATE = 4,
+    ERR_SIZE = 5,
+    ERR_NONFINITE = 6,
+};
+
+enum Precision { PREC_F64 = 1, PREC_F32 = 2 };
+enum OpKind { OP_AN = 0, OP_N = 1, OP_L = 2 };
+
+void set_error(const std::string& msg);
+const char* get_error();
+
+#define MARIE_HIP(call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::marie::set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+            return ::marie::ERR_HIP;                                                       \
+        }                                                                                  \
+    } while (0)
+
+struct Op {
+    int L = 0, M = 0, N = 0;        // voxel grid
+    int Px = 0, Py = 0, Pz = 0;     // circulant (FFT) grid
+    int precisions = 0;             // bitmask of Precision
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    AxisPlan plan[3];
+
+    // device data
+    void* fN64 = nullptr;           // c2<double>[6][Pz][Py][Px]
+    void* fN32 = nullptr;           // c2<float>
+    void* tw64 = nullptr;           // 3 axis tables, offsets 0, Px, Px+Py
+    void* tw32 = nullptr;
+    void* Mr64 = nullptr;           // c2<double>[N][M][L]
+    void* Mc64 = nullptr;
+    void* Mr32 = nullptr;
+    void* Mc32 = nullptr;
+    double Gram = 0.0;
+    bool have_fN = false, have_mat = false, have_idx = false;
+
+    int* pos = nullptr;             // [3 L M N] -> compact position or -1
+    int64_t* idx = nullptr;         // [nidx] 0-based grid index of each compact entry
+    int64_t nidx = 0;
+
+    // work buffers (bytes), sized for `nrhs_cap` right-hand sides at the widest precision
+    void* A = nullptr;              // [3 nrhs][N][M][Px]
+    void* B = nullptr;              // [3 nrhs][N][Py][Px]
+    int nrhs_cap = 0;
+
+    // optional per-kernel timing with HIP events on the handle's stream
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    double kernel_ms[8] = {};
+    int64_t kernel_calls = 0;
+
+    int64_t nvox() const { return (int64_t)L * M * N; }
+};
+
+int ensure_work(Op* h, int nrhs);
+int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s);
+int build_plan(int P, AxisPlan* pl);
+int upload_twiddles(Op* h);
+
+}  // namespace marie

@@ -1,0 +1,531 @@
+// Device-resident restarted GMRES with MARIE's pgmres semantics, plus the
+// mixed-precision variant (complex64 matvecs, FP64 Arnoldi, FP64 iterative
+// refinement on the true residual).
+//
+// Semantics followed (src_numeric/src_iterative/pgmres.m:70-181,
+// pgmres_iter.m:22-150): one-pass classical Gram-Schmidt h = V^H w,
+// w -= V h; convergence test on the least-squares residual against
+// tol*||b||; at the end of a cycle x += V y and the implicit residual
+// r = V (rhs - H y); restart on resvec; flag/relres from the true residual.
+// The least-squares problem is updated with Givens rotations (equal to the
+// reference's from-scratch solve in exact arithmetic); the small dense
+// algebra runs on the host in FP64, the O(n) vector work on the device with
+// deterministic two-stage reductions (no atomics).
+#include "../../include/marie_vie.h"
+#include "vie_handle.hpp"
+
+#include <cmath>
+#include <complex>
+#include <vector>
+
+namespace marie {
+
+using cd = std::complex<double>;
+constexpr int KT = 256;      // threads per block
+constexpr int KG = 1024;     // max blocks of a reduction
+constexpr int KV = 4;        // basis vectors per pass of the multi-dot
+
+__device__ __forceinline__ double2 ld2(const c2<double>* p) { return *reinterpret_cast<const double2*>(p); }
+
+template <int W>
+__device__ __forceinline__ void block_reduce(double (&v)[W], double* sh) {
+    // sh: KT/64 * W doubles; wave64 shuffle then across waves
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        double s = v[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        v[q] = s;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < W; ++q) sh[wid * W + q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            double s = 0.0;
+            for (int w = 0; w < KT / 64; ++w) s += sh[w * W + q];
+            v[q] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// partial[(i) * G + blk] = sum over the block's range of conj(V_i) * w, i in [i0, i0+nv)
+__global__ __launch_bounds__(KT) void k_mdot(const c2<double>* __restrict__ V, int64_t ldv, int i0, int nv,
+                                             const c2<double>* __restrict__ w, int64_t n,
+                                             c2<double>* __restrict__ partial, int G) {
+    __shared__ double sh[KT / 64 * 2 * KV];
+    double acc[2 * KV];
+#pragma unroll
+    for (int q = 0; q < 2 * KV; ++q) acc[q] = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)G * KT) {
+        const double2 wv = ld2(w + e);
+#pragma unroll
+        for (int q = 0; q < KV; ++q) {
+            if (q < nv) {
+                const double2 a = ld2(V + (i0 + q) * ldv + e);
+                acc[2 * q] += a.x * wv.x + a.y * wv.y;
+                acc[2 * q + 1] += a.x * wv.y - a.y * wv.x;
+            }
+        }
+    }
+    block_reduce<2 * KV>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < nv; ++q) partial[(int64_t)(i0 + q) * G + blockIdx.x] = c2<double>{acc[2 * q], acc[2 * q + 1]};
+}
+
+// out[i] = sum_b partial[i*G + b], fixed order; one block per i
+__global__ __launch_bounds__(KT) void k_reduce(const c2<double>* __restrict__ partial, int G,
+                                               c2<double>* __restrict__ out) {
+    __shared__ double sh[KT / 64 * 2];
+    double acc[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < G; b += KT) {
+        const c2<double> p = partial[(int64_t)blockIdx.x * G + b];
+        acc[0] += p.x;
+        acc[1] += p.y;
+    }
+    block_reduce<2>(acc, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = c2<double>{acc[0], acc[1]};
+}
+
+// y = beta*y + sum_{i<k} V_i c_i ; optional partial ||y||^2 into npart[blk]
+__global__ __launch_bounds__(KT) void k_vcomb(c2<double>* __restrict__ y, double beta,
+                                              const c2<double>* __restrict__ V, int64_t ldv, int k,
+                                              const c2<double>* __restrict__ coef, double csign, int64_t n,
+                                              double* __restrict__ npart, int G) {
+    __shared__ c2<double> cs[80];
+    __shared__ double sh[KT / 64];
+    for (int i = threadIdx.x; i < k; i += KT) cs[i] = coef[i];
+    __syncthreads();
+    double nrm[1] = {0.0};
+    for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)G * KT) {
+        double2 acc = {0.0, 0.0};
+        if (beta != 0.0) {
+            const double2 yv = ld2(y + e);
+            acc.x = beta * yv.x;
+            acc.y = beta * yv.y;
+        }
+        for (int i = 0; i < k; ++i) {
+            const double2 a = ld2(V + i * ldv + e);
+            const c2<double> c = cs[i];
+            acc.x += csign * (a.x * c.x - a.y * c.y);
+            acc.y += csign * (a.x * c.y + a.y * c.x);
+        }
+        y[e] = c2<double>{acc.x, acc.y};
+        nrm[0] += acc.x * acc.x + acc.y * acc.y;
+    }
+    if (npart) {
+        block_reduce<1>(nrm, sh);
+        if (threadIdx.x == 0) npart[blockIdx.x] = nrm[0];
+    }
+}
+
+__global__ __launch_bounds__(KT) void k_norm2(const c2<double>* __restrict__ v, int64_t n,
+                                              double* __restrict__ npart, int G) {
+    __shared__ double sh[KT / 64];
+    double acc[1] = {0.0};
+    for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)G * KT) {
+        const double2 a = ld2(v + e);
+        acc[0] += a.x * a.x + a.y * a.y;
+    }
+    block_reduce<1>(acc, sh);
+    if (threadIdx.x == 0) npart[blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(KT) void k_reduce_real(const double* __restrict__ p, int G, double* __restrict__ out) {
+    __shared__ double sh[KT / 64];
+    double acc[1] = {0.0};
+    for (int b = threadIdx.x; b < G; b += KT) acc[0] += p[b];
+    block_reduce<1>(acc, sh);
+    if (threadIdx.x == 0) out[0] = acc[0];
+}
+
+// z = a*x + b*y (real scalars)
+__global__ void k_axpby(c2<double>* __restrict__ z, double a, const c2<double>* __restrict__ x, double b,
+                        const c2<double>* __restrict__ y, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double2 xv = ld2(x + e), yv = ld2(y + e);
+        z[e] = c2<double>{a * xv.x + b * yv.x, a * xv.y + b * yv.y};
+    }
+}
+
+__global__ void k_f64_to_f32(const c2<double>* __restrict__ s, c2<float>* __restrict__ d, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        d[e] = c2<float>{(float)s[e].x, (float)s[e].y};
+}
+
+__global__ void k_f32_to_f64(const c2<float>* __restrict__ s, c2<double>* __restrict__ d, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        d[e] = c2<double>{(double)s[e].x, (double)s[e].y};
+}
+
+// ------------------------------------------------------------------ host side
+struct Ws {
+    c2<double>* V = nullptr;     // (m+1) x n
+    c2<double>* w = nullptr;     // n
+    c2<double>* r = nullptr;     // n
+    c2<double>* t = nullptr;     // n
+    c2<float>* xs = nullptr;     // n (mixed precision staging)
+    c2<float>* ys = nullptr;
+    c2<double>* part = nullptr;  // (m+1) x KG
+    c2<double>* h = nullptr;     // m+1
+    double* npart = nullptr;     // KG
+    double* nout = nullptr;      // 1
+    void free_all() {
+        void* p[] = {V, w, r, t, xs, ys, part, h, npart, nout};
+        for (void* q : p)
+            if (q) (void)hipFree(q);
+    }
+};
+
+struct Ctx {
+    Op* h;
+    hipStream_t s;
+    int64_t n;
+    int G;
+    Ws ws;
+    int mode;
+};
+
+static int grid_for(int64_t n) {
+    int64_t g = (n + KT * 4 - 1) / (KT * 4);
+    if (g < 1) g = 1;
+    if (g > KG) g = KG;
+    return (int)g;
+}
+
+static int norm(Ctx& c, const c2<double>* v, double* out) {
+    hipLaunchKernelGGL(k_norm2, dim3(c.G), dim3(KT), 0, c.s, v, c.n, c.ws.npart, c.G);
+    hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, c.ws.nout);
+    MARIE_HIP(hipGetLastError());
+    double v2 = 0.0;
+    MARIE_HIP(hipMemcpyAsync(&v2, c.ws.nout, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    MARIE_HIP(hipStreamSynchronize(c.s));
+    *out = std::sqrt(v2);
+    return OK;
+}
+
+// y = A x in FP64 (mode 0) or through complex64 (mode 1)
+static int matvec(Ctx& c, const c2<double>* x, c2<double>* y, bool f32) {
+    if (!f32) return apply(c.h, OP_AN, 0, PREC_F64, x, y, 1, c.s);
+    const int g = grid_for(c.n);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(g), dim3(KT), 0, c.s, x, c.ws.xs, c.n);
+    int st = apply(c.h, OP_AN, 0, PREC_F32, c.ws.xs, c.ws.ys, 1, c.s);
+    if (st) return st;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(g), dim3(KT), 0, c.s, c.ws.ys, y, c.n);
+    MARIE_HIP(hipGetLastError());
+    return OK;
+}
+
+// One restart cycle of pgmres_iter on A z = r (r = current residual, device).
+// Updates x in place and r to the implicit residual. Returns iterations in *kout
+// and appends |LS residual| values to resv.
+static int gmres_cycle(Ctx& c, c2<double>* x, c2<double>* r, int m, double abstol, bool f32, int* kout,
+                       std::vector<double>& resv) {
+    const int64_t n = c.n;
+    double beta = 0.0;
+    int st = norm(c, r, &beta);
+    if (st) return st;
+    // V0 = r / beta
+    hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.V, 1.0 / beta, r, 0.0, r, n);
+    std::vector<cd> H((size_t)(m + 1) * m, cd(0, 0));   // column-major, original Hessenberg
+    std::vector<cd> R((size_t)(m + 1) * m, cd(0, 0));   // rotated
+    std::vector<double> cs(m);
+    std::vector<cd> sn(m);
+    std::vector<cd> g(m + 1, cd(0, 0));
+    g[0] = cd(beta, 0);
+    std::vector<cd> hcol(m + 1);
+    int k = 0;
+    for (k = 1; k <= m; ++k) {
+        c2<double>* vk = c.ws.V + (int64_t)(k - 1) * n;
+        st = matvec(c, vk, c.ws.w, f32);
+        if (st) return st;
+        // h = V(:,1:k)^H w  (CGS, pgmres_iter.m:106)
+        for (int i0 = 0; i0 < k; i0 += KV) {
+            const int nv = (k - i0) < KV ? (k - i0) : KV;
+            hipLaunchKernelGGL(k_mdot, dim3(c.G), dim3(KT), 0, c.s, c.ws.V, n, i0, nv, c.ws.w, n, c.ws.part, c.G);
+        }
+        hipLaunchKernelGGL(k_reduce, dim3(k), dim3(KT), 0, c.s, c.ws.part, c.G, c.ws.h);
+        // w = w - V h, with ||w||^2 partials (pgmres_iter.m:107, :109)
+        hipLaunchKernelGGL(k_vcomb, dim3(c.G), dim3(KT), 0, c.s, c.ws.w, 1.0, c.ws.V, n, k, c.ws.h, -1.0, n,
+                           c.ws.npart, c.G);
+        hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, c.ws.nout);
+        MARIE_HIP(hipGetLastError());
+        std::vector<c2<double>> hh(k);
+        double w2 = 0.0;
+        MARIE_HIP(hipMemcpyAsync(hh.data(), c.ws.h, k * sizeof(c2<double>), hipMemcpyDeviceToHost, c.s));
+        MARIE_HIP(hipMemcpyAsync(&w2, c.ws.nout, sizeof(double), hipMemcpyDeviceToHost, c.s));
+        MARIE_HIP(hipStreamSynchronize(c.s));
+        const double hk1 = std::sqrt(w2);
+        // V(:,k+1) = w / H(k+1,k)  (pgmres_iter.m:110)
+        hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.V + (int64_t)k * n, 1.0 / hk1, c.ws.w, 0.0,
+                           c.ws.w, n);
+        for (int i = 0; i < k; ++i) hcol[i] = cd(hh[i].x, hh[i].y);
+        hcol[k] = cd(hk1, 0);
+        for (int i = 0; i <= k; ++i) H[(size_t)(k - 1) * (m + 1) + i] = hcol[i];
+        // apply previous rotations, then a new one zeroing hcol[k]
+        for (int i = 0; i < k - 1; ++i) {
+            const cd a = hcol[i], b = hcol[i + 1];
+            hcol[i] = cs[i] * a + sn[i] * b;
+            hcol[i + 1] = -std::conj(sn[i]) * a + cs[i] * b;
+        }
+        {
+            const cd a = hcol[k - 1], b = hcol[k];
+            const double na = std::abs(a), nb = std::abs(b);
+            if (nb == 0.0) {
+                cs[k - 1] = 1.0;
+                sn[k - 1] = cd(0, 0);
+            } else if (na == 0.0) {
+                cs[k - 1] = 0.0;
+                sn[k - 1] = std::conj(b) / nb;
+            } else {
+                const double t = std::hypot(na, nb);
+                cs[k - 1] = na / t;
+                sn[k - 1] = (a / na) * std::conj(b) / t;
+            }
+            hcol[k - 1] = cs[k - 1] * a + sn[k - 1] * b;
+            hcol[k] = cd(0, 0);
+            const cd gk = g[k - 1];
+            g[k - 1] = cs[k - 1] * gk;
+            g[k] = -std::conj(sn[k - 1]) * gk;
+        }
+        for (int i = 0; i < k; ++i) R[(size_t)(k - 1) * (m + 1) + i] = hcol[i];
+        const double res = std::abs(g[k]);
+        resv.push_back(res);
+        if (res < abstol) break;
+        if (k == m) break;
+    }
+    if (k > m) k = m;
+    // y = R^{-1} g (upper triangular k x k)
+    std::vector<cd> y(k);
+    for (int i = k - 1; i >= 0; --i) {
+        cd s = g[i];
+        for (int j = i + 1; j < k; ++j) s -= R[(size_t)j * (m + 1) + i] * y[j];
+        y[i] = s / R[(size_t)i * (m + 1) + i];
+    }
+    // LS residual vector res = rhs - Hc y (pgmres_iter.m:119) -> r = V(:,1:k+1) res (:150)
+    std::vector<cd> resv_vec(k + 1, cd(0, 0));
+    resv_vec[0] = cd(beta, 0);
+    for (int j = 0; j < k; ++j)
+        for (int i = 0; i <= j + 1; ++i) resv_vec[i] -= H[(size_t)j * (m + 1) + i] * y[j];
+    std::vector<c2<double>> coef(k + 1);
+    for (int i = 0; i < k; ++i) coef[i] = c2<double>{y[i].real(), y[i].imag()};
+    MARIE_HIP(hipMemcpyAsync(c.ws.h, coef.data(), k * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
+    hipLaunchKernelGGL(k_vcomb, dim3(c.G), dim3(KT), 0, c.s, x, 1.0, c.ws.V, n, k, c.ws.h, 1.0, n,
+                       (double*)nullptr, c.G);
+    MARIE_HIP(hipStreamSynchronize(c.s));
+    for (int i = 0; i <= k; ++i) coef[i] = c2<double>{resv_vec[i].real(), resv_vec[i].imag()};
+    MARIE_HIP(hipMemcpyAsync(c.ws.h, coef.data(), (k + 1) * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
+    hipLaunchKernelGGL(k_vcomb, dim3(c.G), dim3(KT), 0, c.s, r, 0.0, c.ws.V, n, k + 1, c.ws.h, 1.0, n,
+                       (double*)nullptr, c.G);
+    MARIE_HIP(hipGetLastError());
+    MARIE_HIP(hipStreamSynchronize(c.s));
+    *kout = k;
+    return OK;
+}
+
+static int alloc_ws(Ctx& c, int m, bool f32) {
+    const int64_t n = c.n;
+    auto A = [&](void** p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            set_error("out of device memory for the Krylov workspace");
+            return ERR_ALLOC;
+        }
+        return OK;
+    };
+    int st = A((void**)&c.ws.V, (size_t)(m + 1) * n * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.w, n * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.r, n * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.t, n * sizeof(c2<double>));
+    if (!st && f32) st = A((void**)&c.ws.xs, n * sizeof(c2<float>));
+    if (!st && f32) st = A((void**)&c.ws.ys, n * sizeof(c2<float>));
+    if (!st) st = A((void**)&c.ws.part, (size_t)(m + 1) * KG * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.h, (m + 2) * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.npart, KG * sizeof(double));
+    if (!st) st = A((void**)&c.ws.nout, sizeof(double));
+    return st;
+}
+
+int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxit, int mode, double inner_tol,
+          int* flag, double* relres, int* iter, std::vector<double>& resvec) {
+    Ctx c;
+    c.h = h;
+    c.s = h->stream;
+    c.n = h->nidx;
+    c.G = grid_for(c.n);
+    c.mode = mode;
+    const bool mixed = mode == 1;
+    int st = alloc_ws(c, m, mixed);
+    if (st) { c.ws.free_all(); return st; }
+    auto done = [&](int code) { (void)hipStreamSynchronize(c.s); c.ws.free_all(); return code; };
+    double bnorm = 0.0;
+    st = norm(c, b, &bnorm);
+    if (st) return done(st);
+    if (bnorm == 0.0) {
+        // zero right-hand side: x = 0 (jvie2solver.m:30-33 fast return)
+        MARIE_HIP(hipMemsetAsync(x, 0, c.n * sizeof(c2<double>), c.s));
+        *flag = 0; *relres = 0.0; iter[0] = 0; iter[1] = 0;
+        resvec.assign(1, 0.0);
+        return done(OK);
+    }
+    // r = b - A x0 (x0 may be zero)
+    double xn = 0.0;
+    st = norm(c, x, &xn);
+    if (st) return done(st);
+    if (xn > 0.0) {
+        st = matvec(c, x, c.ws.t, false);
+        if (st) return done(st);
+        hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.r, 1.0, b, -1.0, c.ws.t, c.n);
+    } else {
+        MARIE_HIP(hipMemcpyAsync(c.ws.r, b, c.n * sizeof(c2<double>), hipMemcpyDeviceToDevice, c.s));
+    }
+    double rn = 0.0;
+    st = norm(c, c.ws.r, &rn);
+    if (st) return done(st);
+    resvec.assign(1, rn / bnorm);
+    int outit = 0, p = 0;
+    if (!mixed) {
+        // pgmres.m:153-167
+        while (resvec.back() > tol && outit < maxit) {
+            ++outit;
+            std::vector<double> rv;
+            st = gmres_cycle(c, x, c.ws.r, m, tol * bnorm, false, &p, rv);
+            if (st) return done(st);
+            for (double v : rv) resvec.push_back(v / bnorm);
+        }
+    } else {
+        // iterative refinement: true FP64 residual, complex64 correction solves
+        while (outit < maxit) {
+            ++outit;
+            // correction solve A d = r to relative inner tolerance, d accumulated in t
+            double rcur = 0.0;
+            st = norm(c, c.ws.r, &rcur);
+            if (st) return done(st);
+            double rel = inner_tol;
+            const double need = 0.5 * tol * bnorm / rcur;   // just enough to reach tol
+            if (need > rel) rel = need;
+            if (rel > 0.5) rel = 0.5;
+            MARIE_HIP(hipMemsetAsync(c.ws.t, 0, c.n * sizeof(c2<double>), c.s));
+            MARIE_HIP(hipMemcpyAsync(c.ws.w, c.ws.r, c.n * sizeof(c2<double>), hipMemcpyDeviceToDevice, c.s));
+            std::vector<double> rv;
+            // the inner cycles restart on their implicit residual, kept in r
+            double inner_res = rcur;
+            int cycles = 0;
+            while (inner_res > rel * rcur && cycles < maxit) {
+                ++cycles;
+                std::vector<double> rvc;
+                st = gmres_cycle(c, c.ws.t, c.ws.r, m, rel * rcur, true, &p, rvc);
+                if (st) return done(st);
+                for (double v : rvc) rv.push_back(v);
+                inner_res = rvc.empty() ? 0.0 : rvc.back();
+            }
+            // x += d ; r = b - A64 x
+            hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, x, 1.0, x, 1.0, c.ws.t, c.n);
+            st = matvec(c, x, c.ws.t, false);
+            if (st) return done(st);
+            hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.r, 1.0, b, -1.0, c.ws.t, c.n);
+            MARIE_HIP(hipGetLastError());
+            double rn2 = 0.0;
+            st = norm(c, c.ws.r, &rn2);
+            if (st) return done(st);
+            // resvec: the inner LS residuals scaled to ||b||, then the refined true residual
+            for (double v : rv) resvec.push_back(v / bnorm);
+            resvec.push_back(rn2 / bnorm);
+            if (rn2 / bnorm <= tol) break;
+        }
+    }
+    iter[0] = p;
+    iter[1] = outit;
+    // true residual (pgmres.m:175)
+    st = matvec(c, x, c.ws.t, false);
+    if (st) return done(st);
+    hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.t, 1.0, b, -1.0, c.ws.t, c.n);
+    double tn = 0.0;
+    st = norm(c, c.ws.t, &tn);
+    if (st) return done(st);
+    *relres = tn / bnorm;
+    *flag = (*relres <= tol) ? 0 : 1;
+    return done(OK);
+}
+
+}  // namespace marie
+
+using namespace marie;
+
+extern "C" int marie_vie_gmres_device(marie_vie_op* hh, const void* b, void* x, int restart, double tol, int maxit,
+                                      int mode, double inner_tol, int* flag, double* relres, int* iter,
+                                      double* resvec, int resvec_cap, int* nres) {
+    Op* h = reinterpret_cast<Op*>(hh);
+    if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
+    MARIE_HIP(hipSetDevice(h->device));
+    if (!b || !x || !flag || !relres || !iter || restart < 1 || restart > 64 || maxit < 1 || !(tol > 0) ||
+        (mode != 0 && mode != 1)) {
+        set_error("invalid gmres arguments (restart must be 1..64)");
+        return MARIE_ERR_ARG;
+    }
+    if (!(h->precisions & PREC_F64) || (mode == 1 && !(h->precisions & PREC_F32))) {
+        set_error("gmres needs an FP64 operator (and FP32 for mode 1)");
+        return MARIE_ERR_STATE;
+    }
+    if (!h->have_fN || !h->have_mat || !h->have_idx) {
+        set_error("operator, index or material not set");
+        return MARIE_ERR_STATE;
+    }
+    if (h->nidx == 0) {
+        *flag = 0; *relres = 0.0; iter[0] = iter[1] = 0;
+        if (nres) *nres = 0;
+        return MARIE_OK;
+    }
+    std::vector<double> rv;
+    int st = gmres(h, (const c2<double>*)b, (c2<double>*)x, restart, tol, maxit, mode, inner_tol, flag, relres,
+                   iter, rv);
+    if (st) return st;
+    if (resvec && resvec_cap > 0) {
+        const int nc = (int)rv.size() < resvec_cap ? (int)rv.size() : resvec_cap;
+        for (int i = 0; i < nc; ++i) resvec[i] = rv[i];
+    }
+    if (nres) *nres = (int)rv.size();
+    return MARIE_OK;
+}
+
+extern "C" int marie_vie_gmres(marie_vie_op* hh, const double* b, double* x, int restart, double tol, int maxit,
+                               int mode, double inner_tol, int* flag, double* relres, int* iter, double* resvec,
+                               int resvec_cap, int* nres) {
+    Op* h = reinterpret_cast<Op*>(hh);
+    if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
+    MARIE_HIP(hipSetDevice(h->device));
+    if (!b || !x) { set_error("null vector"); return MARIE_ERR_ARG; }
+    const size_t n = (size_t)h->nidx;
+    for (size_t i = 0; i < 2 * n; ++i)
+        if (!std::isfinite(b[i]) || !std::isfinite(x[i])) {
+            set_error("right-hand side or initial guess has non-finite entries");
+            return MARIE_ERR_NONFINITE;
+        }
+    void *db = nullptr, *dxp = nullptr;
+    if (n) {
+        MARIE_HIP(hipMalloc(&db, n * sizeof(c2<double>)));
+        if (hipMalloc(&dxp, n * sizeof(c2<double>)) != hipSuccess) {
+            (void)hipFree(db);
+            set_error("out of device memory");
+            return MARIE_ERR_ALLOC;
+        }
+        MARIE_HIP(hipMemcpy(db, b, n * sizeof(c2<double>), hipMemcpyHostToDevice));
+        MARIE_HIP(hipMemcpy(dxp, x, n * sizeof(c2<double>), hipMemcpyHostToDevice));
+    }
+    int st = marie_vie_gmres_device(hh, db, dxp, restart, tol, maxit, mode, inner_tol, flag, relres, iter, resvec,
+                                    resvec_cap, nres);
+    if (st == MARIE_OK && n) {
+        if (hipMemcpy(x, dxp, n * sizeof(c2<double>), hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("D2H copy failed");
+            st = MARIE_ERR_HIP;
+        }
+    }
+    if (db) (void)hipFree(db);
+    if (dxp) (void)hipFree(dxp);
+    return st;
+}

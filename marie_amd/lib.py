@@ -1,0 +1,221 @@
+"""ctypes binding of libmarie_vie.so (the C-ABI declared in include/marie_vie.h).
+
+The library is the only compute path: loading fails loudly when the shared
+object is missing or no GPU is present; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libmarie_vie.so')
+
+PREC_F64 = 1
+PREC_F32 = 2
+OP_AN = 0
+OP_N = 1
+OP_L = 2
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_I64 = ctypes.c_int64
+_DP = ctypes.POINTER(ctypes.c_double)
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ('marie_vie_version', _I, []),
+    ('marie_vie_last_error', ctypes.c_char_p, []),
+    ('marie_vie_create', _I, [ctypes.POINTER(_P), _I, _I, _I, _I, _I, _I, _I, _I]),
+    ('marie_vie_destroy', _I, [_P]),
+    ('marie_vie_set_stream', _I, [_P, _P]),
+    ('marie_vie_set_fN', _I, [_P, _P]),
+    ('marie_vie_set_fN_device', _I, [_P, _P]),
+    ('marie_vie_get_fN', _I, [_P, _P]),
+    ('marie_vie_assemble_N', _I, [_P, _D, _D, _P]),
+    ('marie_vie_singular_terms', _I, [_D, _D, _P]),
+    ('marie_vie_set_material', _I, [_P, _P, _P, _D]),
+    ('marie_vie_set_index', _I, [_P, _P, _I64, _I]),
+    ('marie_vie_apply', _I, [_P, _I, _I, _P, _P, _I]),
+    ('marie_vie_apply_device', _I, [_P, _I, _I, _I, _P, _P, _I]),
+    ('marie_vie_gmres_device', _I, [_P, _P, _P, _I, _D, _I, _I, _D, ctypes.POINTER(_I), _DP,
+                                    ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_gmres', _I, [_P, _P, _P, _I, _D, _I, _I, _D, ctypes.POINTER(_I), _DP,
+                             ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_enable_timing', _I, [_P, _I]),
+    ('marie_vie_get_timing', _I, [_P, _DP, ctypes.POINTER(_I64)]),
+]
+
+_lib = None
+
+
+class MarieError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load the shared library (no GPU call is made)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MarieError('libmarie_vie.so is not built: run `python -m marie_amd.build` '
+                         '(or __graft_entry__.build())')
+    lib = ctypes.CDLL(path)
+    for name, res, args in SIGNATURES:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != 0:
+        msg = _lib.marie_vie_last_error().decode(errors='replace')
+        raise MarieError('marie_vie error %d: %s' % (status, msg))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _cplx(a):
+    """C-contiguous complex128 view in MATLAB column-major element order."""
+    a = np.asarray(a, dtype=np.complex128)
+    return np.ascontiguousarray(a.ravel(order='F'))
+
+
+class Operator:
+    """A device-resident JVIE operator: fN, material, DOF index and work buffers.
+
+    Mirrors the state that MARIE keeps in the closures built by
+    JVIE_Solver.m:271-278 (fN_gpu, Mr_gpu, Mc_gpu, idxS3).
+    """
+
+    def __init__(self, L, M, N, P=None, precisions=PREC_F64, device=0):
+        lib = load()
+        P = P or (2 * L, 2 * M, 2 * N)
+        self.L, self.M, self.N = L, M, N
+        self.P = tuple(P)
+        self.precisions = precisions
+        h = ctypes.c_void_p()
+        check(lib.marie_vie_create(ctypes.byref(h), L, M, N, P[0], P[1], P[2], precisions, device))
+        self._h = h
+        self.nidx = 0
+
+    def close(self):
+        if getattr(self, '_h', None):
+            load().marie_vie_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr):
+        check(load().marie_vie_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
+
+    # -- operator ------------------------------------------------------------
+    def set_fN(self, fN):
+        Px, Py, Pz = self.P
+        fN = np.asarray(fN)
+        if fN.shape != (Px, Py, Pz, 6):
+            raise ValueError('fN must be %s x 6' % (self.P,))
+        buf = _cplx(fN)
+        check(load().marie_vie_set_fN(self._h, _ptr(buf)))
+
+    def set_fN_device(self, ptr):
+        check(load().marie_vie_set_fN_device(self._h, ctypes.c_void_p(ptr)))
+
+    def get_fN(self):
+        Px, Py, Pz = self.P
+        out = np.empty(Px * Py * Pz * 6, np.complex128)
+        check(load().marie_vie_get_fN(self._h, _ptr(out)))
+        return out.reshape(Px, Py, Pz, 6, order='F')
+
+    def assemble_N(self, dx, ko, singular=None):
+        if singular is not None:
+            s = np.ascontiguousarray(np.asarray(singular, np.complex128))
+            if s.shape != (5,):
+                raise ValueError('singular must hold 5 complex values')
+            check(load().marie_vie_assemble_N(self._h, float(dx), float(ko), _ptr(s)))
+        else:
+            check(load().marie_vie_assemble_N(self._h, float(dx), float(ko), None))
+
+    # -- material / index -----------------------------------------------------
+    def set_material(self, Mr, Mc, Gram):
+        shp = (self.L, self.M, self.N)
+        Mr = np.broadcast_to(np.asarray(Mr, np.complex128), shp)
+        Mc = np.broadcast_to(np.asarray(Mc, np.complex128), shp)
+        a, b = _cplx(Mr), _cplx(Mc)
+        check(load().marie_vie_set_material(self._h, _ptr(a), _ptr(b), float(Gram)))
+
+    def set_index(self, idx, one_based=False):
+        idx = np.ascontiguousarray(np.asarray(idx, np.int64).ravel())
+        check(load().marie_vie_set_index(self._h, _ptr(idx), int(idx.size), 1 if one_based else 0))
+        self.nidx = int(idx.size)
+
+    # -- products ---------------------------------------------------------------
+    def apply(self, x, op=OP_AN, transp=False):
+        """Host-to-host product (complex128); x is nidx or nidx x nrhs (columns)."""
+        x = np.asarray(x, np.complex128)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        xb = np.ascontiguousarray(x.T if x.ndim == 2 else x)
+        y = np.empty_like(xb)
+        check(load().marie_vie_apply(self._h, op, 1 if transp else 0, _ptr(xb), _ptr(y), nrhs))
+        return y.T if x.ndim == 2 else y
+
+    def apply_device(self, x_ptr, y_ptr, op=OP_AN, transp=False, prec=PREC_F64, nrhs=1):
+        check(load().marie_vie_apply_device(self._h, op, 1 if transp else 0, prec, ctypes.c_void_p(x_ptr),
+                                            ctypes.c_void_p(y_ptr), nrhs))
+
+    def gmres_device(self, b_ptr, x_ptr, restart=50, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4,
+                     resvec_cap=20000):
+        flag = ctypes.c_int()
+        relres = ctypes.c_double()
+        it = (ctypes.c_int * 2)()
+        resvec = np.zeros(resvec_cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_gmres_device(self._h, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), restart, tol,
+                                            maxit, mode, inner_tol, ctypes.byref(flag), ctypes.byref(relres), it,
+                                            resvec.ctypes.data_as(_DP), resvec_cap, ctypes.byref(nres)))
+        return flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    def gmres(self, b, x0=None, restart=50, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4, resvec_cap=20000):
+        """Host-buffer GMRES; returns (x, flag, relres, [inner, outer], resvec)."""
+        b = np.ascontiguousarray(np.asarray(b, np.complex128))
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(np.asarray(x0, np.complex128)).copy()
+        flag = ctypes.c_int()
+        relres = ctypes.c_double()
+        it = (ctypes.c_int * 2)()
+        resvec = np.zeros(resvec_cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_gmres(self._h, _ptr(b), _ptr(x), restart, tol, maxit, mode, inner_tol,
+                                     ctypes.byref(flag), ctypes.byref(relres), it, resvec.ctypes.data_as(_DP),
+                                     resvec_cap, ctypes.byref(nres)))
+        return x, flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    # -- timing -------------------------------------------------------------------
+    def enable_timing(self, on=True):
+        check(load().marie_vie_enable_timing(self._h, 1 if on else 0))
+
+    def timing(self):
+        ms = (ctypes.c_double * 5)()
+        calls = ctypes.c_int64()
+        check(load().marie_vie_get_timing(self._h, ms, ctypes.byref(calls)))
+        return list(ms), calls.value
+
+
+def singular_terms(dx, ko):
+    """The library's five self-term integrals (I_ST, I_EAc, I_EAo, I_VAc, I_VAo)."""
+    out = np.zeros(5, np.complex128)
+    check(load().marie_vie_singular_terms(float(dx), float(ko), _ptr(out)))
+    return out

@@ -1,0 +1,194 @@
+"""MATLAB-signature mirror of MARIE's JVIE hot path on the HIP library.
+
+Same names, argument meaning and return values as the reference functions
+(indices are MATLAB's 1-based `idx`):
+
+  mv_AN(JIn0, fN, Mr, Mc, Gram, transp_flag, idx, GPU_flag)    src_vie/src_mvp/mv_AN.m:1
+  mv_N(JIn0, fN, r, idx, GPU_flag)                              src_vie/src_mvp/mv_N.m:1
+  mv_L(JIn0, fN, r, Gram, idx, GPU_flag)                        src_vie/src_mvp/mv_L.m:1
+  getOPERATORS(r, f, op, type, singular)                        src_vie/src_operators/getOPERATORS.m:1
+  PlaneWave_Excitation(r, k, omega_mu, Eo)                      .../PlaneWave_Excitation.m:1
+  JVIE_Solver(Einc, e_r, r, freq, fN, tol, form, solver, ...)   src_vie/src_solver/JVIE_Solver.m:1
+  EMconstants(freq)                                             src_vie/src_solver/EMconstants.m
+
+Every product runs in libmarie_vie.so on the GPU. `fN` may be the host array
+returned by getOPERATORS or an already-resident `lib.Operator`; device
+operators are cached per fN array so a solver loop does not re-upload it
+(the reference keeps fN_gpu in a closure, JVIE_Solver.m:271-276).
+GPU_flag = 0 is rejected: this package has no CPU path.
+"""
+import time
+
+import numpy as np
+
+from . import lib
+
+C0 = 299792458.0
+
+
+def EMconstants(freq):
+    """mu, co, eo, omega, lambda, ko, omega_mu, eta (EMconstants.m:13-20)."""
+    mu = 4 * np.pi * 1e-7
+    co = C0
+    eo = 1.0 / co ** 2 / mu
+    omega = 2 * np.pi * freq
+    lam = co / freq
+    return dict(mu=mu, co=co, eo=eo, omega=omega, lam=lam, ko=2 * np.pi / lam, omega_mu=omega * mu,
+                eta=3.767303134617706e+002)
+
+
+_cache = {}
+
+
+def _operator(fN, shape, precisions=lib.PREC_F64):
+    if isinstance(fN, lib.Operator):
+        return fN
+    key = (id(fN), shape, precisions)
+    ent = _cache.get(key)
+    if ent is not None and ent[0] is fN:
+        return ent[1]
+    L, M, N = shape
+    op = lib.Operator(L, M, N, fN.shape[:3], precisions=precisions)
+    op.set_fN(fN)
+    _cache.clear()
+    _cache[key] = (fN, op)
+    return op
+
+
+def _require_gpu(GPU_flag):
+    if not GPU_flag:
+        raise ValueError('GPU_flag = 0: this package computes on the GPU only')
+
+
+def mv_AN(JIn0, fN, Mr, Mc, Gram, transp_flag, idx, GPU_flag=1):
+    _require_gpu(GPU_flag)
+    Mr = np.asarray(Mr)
+    op = _operator(fN, Mr.shape)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx, one_based=True)
+    if transp_flag not in ('transp', 'notransp'):
+        raise ValueError(transp_flag)
+    return op.apply(np.asarray(JIn0).ravel(), lib.OP_AN, transp_flag == 'transp')
+
+
+def mv_N(JIn0, fN, r, idx, GPU_flag=1):
+    _require_gpu(GPU_flag)
+    shape = np.shape(r)[:3]
+    op = _operator(fN, shape)
+    op.set_index(idx, one_based=True)
+    return op.apply(np.asarray(JIn0).ravel(), lib.OP_N)
+
+
+def mv_L(JIn0, fN, r, Gram, idx, GPU_flag=1):
+    _require_gpu(GPU_flag)
+    shape = np.shape(r)[:3]
+    op = _operator(fN, shape)
+    L, M, N = shape
+    op.set_material(np.ones(shape), np.zeros(shape), Gram)
+    op.set_index(idx, one_based=True)
+    return op.apply(np.asarray(JIn0).ravel(), lib.OP_L)
+
+
+def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operator=False,
+                 precisions=lib.PREC_F64):
+    """fN (2L x 2M x 2N x 6, complex128) assembled on the device.
+
+    `singular` 'DIRECTFN' / 'DEMCEM' select the reference's self-term method by
+    name; both are served by the library's own machine-precision quadrature.
+    A length-5 complex vector supplies the five self-terms verbatim (e.g. the
+    values of the reference's mexdirectfn_* routines). With
+    return_operator=True the device-resident lib.Operator is returned instead
+    of the host array.
+    """
+    if op != 'N':
+        raise NotImplementedError("operator '%s' is not on this package's path" % op)
+    if type != 'FFTofC':
+        raise NotImplementedError("type '%s': only 'FFTofC' is produced on the device" % type)
+    r = np.asarray(r)
+    L, M, N = r.shape[:3]
+    dx = r[1, 0, 0, 0] - r[0, 0, 0, 0]
+    ko = 2 * np.pi / C0 * f                    # getOPERATORS.m:48
+    sv = None if isinstance(singular, str) else np.asarray(singular, np.complex128)
+    o = lib.Operator(L, M, N, precisions=precisions)
+    o.assemble_N(dx, ko, sv)
+    if return_operator:
+        return o
+    return o.get_fN()
+
+
+def PlaneWave_Excitation(r, k, omega_mu, Eo):
+    """Voxel-averaged plane wave (PlaneWave_Excitation.m:39-95)."""
+    r = np.asarray(r)
+    L, M, N, _ = r.shape
+    d = (r[1, 0, 0, 0] - r[0, 0, 0, 0], r[0, 1, 0, 1] - r[0, 0, 0, 1], r[0, 0, 1, 2] - r[0, 0, 0, 2])
+    k = np.asarray(k, float)
+    Eo = np.asarray(Eo, complex)
+    Ho = np.cross(k, Eo) / omega_mu
+    E = np.ones((L, M, N), complex)
+    for a in range(3):
+        if k[a] != 0:
+            E = E * (np.exp(-1j * k[a] * r[..., a]) * (np.exp(-1j * k[a] * d[a] / 2) - np.exp(1j * k[a] * d[a] / 2))
+                     / (-1j * k[a]))
+        else:
+            E = E * d[a]
+    vol = d[0] * d[1] * d[2]
+    Ef = np.stack([Eo[c] * E for c in range(3)], axis=-1) / vol
+    Hf = np.stack([Ho[c] * E for c in range(3)], axis=-1) / vol
+    return Ef, Hf
+
+
+def jvie_system(Einc, e_r, r, freq, form=2):
+    """Mr, Mc, Gram, idxS3 (1-based) and Vrhs of JVIE_Solver.m:131-156, :242-244."""
+    em = EMconstants(freq)
+    r = np.asarray(r)
+    dx = r[1, 0, 0, 0] - r[0, 0, 0, 0]
+    L, M, N = e_r.shape
+    nD = L * M * N
+    Mr = np.asarray(e_r, complex).copy()
+    Mc = Mr - 1.0
+    Gram = dx ** 3
+    idxS = np.nonzero(np.abs(Mc.ravel(order='F')) > 1e-12)[0]
+    idxS3 = np.concatenate([idxS, nD + idxS, 2 * nD + idxS])
+    tau = 1j * em['omega'] * em['eo'] * Mc
+    Vrhs = Gram * np.tile(tau.ravel(order='F'), 3)[idxS3] * np.asarray(Einc).ravel(order='F')[idxS3]
+    if form != 1:
+        Mrf = Mr.ravel(order='F')
+        Mc = Mc / Mr
+        Vrhs = Vrhs / np.tile(Mrf[idxS], 3)
+        Mr = Mr / Mr
+    return Mr, Mc, Gram, idxS3 + 1, Vrhs
+
+
+def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='G', precond=0, max_it=1000,
+                inner_it=50, outer_it=200, ritz=5, gpu_flag=1, logfile=None, precision='fp64',
+                inner_tol=1e-4):
+    """[Je, flag, relres, iter, resvec, solvert] of JVIE_Solver.m:1 on the device.
+
+    Implemented: solver 'G' (restarted GMRES, pgmres semantics) without
+    preconditioner, forms I and II. precision='fp32-refined' runs complex64
+    matvecs inside FP64 GMRES with FP64 iterative refinement.
+    """
+    _require_gpu(gpu_flag)
+    if tol < 1e-15 or tol >= 1:          # JVIE_Solver.m:78-80
+        tol = 1e-3
+    if solver != 'G':
+        raise NotImplementedError("solver '%s': the device path provides GMRES ('G')" % solver)
+    if precond in ('L', 'R'):
+        raise NotImplementedError('preconditioned GMRES is not on the device path')
+    Mr, Mc, Gram, idx, Vrhs = jvie_system(Einc, e_r, r, freq, form)
+    L, M, N = e_r.shape
+    prec = lib.PREC_F64 | (lib.PREC_F32 if precision == 'fp32-refined' else 0)
+    if isinstance(fN, lib.Operator):
+        op = fN
+    else:
+        op = lib.Operator(L, M, N, np.shape(fN)[:3], precisions=prec)
+        op.set_fN(fN)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx, one_based=True)
+    t0 = time.perf_counter()
+    x, flag, relres, it, resvec = op.gmres(Vrhs, None, inner_it, tol, outer_it,
+                                           mode=1 if precision == 'fp32-refined' else 0, inner_tol=inner_tol)
+    solvert = time.perf_counter() - t0
+    Je = np.zeros(3 * L * M * N, complex)
+    Je[idx - 1] = x
+    return Je.reshape(L, M, N, 3, order='F'), flag, relres, it, resvec, solvert

@@ -1,0 +1,65 @@
+"""Writes tests/golden/ref_cubature.npz from the reference's own C++ sources.
+
+Run in a container that has the reference tree and oracle/_ref built
+(`make -C oracle`). The fixture holds inputs and outputs only:
+  * cubature_nop 6x6 blocks (cubature_nop.cpp) at the orders 4/10/20 the
+    assembly uses (assembly_nop.m:17-19), for several offsets and (dx, f);
+  * the five self-term integrals from DIRECTFN and DEMCEM at Np = 20
+    (singular_st/ea/va.m, assembly_nop.m:16);
+  * Gauss-Legendre nodes/weights quoted from gauss_1d.m for orders 4, 10, 20.
+"""
+import os
+import re
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import operators as oop  # noqa: E402
+from oracle import ref  # noqa: E402
+
+CASES = [(6.25e-3, 298.2e6), (2e-3, 128e6), (2e-3, 298e6), (1.5e-3, 298e6)]
+OFFSETS = [(0, 0, 0), (1, 0, 0), (1, 1, 1), (2, 1, 0), (3, 1, 2), (4, 4, 4), (7, 0, 0), (10, 3, 5)]
+C0 = 299792458.0
+
+
+def gauss_table(order, path='/root/reference/src_vie/src_operators/cubatures/gauss_1d.m'):
+    txt = open(path).read()
+    blk = re.split(r'order == %d \)' % order, txt)[1]
+    blk = re.split(r'elseif|else\s', blk)[0]
+    xs = {int(i): float(v.replace(' ', '')) for i, v in re.findall(r'xtab\((\d+)\)\s*=\s*([-+ 0-9.Ee]+);', blk)}
+    ws = {int(i): float(v.replace(' ', '')) for i, v in re.findall(r'weight\((\d+)\)\s*=\s*([-+ 0-9.Ee]+);', blk)}
+    n = len(xs)
+    return np.array([xs[i + 1] for i in range(n)]), np.array([ws[i + 1] for i in range(n)])
+
+
+def main():
+    R = ref.Ref()
+    out = {}
+    blocks, meta = [], []
+    for dx, f in CASES:
+        ko = 2 * np.pi / C0 * f
+        Rf = oop.cube_faces(dx)
+        for order in (4, 10, 20):
+            Np, w, u, v = oop.gauss_2d(order)
+            for m in OFFSETS:
+                rm = np.array(m, float) * dx
+                blocks.append(R.cubature_nop(Np, w, u, v, rm, np.zeros(3), Rf, ko, dx))
+                meta.append([dx, f, order, m[0], m[1], m[2]])
+        out_d = np.array(R.singular_terms(ko, dx, 'DIRECTFN', 20))
+        out_m = np.array(R.singular_terms(ko, dx, 'DEMCEM', 20))
+        out.setdefault('sing_directfn', []).append(out_d)
+        out.setdefault('sing_demcem', []).append(out_m)
+    np.savez_compressed(os.path.join(os.path.dirname(__file__), 'ref_cubature.npz'),
+                        cub_blocks=np.array(blocks), cub_meta=np.array(meta),
+                        sing_cases=np.array(CASES), sing_directfn=np.array(out['sing_directfn']),
+                        sing_demcem=np.array(out['sing_demcem']),
+                        **{'gauss_x_%d' % o: gauss_table(o)[0] for o in (4, 10, 20)},
+                        **{'gauss_w_%d' % o: gauss_table(o)[1] for o in (4, 10, 20)})
+    print('wrote', len(blocks), 'cubature blocks')
+
+
+if __name__ == '__main__':
+    main()

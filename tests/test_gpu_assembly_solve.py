@@ -1,0 +1,129 @@
+"""GPU parity of operator assembly (getOPERATORS 'N') and of the GMRES solve.
+
+fN from the device assembly is compared with oracle.operators.get_operators
+on small grids (<= 1e-12 relative, 2-norm over all six components); the
+self-terms with oracle.singular (<= 1e-13); GMRES with oracle.krylov.pgmres
+(same resvec to 1e-9, same solution to 1e-9, same flag / iteration counts).
+"""
+import numpy as np
+import pytest
+
+from oracle import fields, krylov, mvp, singular
+from oracle import operators as oop
+
+pytestmark = pytest.mark.gpu
+
+C0 = 299792458.0
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("dx,f", [(6.25e-3, 298.2e6), (2e-3, 128e6), (1.5e-3, 298e6)])
+def test_singular_terms(dx, f):
+    from marie_amd import lib
+    print("case singular", dx, f, flush=True)
+    ko = 2 * np.pi / C0 * f
+    got = lib.singular_terms(dx, ko)
+    ref = np.array(singular.all_terms(ko, dx))
+    assert np.max(np.abs(got - ref) / np.abs(ref)) <= 1e-13
+
+
+@pytest.mark.parametrize("shape,dx,f", [((8, 7, 6), 6.25e-3, 298.2e6), ((6, 6, 9), 2e-3, 128e6),
+                                        ((12, 12, 12), 5e-3, 64e6)])
+def test_assembly_parity(shape, dx, f):
+    from marie_amd import lib
+    L, M, N = shape
+    print("case assembly", shape, dx, f, flush=True)
+    ko = 2 * np.pi / C0 * f
+    op = lib.Operator(L, M, N)
+    op.assemble_N(dx, ko)
+    got = op.get_fN()
+    ref = oop.get_operators(L, M, N, dx, f, 'N')
+    assert _rel(got, ref) <= 1e-12
+    # caller-supplied self-terms (the reference's DIRECTFN/DEMCEM values) are used verbatim
+    sv = np.array(singular.all_terms(ko, dx)) * (1 + 1e-3)
+    op.assemble_N(dx, ko, sv)
+    got2 = op.get_fN()
+    G = oop.assembly_nop(L, M, N, ko, dx, method='given', ref=_Given(sv))
+    ref2 = oop.fft_operator(oop.circulant_nop(G))
+    assert _rel(got2, ref2) <= 1e-12
+
+
+class _Given:
+    def __init__(self, sv):
+        self.sv = sv
+
+    def singular_terms(self, ko, dx, method):
+        return tuple(self.sv)
+
+
+def _sphere_problem(n=12, dx=6.25e-3, f=298.2e6, eps=50.0, sig=0.5):
+    x = (np.arange(n) - (n - 1) / 2) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    r = np.stack([X, Y, Z], axis=-1)
+    inside = X ** 2 + Y ** 2 + Z ** 2 < (0.45 * n * dx) ** 2
+    em = fields.em_constants(f)
+    e_r = np.where(inside, eps - 1j * sig / (em['eo'] * em['omega']), 1.0 + 0j)
+    Einc, _ = fields.plane_wave_excitation(r, [0, 0, em['ko']], em['omega_mu'], [1, 0, 0])
+    return r, e_r, Einc, em
+
+
+@pytest.mark.parametrize("restart", [50, 7])
+def test_gmres_parity(restart):
+    from marie_amd import lib
+    n, dx, f = 12, 6.25e-3, 298.2e6
+    print("case gmres restart", restart, flush=True)
+    r, e_r, Einc, em = _sphere_problem(n, dx, f)
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(Einc, e_r, dx, f, form=2)
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx)
+    xr, flr, rr, itr, rvr = krylov.pgmres(A, b, restart, 1e-6, 200)
+    op = lib.Operator(n, n, n)
+    op.set_fN(fN)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx)
+    x, fl, rel, it, rv = op.gmres(b, restart=restart, tol=1e-6, maxit=200)
+    assert fl == flr == 0
+    assert it == itr
+    assert rv.size == rvr.size
+    assert np.max(np.abs(rv - rvr)) <= 1e-9
+    assert _rel(x, xr) <= 1e-9
+    assert abs(rel - rr) <= 1e-9
+
+
+def test_gmres_mixed_precision():
+    """complex64 matvecs + FP64 refinement reach the FP64 solution to the tolerance."""
+    from marie_amd import lib
+    n, dx, f = 14, 6.25e-3, 298.2e6
+    print("case gmres mixed", flush=True)
+    r, e_r, Einc, em = _sphere_problem(n, dx, f)
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(Einc, e_r, dx, f, form=2)
+    op = lib.Operator(n, n, n, precisions=lib.PREC_F64 | lib.PREC_F32)
+    op.set_fN(fN)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx)
+    x64, fl64, rel64, _, _ = op.gmres(b, restart=50, tol=1e-8, maxit=200, mode=0)
+    x32, fl32, rel32, it32, rv32 = op.gmres(b, restart=50, tol=1e-6, maxit=50, mode=1, inner_tol=1e-4)
+    assert fl64 == 0 and fl32 == 0
+    assert rel32 <= 1e-6
+    assert _rel(x32, x64) <= 1e-5
+
+
+def test_gmres_zero_rhs_and_maxit_flag():
+    from marie_amd import lib
+    print("case gmres zero rhs / flag", flush=True)
+    n, dx, f = 8, 6.25e-3, 298.2e6
+    r, e_r, Einc, em = _sphere_problem(n, dx, f)
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(Einc, e_r, dx, f, form=2)
+    op = lib.Operator(n, n, n)
+    op.set_fN(fN)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx)
+    x, fl, rel, it, rv = op.gmres(np.zeros_like(b))
+    assert fl == 0 and np.all(x == 0)
+    x, fl, rel, it, rv = op.gmres(b, restart=2, tol=1e-12, maxit=1)
+    assert fl == 1 and it[1] == 1
