@@ -59,7 +59,18 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--profile-only', action='store_true',
                     help='assemble and run the timed FP64 matvecs only (for rocprofv3 runs)')
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='run a plain 1 GiB device copy five times (scale factor for FETCH/WRITE_SIZE)')
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 26, dtype=torch.complex128, device='cuda:0')   # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes': a.numel() * 16, 'copies': 5}))
+        return
     if args.gpus != 1:
         print('bench.py measures one GPU: this build has no multi-GPU path (the sharding design is '
               'described in DESIGN.md); run with --gpus 1', file=sys.stderr)
@@ -143,8 +154,17 @@ def main():
     n = L * M * N
     nS = nidx // 3
     bmv = 16 * (8 * nS + 48 * n)
+    # measured HBM bytes per launch of the dominant kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    # scaled by a 1 GiB copy), recorded for this workload in profiles/
+    traffic = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                         'jvie_matvec_128cubed_fp64_traffic.json')
+    if n1 == 128 and os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        if dom in tj['kernels']:
+            traffic = tj['kernels'][dom]['fetch_bytes'] + tj['kernels'][dom]['write_bytes']
     roof = {'bound': 'hbm', 'kernel': dom, 'achieved': d['GBps'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-            'frac': d['GBps'] / HBM_PEAK_GBS, 'traffic': None,
+            'frac': d['GBps'] / HBM_PEAK_GBS, 'traffic': traffic, 'algorithmic_bytes': d['bytes'],
             'matvec_algorithmic_bytes': bmv, 'matvec_achieved_GBps': bmv * mv64 / 1e9,
             'matvec_frac': bmv * mv64 / 1e9 / HBM_PEAK_GBS}
 
