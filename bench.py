@@ -127,6 +127,13 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
+    # tile autotuning (part of the set-up, like an FFT plan)
+    tiles = {}
+    for prec, name in ((lib.PREC_F64, 'fp64'), (lib.PREC_F32, 'fp32')):
+        em, ms = op.autotune(prec, iters=5)
+        tiles[name] = em
+        log('tiles %s %s  (%s ms)' % (name, em, ' '.join('%.3f' % m for m in ms)))
+
     K, W = args.steps, args.warmup
     T64 = timed(lib.PREC_F64, K, W)
     mv64 = K / T64
@@ -228,6 +235,7 @@ def main():
                    'phantom_scale': scale, 'phantom': ph, 'non_air_dof': nidx, 'form': 2},
         'fp32_matvecs_per_s': mv32, 'fp32_ms_per_step': 1e3 * T32 / K,
         'assembly_s': t_asm,
+        'tiles_elements_per_thread': tiles,
         'kernels': kern,
         'roofline': roof,
         'cpu_baseline': cpu,

@@ -123,6 +123,17 @@ int marie_vie_gmres(marie_vie_op* h, const double* b, double* x, int restart, do
                     int mode, double inner_tol, int* flag, double* relres, int* iter, double* resvec,
                     int resvec_cap, int* nres);
 
+/* Tile autotuner: times every compiled tile variant of the five matvec
+ * kernels (elements per thread -> lines per workgroup -> LDS footprint) with
+ * HIP events on the handle's stream, `iters` products each, and keeps the
+ * fastest per kernel for precision `prec`. emax5 / best_ms5 (optional)
+ * receive the choice and its mean kernel times. No counterpart in the
+ * reference (MATLAB's fftn plans internally). */
+int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* emax5, double* best_ms5);
+
+/* Set the tile variant of each of the five kernels explicitly. */
+int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* emax5);
+
 /* Per-kernel timing with HIP events on the handle's stream. When enabled,
  * every apply records an event between its five kernels and accumulates
  * the durations; marie_vie_get_timing returns the mean ms per kernel

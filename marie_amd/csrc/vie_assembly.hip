@@ -560,13 +560,11 @@ int assemble_N(Op* h, double dx, double ko, const double* singular) {
     hipStream_t s = h->stream;
     const int64_t nv = (int64_t)L * M * N;
     const size_t nf = (size_t)6 * h->Px * h->Py * h->Pz;
-    if (!h->fN64) {
-        if (hipMalloc(&h->fN64, nf * sizeof(c2<double>)) != hipSuccess) {
-            (void)hipGetLastError();
-            h->fN64 = nullptr;
-            set_error("out of device memory for fN");
-            return ERR_ALLOC;
-        }
+    void* full = nullptr;
+    if (hipMalloc(&full, nf * sizeof(c2<double>)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("out of device memory for fN");
+        return ERR_ALLOC;
     }
     c2<double>* G = nullptr;
     c2<double>* Ju = nullptr;
@@ -580,6 +578,7 @@ int assemble_N(Op* h, double dx, double ko, const double* singular) {
         if (Ju) (void)hipFree(Ju);
         if (sing) (void)hipFree(sing);
         if (offs) (void)hipFree(offs);
+        if (full) (void)hipFree(full);
         if (r4.buf) (void)hipFree(r4.buf);
         if (r10.buf) (void)hipFree(r10.buf);
         if (r20.buf) (void)hipFree(r20.buf);
@@ -612,20 +611,18 @@ int assemble_N(Op* h, double dx, double ko, const double* singular) {
     hipLaunchKernelGGL(k_pairs, dim3(nmed * 21), dim3(AT), 0, s, Ju, offs, dx, ko, r10.r);
     hipLaunchKernelGGL(k_pairs, dim3(8 * 21), dim3(AT), 0, s, Ju + nmed * 21, offs + 3 * nmed, dx, ko, r20.r);
     hipLaunchKernelGGL(k_fixup, dim3(1), dim3(192), 0, s, G, Ju, offs, nmed, sing, L, M, N, dx);
-    hipLaunchKernelGGL(k_circulant, dim3(4096), dim3(256), 0, s, G, (c2<double>*)h->fN64, L, M, N, h->Px, h->Py, h->Pz);
+    hipLaunchKernelGGL(k_circulant, dim3(4096), dim3(256), 0, s, G, (c2<double>*)full, L, M, N, h->Px, h->Py, h->Pz);
     TRYH(hipGetLastError());
-    TRY(fft3_forward_inplace(h, (c2<double>*)h->fN64, 6, s));
-    if (h->precisions & PREC_F32) {
-        if (!h->fN32) TRYH(hipMalloc(&h->fN32, nf * sizeof(c2<float>)));
-        hipLaunchKernelGGL(k_to_f32, dim3(4096), dim3(256), 0, s, (const c2<double>*)h->fN64, (c2<float>*)h->fN32,
-                           (int64_t)nf);
-        TRYH(hipGetLastError());
-    }
+    TRY(fft3_forward_inplace(h, (c2<double>*)full, 6, s));
     TRYH(hipStreamSynchronize(s));
+    {
+        void* f = full;
+        full = nullptr;          // ownership passes to the handle
+        TRY(store_fN(h, f));
+    }
 #undef TRY
 #undef TRYH
     cleanup();
-    h->have_fN = true;
     return OK;
 }
 
@@ -637,15 +634,6 @@ extern "C" int marie_vie_assemble_N(marie_vie_op* hh, double dx, double ko, cons
     Op* h = reinterpret_cast<Op*>(hh);
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
     MARIE_HIP(hipSetDevice(h->device));
-    if (!(h->precisions & PREC_F64)) {
-        // the FP32 operator is rounded from an FP64 assembly; keep a temporary FP64 copy
-        const int keep = h->precisions;
-        h->precisions |= PREC_F64;
-        int st = assemble_N(h, dx, ko, singular);
-        h->precisions = keep;
-        if (h->fN64) { (void)hipFree(h->fN64); h->fN64 = nullptr; }
-        return st;
-    }
     return assemble_N(h, dx, ko, singular);
 }
 

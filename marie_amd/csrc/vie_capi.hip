@@ -149,29 +149,52 @@ int marie_vie_set_fN(marie_vie_op* h, const double* fN) {
     const size_t n = (size_t)6 * h->Px * h->Py * h->Pz;
     for (size_t i = 0; i < 2 * n; ++i)
         if (!std::isfinite(fN[i])) { set_error("fN has non-finite entries"); return MARIE_ERR_NONFINITE; }
-    int st = to_device_both(h, fN, n, &h->fN64, &h->fN32, false);
-    if (st) return st;
-    h->have_fN = true;
-    return MARIE_OK;
+    void* full = nullptr;
+    if (hipMalloc(&full, n * sizeof(c2<double>)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("out of device memory for fN");
+        return MARIE_ERR_ALLOC;
+    }
+    if (hipMemcpy(full, fN, n * sizeof(c2<double>), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(full);
+        set_error("fN upload failed");
+        return MARIE_ERR_HIP;
+    }
+    return store_fN(h, full);
 }
 
 int marie_vie_set_fN_device(marie_vie_op* h, const void* fN_dev) {
     CHECK_H(h);
     if (!fN_dev) { set_error("null fN"); return MARIE_ERR_ARG; }
     const size_t n = (size_t)6 * h->Px * h->Py * h->Pz;
-    int st = to_device_both(h, fN_dev, n, &h->fN64, &h->fN32, true);
-    if (st) return st;
-    h->have_fN = true;
-    return MARIE_OK;
+    void* full = nullptr;
+    if (hipMalloc(&full, n * sizeof(c2<double>)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("out of device memory for fN");
+        return MARIE_ERR_ALLOC;
+    }
+    if (hipMemcpyAsync(full, fN_dev, n * sizeof(c2<double>), hipMemcpyDeviceToDevice, h->stream) != hipSuccess) {
+        (void)hipFree(full);
+        set_error("fN copy failed");
+        return MARIE_ERR_HIP;
+    }
+    return store_fN(h, full);
 }
 
 int marie_vie_get_fN(marie_vie_op* h, double* fN) {
     CHECK_H(h);
     if (!h->have_fN || !h->fN64) { set_error("no FP64 fN on the handle"); return MARIE_ERR_STATE; }
     const size_t n = (size_t)6 * h->Px * h->Py * h->Pz;
-    MARIE_HIP(hipMemcpyAsync(fN, h->fN64, n * sizeof(c2<double>), hipMemcpyDeviceToHost, h->stream));
-    MARIE_HIP(hipStreamSynchronize(h->stream));
-    return MARIE_OK;
+    void* full = nullptr;
+    MARIE_HIP(hipMalloc(&full, n * sizeof(c2<double>)));
+    int st = load_fN(h, full);
+    if (st == OK && hipMemcpyAsync(fN, full, n * sizeof(c2<double>), hipMemcpyDeviceToHost, h->stream) != hipSuccess) {
+        set_error("fN download failed");
+        st = MARIE_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(full);
+    return st;
 }
 
 int marie_vie_set_material(marie_vie_op* h, const double* Mr, const double* Mc, double Gram) {
@@ -306,3 +329,33 @@ int marie_vie_get_timing(marie_vie_op* h, double* mean_ms5, int64_t* calls) {
 }
 
 }  // extern "C"
+
+extern "C" int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* emax5, double* best_ms5) {
+    CHECK_H(h);
+    if ((prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || !(h->precisions & prec) || iters < 1) {
+        set_error("invalid autotune arguments");
+        return MARIE_ERR_ARG;
+    }
+    if (!h->have_fN || !h->have_idx || !h->have_mat) {
+        set_error("operator, index or material not set");
+        return MARIE_ERR_STATE;
+    }
+    int st = autotune(h, prec, iters, best_ms5);
+    if (st) return st;
+    if (emax5) {
+        const int* em = prec == MARIE_PREC_F64 ? h->emax64 : h->emax32;
+        for (int k = 0; k < 5; ++k) emax5[k] = em[k];
+    }
+    return MARIE_OK;
+}
+
+extern "C" int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* emax5) {
+    CHECK_H(h);
+    if ((prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || !emax5) {
+        set_error("invalid tile arguments");
+        return MARIE_ERR_ARG;
+    }
+    int* em = prec == MARIE_PREC_F64 ? h->emax64 : h->emax32;
+    for (int k = 0; k < 5; ++k) em[k] = emax5[k];
+    return MARIE_OK;
+}

@@ -43,8 +43,11 @@ struct Op {
     AxisPlan plan[3];
 
     // device data
-    void* fN64 = nullptr;           // c2<double>[6][Pz][Py][Px]
-    void* fN32 = nullptr;           // c2<float>
+    // fN: full c2<T>[6][Pz][Py][Px], or its octant [6][Pz/2+1][Py/2+1][Px/2+1]
+    // when fN_oct (every component has the circulant's per-axis parity)
+    void* fN64 = nullptr;
+    void* fN32 = nullptr;
+    bool fN_oct = false;
     void* tw64 = nullptr;           // 3 axis tables, offsets 0, Px, Px+Py
     void* tw32 = nullptr;
     void* Mr64 = nullptr;           // c2<double>[N][M][L]
@@ -63,6 +66,10 @@ struct Op {
     void* B = nullptr;              // [3 nrhs][N][Py][Px]
     int nrhs_cap = 0;
 
+    // elements per thread of each pass (fwd_x, fwd_y, z, inv_y, inv_x) -> tile shape
+    int emax64[5] = {16, 16, 8, 16, 16};
+    int emax32[5] = {32, 32, 16, 32, 32};
+
     // optional per-kernel timing with HIP events on the handle's stream
     bool timing = false;
     hipEvent_t ev[8] = {};
@@ -76,5 +83,8 @@ int ensure_work(Op* h, int nrhs);
 int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s);
 int build_plan(int P, AxisPlan* pl);
 int upload_twiddles(Op* h);
+int store_fN(Op* h, void* full64);   // takes ownership of a device complex128 fN
+int load_fN(Op* h, void* out64);     // full complex128 fN into a device buffer
+int autotune(Op* h, int prec, int iters, double* best_ms5);
 
 }  // namespace marie

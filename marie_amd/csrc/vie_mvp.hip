@@ -25,6 +25,11 @@ namespace marie {
 
 constexpr int NT = 256;
 
+__global__ void k_to_f32_full(const c2<double>* __restrict__ a, c2<float>* __restrict__ b, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        b[e] = c2<float>{(float)a[e].x, (float)a[e].y};
+}
+
 // -------------------------------------------------------------------- kernels
 template <typename T, int EMAX, int RSET>
 __global__ __launch_bounds__(NT) void k_fwd_x(const c2<T>* __restrict__ x, c2<T>* __restrict__ A,
@@ -98,7 +103,12 @@ __global__ __launch_bounds__(NT) void k_strided(const c2<T>* __restrict__ src, c
 }
 
 // Fused z pass: forward z FFT of the three components, tensor product, inverse.
-template <typename T, int EMAX, int RSET>
+// OCT: fN is stored as its octant [6][Pz/2+1][Py/2+1][Px/2+1]. The circulant of
+// each tensor component is even or odd under reflection of each axis
+// (gperiodic_coeff_nop.m), so its DFT has the same parity:
+// F(P-k) = s F(k) with s = -1 iff exactly one of the component's two indices is
+// the reflected axis (xy: x,y; xz: x,z; yz: y,z).
+template <typename T, int EMAX, int RSET, bool OCT>
 __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __restrict__ fN,
                                           const c2<T>* __restrict__ tw, AxisPlan pl, int Px, int Py,
                                           int N, int lg) {
@@ -112,7 +122,6 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
     const int rhs = blockIdx.z;
     const int64_t plane = (int64_t)Py * Px;
     const int64_t cstride = plane * N;
-    const int64_t fstride = plane * P;
     const int nx = min(TX, Px - i0);
     const int tot = TX * P;
     c2<T>* base = B + (int64_t)rhs * 3 * cstride + (int64_t)j * Px + i0;
@@ -134,16 +143,42 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
         }
         __syncthreads();
     }
-    const c2<T>* f = fN + (int64_t)j * Px + i0;
+    // fN addressing
+    int64_t fplane, fstride, frow;
+    int hx = 0, hy = 0, hz = 0;
+    bool fy = false;
+    if (OCT) {
+        hx = Px / 2; hy = Py / 2; hz = P / 2;
+        fplane = (int64_t)(hy + 1) * (hx + 1);
+        fstride = fplane * (hz + 1);
+        fy = j > hy;
+        frow = (int64_t)(fy ? Py - j : j) * (hx + 1);
+    } else {
+        fplane = plane;
+        fstride = plane * P;
+        frow = (int64_t)j * Px + i0;
+    }
 #pragma unroll
     for (int q = 0; q < EMAX; ++q) {
         const int e = threadIdx.x + q * NT;
         if (e < tot) {
             const int b = e & msk, k = e >> lg;
             if (b < nx) {
-                const int64_t o = k * plane + b;
-                const c2<T> f0 = f[o], f1 = f[o + fstride], f2 = f[o + 2 * fstride];
-                const c2<T> f3 = f[o + 3 * fstride], f4 = f[o + 4 * fstride], f5 = f[o + 5 * fstride];
+                c2<T> f0, f1, f2, f3, f4, f5;
+                if (OCT) {
+                    const int i = i0 + b;
+                    const bool fx = i > hx, fz = k > hz;
+                    const int64_t o = (int64_t)(fz ? P - k : k) * fplane + frow + (fx ? Px - i : i);
+                    f0 = fN[o]; f1 = fN[o + fstride]; f2 = fN[o + 2 * fstride];
+                    f3 = fN[o + 3 * fstride]; f4 = fN[o + 4 * fstride]; f5 = fN[o + 5 * fstride];
+                    if (fx != fy) f1 = c2<T>{-f1.x, -f1.y};
+                    if (fx != fz) f2 = c2<T>{-f2.x, -f2.y};
+                    if (fy != fz) f4 = c2<T>{-f4.x, -f4.y};
+                } else {
+                    const int64_t o = k * fplane + frow + b;
+                    f0 = fN[o]; f1 = fN[o + fstride]; f2 = fN[o + 2 * fstride];
+                    f3 = fN[o + 3 * fstride]; f4 = fN[o + 4 * fstride]; f5 = fN[o + 5 * fstride];
+                }
                 const c2<T> x0 = X[0][q], x1 = X[1][q], x2 = X[2][q];
                 X[0][q] = cadd(cadd(cmul(f0, x0), cmul(f1, x1)), cmul(f2, x2));
                 X[1][q] = cadd(cadd(cmul(f1, x0), cmul(f3, x1)), cmul(f4, x2));
@@ -166,6 +201,153 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
         }
         __syncthreads();
     }
+}
+
+// ------------------------------------------------------ fN storage (octant)
+// max |F(k) - s F(P-k)| over all entries (per-block partial maxima), and max |F|
+__global__ void k_sym_check(const c2<double>* __restrict__ F, int Px, int Py, int Pz, double* __restrict__ part) {
+    const int64_t tot = (int64_t)Px * Py * Pz * 6;
+    double md = 0.0, ma = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % Px);
+        const int j = (int)((e / Px) % Py);
+        const int k = (int)((e / ((int64_t)Px * Py)) % Pz);
+        const int m = (int)(e / ((int64_t)Px * Py * Pz));
+        const int mi = i ? Px - i : 0, mj = j ? Py - j : 0, mk = k ? Pz - k : 0;
+        const c2<double> a = F[e];
+        // reflect each axis separately (parity per axis)
+        const int ax[6] = {0, 1, 1, 0, 0, 0}, ay[6] = {0, 1, 0, 0, 1, 0}, az[6] = {0, 0, 1, 0, 1, 0};
+        const int64_t base = (int64_t)m * Px * Py * Pz;
+        const c2<double> bx = F[base + ((int64_t)k * Py + j) * Px + mi];
+        const c2<double> by = F[base + ((int64_t)k * Py + mj) * Px + i];
+        const c2<double> bz = F[base + ((int64_t)mk * Py + j) * Px + i];
+        const double sx = ax[m] ? -1.0 : 1.0, sy = ay[m] ? -1.0 : 1.0, sz = az[m] ? -1.0 : 1.0;
+        md = fmax(md, fabs(a.x - sx * bx.x) + fabs(a.y - sx * bx.y));
+        md = fmax(md, fabs(a.x - sy * by.x) + fabs(a.y - sy * by.y));
+        md = fmax(md, fabs(a.x - sz * bz.x) + fabs(a.y - sz * bz.y));
+        ma = fmax(ma, fabs(a.x) + fabs(a.y));
+    }
+    __shared__ double s1[256], s2[256];
+    s1[threadIdx.x] = md;
+    s2[threadIdx.x] = ma;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + o]);
+            s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1[0];
+        part[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+template <typename T>
+__global__ void k_oct_compress(const c2<double>* __restrict__ F, c2<T>* __restrict__ O, int Px, int Py, int Pz) {
+    const int hx = Px / 2 + 1, hy = Py / 2 + 1, hz = Pz / 2 + 1;
+    const int64_t tot = (int64_t)hx * hy * hz * 6;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % hx);
+        const int j = (int)((e / hx) % hy);
+        const int k = (int)((e / ((int64_t)hx * hy)) % hz);
+        const int m = (int)(e / ((int64_t)hx * hy * hz));
+        const c2<double> v = F[(((int64_t)m * Pz + k) * Py + j) * Px + i];
+        O[e] = c2<T>{(T)v.x, (T)v.y};
+    }
+}
+
+__global__ void k_oct_expand(const c2<double>* __restrict__ O, c2<double>* __restrict__ F, int Px, int Py, int Pz) {
+    const int hx = Px / 2, hy = Py / 2, hz = Pz / 2;
+    const int64_t tot = (int64_t)Px * Py * Pz * 6;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % Px);
+        const int j = (int)((e / Px) % Py);
+        const int k = (int)((e / ((int64_t)Px * Py)) % Pz);
+        const int m = (int)(e / ((int64_t)Px * Py * Pz));
+        const bool fx = i > hx, fy = j > hy, fz = k > hz;
+        const int ie = fx ? Px - i : i, je = fy ? Py - j : j, ke = fz ? Pz - k : k;
+        c2<double> v = O[(((int64_t)m * (hz + 1) + ke) * (hy + 1) + je) * (hx + 1) + ie];
+        bool neg = false;
+        if (m == 1) neg = fx != fy;
+        if (m == 2) neg = fx != fz;
+        if (m == 4) neg = fy != fz;
+        if (neg) v = c2<double>{-v.x, -v.y};
+        F[e] = v;
+    }
+}
+
+// Takes ownership of a device buffer holding the full complex128 fN and stores
+// it on the handle: octant form when every component has the circulant's
+// parity to 1e-14 of max|fN| (even sides only), full form otherwise.
+int store_fN(Op* h, void* full64) {
+    const int Px = h->Px, Py = h->Py, Pz = h->Pz;
+    const size_t nf = (size_t)6 * Px * Py * Pz;
+    hipStream_t s = h->stream;
+    bool oct = false;
+    if (!(Px & 1) && !(Py & 1) && !(Pz & 1)) {
+        const int G = 1024;
+        double* part = nullptr;
+        MARIE_HIP(hipMalloc(&part, 2 * G * sizeof(double)));
+        hipLaunchKernelGGL(k_sym_check, dim3(G), dim3(256), 0, s, (const c2<double>*)full64, Px, Py, Pz, part);
+        std::vector<double> hp(2 * G);
+        MARIE_HIP(hipMemcpyAsync(hp.data(), part, 2 * G * sizeof(double), hipMemcpyDeviceToHost, s));
+        MARIE_HIP(hipStreamSynchronize(s));
+        MARIE_HIP(hipFree(part));
+        double md = 0.0, ma = 0.0;
+        for (int b = 0; b < G; ++b) { md = std::max(md, hp[2 * b]); ma = std::max(ma, hp[2 * b + 1]); }
+        oct = ma > 0.0 && md <= 1e-14 * ma;
+    }
+    if (h->fN64) { MARIE_HIP(hipFree(h->fN64)); h->fN64 = nullptr; }
+    if (h->fN32) { MARIE_HIP(hipFree(h->fN32)); h->fN32 = nullptr; }
+    h->fN_oct = oct;
+    if (oct) {
+        const size_t no = (size_t)6 * (Px / 2 + 1) * (Py / 2 + 1) * (Pz / 2 + 1);
+        if (h->precisions & PREC_F64) {
+            MARIE_HIP(hipMalloc(&h->fN64, no * sizeof(c2<double>)));
+            hipLaunchKernelGGL(k_oct_compress<double>, dim3(4096), dim3(256), 0, s, (const c2<double>*)full64,
+                               (c2<double>*)h->fN64, Px, Py, Pz);
+        }
+        if (h->precisions & PREC_F32) {
+            MARIE_HIP(hipMalloc(&h->fN32, no * sizeof(c2<float>)));
+            hipLaunchKernelGGL(k_oct_compress<float>, dim3(4096), dim3(256), 0, s, (const c2<double>*)full64,
+                               (c2<float>*)h->fN32, Px, Py, Pz);
+        }
+        MARIE_HIP(hipGetLastError());
+        MARIE_HIP(hipStreamSynchronize(s));
+        MARIE_HIP(hipFree(full64));
+    } else {
+        if (h->precisions & PREC_F32) {
+            MARIE_HIP(hipMalloc(&h->fN32, nf * sizeof(c2<float>)));
+            hipLaunchKernelGGL(k_to_f32_full, dim3(4096), dim3(256), 0, s, (const c2<double>*)full64,
+                               (c2<float>*)h->fN32, (int64_t)nf);
+            MARIE_HIP(hipGetLastError());
+        }
+        if (h->precisions & PREC_F64) {
+            h->fN64 = full64;
+        } else {
+            MARIE_HIP(hipStreamSynchronize(s));
+            MARIE_HIP(hipFree(full64));
+        }
+        MARIE_HIP(hipStreamSynchronize(s));
+    }
+    h->have_fN = true;
+    return OK;
+}
+
+// full complex128 fN into a caller device buffer (expands the octant form)
+int load_fN(Op* h, void* out64) {
+    const size_t nf = (size_t)6 * h->Px * h->Py * h->Pz;
+    if (!h->fN64) { set_error("no FP64 fN on the handle"); return ERR_STATE; }
+    if (h->fN_oct) {
+        hipLaunchKernelGGL(k_oct_expand, dim3(4096), dim3(256), 0, h->stream, (const c2<double>*)h->fN64,
+                           (c2<double>*)out64, h->Px, h->Py, h->Pz);
+        MARIE_HIP(hipGetLastError());
+    } else {
+        MARIE_HIP(hipMemcpyAsync(out64, h->fN64, nf * sizeof(c2<double>), hipMemcpyDeviceToDevice, h->stream));
+    }
+    return OK;
 }
 
 // Inverse x pass with the crop, the 1/(Px Py Pz) scale and the material epilogue.
@@ -297,7 +479,49 @@ static int pick_lg(int P, int cap, int txmax) {
     return lg;
 }
 
-template <typename T, int EMAX_XY, int EMAX_Z>
+// Tile variants. Each pass runs with EMAX elements per thread (so a workgroup of
+// NT threads holds TX = EMAX*NT/P lines in LDS); the compiled set below is what
+// the autotuner chooses from. Radix sets beyond {3,5} only get the default.
+template <typename T> struct Variants;
+template <> struct Variants<double> {
+    static constexpr int xy[3] = {4, 8, 16};
+    static constexpr int z[2] = {4, 8};
+    static constexpr int txmax = 16;
+};
+template <> struct Variants<float> {
+    static constexpr int xy[3] = {8, 16, 32};
+    static constexpr int z[2] = {8, 16};
+    static constexpr int txmax = 32;
+};
+
+#define MARIE_EMAX_XY(T, emax, RSv, STMT)                                                   \
+    do {                                                                                    \
+        if constexpr (sizeof(T) == 8) {                                                     \
+            if ((emax) == 4 && RSv < 2) { constexpr int EM = 4; STMT; }                     \
+            else if ((emax) == 8 && RSv < 2) { constexpr int EM = 8; STMT; }                \
+            else { constexpr int EM = 16; STMT; }                                           \
+        } else {                                                                            \
+            if ((emax) == 8 && RSv < 2) { constexpr int EM = 8; STMT; }                     \
+            else if ((emax) == 16 && RSv < 2) { constexpr int EM = 16; STMT; }              \
+            else { constexpr int EM = 32; STMT; }                                           \
+        }                                                                                   \
+    } while (0)
+#define MARIE_EMAX_Z(T, emax, RSv, STMT)                                                    \
+    do {                                                                                    \
+        if constexpr (sizeof(T) == 8) {                                                     \
+            if ((emax) == 4 && RSv < 2) { constexpr int EM = 4; STMT; }                     \
+            else { constexpr int EM = 8; STMT; }                                            \
+        } else {                                                                            \
+            if ((emax) == 8 && RSv < 2) { constexpr int EM = 8; STMT; }                     \
+            else { constexpr int EM = 16; STMT; }                                           \
+        }                                                                                   \
+    } while (0)
+
+static int effective_emax(int emax, const AxisPlan& pl, int dflt) {
+    return plan_rset(pl) == RSET_ALL ? dflt : emax;
+}
+
+template <typename T>
 static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs, hipStream_t s) {
     const bool f64 = sizeof(T) == 8;
     const c2<T>* fN = reinterpret_cast<const c2<T>*>(f64 ? h->fN64 : h->fN32);
@@ -311,10 +535,14 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     c2<T>* B = reinterpret_cast<c2<T>*>(h->B);
     const int L = h->L, M = h->M, N = h->N, Px = h->Px, Py = h->Py, Pz = h->Pz;
     const int ncomp = 3 * nrhs;
-    const int capxy = EMAX_XY * NT, capz = EMAX_Z * NT;
-    const int txmax = f64 ? 16 : 32;
-    if (Px > capxy || Py > capxy || Pz > capz) {
-        set_error("circulant side exceeds the single-tile FFT limit");
+    const int* em = f64 ? h->emax64 : h->emax32;
+    const int dxy = f64 ? 16 : 32, dz = f64 ? 8 : 16;
+    const int txmax = Variants<T>::txmax;
+    const int e0 = effective_emax(em[0], h->plan[0], dxy), e1 = effective_emax(em[1], h->plan[1], dxy);
+    const int e2 = effective_emax(em[2], h->plan[2], dz), e3 = effective_emax(em[3], h->plan[1], dxy);
+    const int e4 = effective_emax(em[4], h->plan[0], dxy);
+    if (Px > e0 * NT || Px > e4 * NT || Py > e1 * NT || Py > e3 * NT || Pz > e2 * NT) {
+        set_error("circulant side exceeds the single-tile FFT limit of the selected tile variant");
         return ERR_SIZE;
     }
     const int64_t nline = (int64_t)M * N;
@@ -323,53 +551,58 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
 
     // forward x
     {
-        const int lg = pick_lg(Px, capxy, 16);
+        const int lg = pick_lg(Px, e0 * NT, 16);
         const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
         const size_t sm = ((size_t)Px << lg) * sizeof(c2<T>);
-        MARIE_RSET(h->plan[0], hipLaunchKernelGGL((k_fwd_x<T, EMAX_XY, RS>), g, dim3(NT), sm, s, x, A, h->pos, Mc, twx, h->plan[0], L,
-                           M, N, lg, h->nidx, (op == OP_AN && transp) ? 1 : 0));
+        MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e0, RS, hipLaunchKernelGGL((k_fwd_x<T, EM, RS>), g, dim3(NT), sm, s,
+                   x, A, h->pos, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, (op == OP_AN && transp) ? 1 : 0)));
         MARIE_HIP(hipGetLastError());
     }
     if (tm) MARIE_HIP(hipEventRecord(h->ev[1], s));
     // forward y: A[c][k][j<M][i] -> B[c][k][j<Py][i]
     {
-        const int lg = pick_lg(Py, capxy, txmax);
+        const int lg = pick_lg(Py, e1 * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, ncomp);
         const size_t sm = ((size_t)Py << lg) * sizeof(c2<T>);
-        MARIE_RSET(h->plan[1], hipLaunchKernelGGL((k_strided<T, EMAX_XY, false, RS>), g, dim3(NT), sm, s, A, B, twy, h->plan[1], Px,
-                           (int64_t)Px, (int64_t)M * Px, (int64_t)Px, (int64_t)Py * Px, M, Py, lg,
-                           (int64_t)N * M * Px, (int64_t)N * Py * Px));
+        MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e1, RS, hipLaunchKernelGGL((k_strided<T, EM, false, RS>), g, dim3(NT),
+                   sm, s, A, B, twy, h->plan[1], Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px, (int64_t)Py * Px, M,
+                   Py, lg, (int64_t)N * M * Px, (int64_t)N * Py * Px)));
         MARIE_HIP(hipGetLastError());
     }
     if (tm) MARIE_HIP(hipEventRecord(h->ev[2], s));
     // z: fused forward, tensor product, inverse
     {
-        const int lg = pick_lg(Pz, capz, txmax);
+        const int lg = pick_lg(Pz, e2 * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), Py, nrhs);
         const size_t sm = ((size_t)Pz << lg) * sizeof(c2<T>);
-        MARIE_RSET(h->plan[2], hipLaunchKernelGGL((k_z<T, EMAX_Z, RS>), g, dim3(NT), sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg));
+        if (h->fN_oct)
+            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e2, RS, hipLaunchKernelGGL((k_z<T, EM, RS, true>), g, dim3(NT), sm, s,
+                       B, fN, twz, h->plan[2], Px, Py, N, lg)));
+        else
+            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e2, RS, hipLaunchKernelGGL((k_z<T, EM, RS, false>), g, dim3(NT), sm,
+                       s, B, fN, twz, h->plan[2], Px, Py, N, lg)));
         MARIE_HIP(hipGetLastError());
     }
     if (tm) MARIE_HIP(hipEventRecord(h->ev[3], s));
     // inverse y: B[c][k][j<Py][i] -> A[c][k][j<M][i]
     {
-        const int lg = pick_lg(Py, capxy, txmax);
+        const int lg = pick_lg(Py, e3 * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, ncomp);
         const size_t sm = ((size_t)Py << lg) * sizeof(c2<T>);
-        MARIE_RSET(h->plan[1], hipLaunchKernelGGL((k_strided<T, EMAX_XY, true, RS>), g, dim3(NT), sm, s, B, A, twy, h->plan[1], Px,
-                           (int64_t)Px, (int64_t)Py * Px, (int64_t)Px, (int64_t)M * Px, Py, M, lg,
-                           (int64_t)N * Py * Px, (int64_t)N * M * Px));
+        MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e3, RS, hipLaunchKernelGGL((k_strided<T, EM, true, RS>), g, dim3(NT),
+                   sm, s, B, A, twy, h->plan[1], Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px, (int64_t)M * Px, Py,
+                   M, lg, (int64_t)N * Py * Px, (int64_t)N * M * Px)));
         MARIE_HIP(hipGetLastError());
     }
     if (tm) MARIE_HIP(hipEventRecord(h->ev[4], s));
     // inverse x + crop + epilogue
     {
-        const int lg = pick_lg(Px, capxy, 16);
+        const int lg = pick_lg(Px, e4 * NT, 16);
         const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
         const size_t sm = ((size_t)Px << lg) * sizeof(c2<T>);
         const T scale = T(1.0 / ((double)Px * (double)Py * (double)Pz));
-        MARIE_RSET(h->plan[0], hipLaunchKernelGGL((k_inv_x<T, EMAX_XY, RS>), g, dim3(NT), sm, s, A, y, x, h->pos, Mr, Mc, twx,
-                           h->plan[0], L, M, N, lg, h->nidx, op, transp, (T)h->Gram, scale));
+        MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e4, RS, hipLaunchKernelGGL((k_inv_x<T, EM, RS>), g, dim3(NT), sm, s,
+                   A, y, x, h->pos, Mr, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, op, transp, (T)h->Gram, scale)));
         MARIE_HIP(hipGetLastError());
     }
     if (tm) {
@@ -396,9 +629,59 @@ int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs,
     }
     int st = ensure_work(h, nrhs);
     if (st) return st;
-    if (prec == PREC_F64)
-        return apply_t<double, 16, 8>(h, op, transp, (const c2<double>*)x, (c2<double>*)y, nrhs, s);
-    return apply_t<float, 32, 16>(h, op, transp, (const c2<float>*)x, (c2<float>*)y, nrhs, s);
+    if (prec == PREC_F64) return apply_t<double>(h, op, transp, (const c2<double>*)x, (c2<double>*)y, nrhs, s);
+    return apply_t<float>(h, op, transp, (const c2<float>*)x, (c2<float>*)y, nrhs, s);
+}
+
+// Times every compiled tile variant of each pass with HIP events on the
+// handle's stream (real buffers, `iters` products each) and keeps the fastest
+// per pass. Results are left in h->emax64 / h->emax32.
+int autotune(Op* h, int prec, int iters, double* best_ms5) {
+    const bool f64 = prec == PREC_F64;
+    int* em = f64 ? h->emax64 : h->emax32;
+    const int nxy = 3, nz = 2;
+    const int* vxy = f64 ? Variants<double>::xy : Variants<float>::xy;
+    const int* vz = f64 ? Variants<double>::z : Variants<float>::z;
+    const size_t es = f64 ? sizeof(c2<double>) : sizeof(c2<float>);
+    void *x = nullptr, *y = nullptr;
+    const size_t n = (size_t)(h->nidx > 0 ? h->nidx : 1);
+    MARIE_HIP(hipMalloc(&x, n * es));
+    MARIE_HIP(hipMalloc(&y, n * es));
+    MARIE_HIP(hipMemsetAsync(x, 0, n * es, h->stream));
+    double best[5] = {1e30, 1e30, 1e30, 1e30, 1e30};
+    int pick[5] = {em[0], em[1], em[2], em[3], em[4]};
+    const bool was = h->timing;
+    int st = OK;
+    const int trials = nxy > nz ? nxy : nz;
+    for (int v = 0; v < trials && st == OK; ++v) {
+        const int exy = vxy[v < nxy ? v : nxy - 1];
+        const int ez = vz[v < nz ? v : nz - 1];
+        const int cand[5] = {exy, exy, ez, exy, exy};
+        // skip variants too small for the line lengths
+        if (h->Px > exy * NT || h->Py > exy * NT || h->Pz > ez * NT) continue;
+        for (int k = 0; k < 5; ++k) em[k] = cand[k];
+        h->timing = false;
+        st = apply(h, OP_AN, 0, prec, x, y, 1, h->stream);      // warm-up
+        if (st) break;
+        h->timing = true;
+        for (int k = 0; k < 8; ++k) h->kernel_ms[k] = 0.0;
+        h->kernel_calls = 0;
+        for (int it = 0; it < iters && st == OK; ++it) st = apply(h, OP_AN, 0, prec, x, y, 1, h->stream);
+        for (int k = 0; k < 5 && st == OK; ++k) {
+            const double ms = h->kernel_ms[k] / (h->kernel_calls ? h->kernel_calls : 1);
+            if (ms < best[k]) { best[k] = ms; pick[k] = cand[k]; }
+        }
+    }
+    for (int k = 0; k < 5; ++k) em[k] = pick[k];
+    h->timing = was;
+    for (int k = 0; k < 8; ++k) h->kernel_ms[k] = 0.0;
+    h->kernel_calls = 0;
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(x);
+    (void)hipFree(y);
+    if (best_ms5)
+        for (int k = 0; k < 5; ++k) best_ms5[k] = best[k];
+    return st;
 }
 
 }  // namespace marie

@@ -43,6 +43,8 @@ SIGNATURES = [
                                     ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_gmres', _I, [_P, _P, _P, _I, _D, _I, _I, _D, ctypes.POINTER(_I), _DP,
                              ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_autotune', _I, [_P, _I, _I, ctypes.POINTER(_I), _DP]),
+    ('marie_vie_set_tiles', _I, [_P, _I, ctypes.POINTER(_I)]),
     ('marie_vie_enable_timing', _I, [_P, _I]),
     ('marie_vie_get_timing', _I, [_P, _DP, ctypes.POINTER(_I64)]),
 ]
@@ -202,6 +204,18 @@ class Operator:
                                      ctypes.byref(flag), ctypes.byref(relres), it, resvec.ctypes.data_as(_DP),
                                      resvec_cap, ctypes.byref(nres)))
         return x, flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    # -- tiles ------------------------------------------------------------------
+    def autotune(self, prec=PREC_F64, iters=5):
+        """Pick the fastest tile variant per kernel; returns (emax5, best_ms5)."""
+        em = (ctypes.c_int * 5)()
+        ms = (ctypes.c_double * 5)()
+        check(load().marie_vie_autotune(self._h, prec, iters, em, ms))
+        return list(em), list(ms)
+
+    def set_tiles(self, emax5, prec=PREC_F64):
+        em = (ctypes.c_int * 5)(*emax5)
+        check(load().marie_vie_set_tiles(self._h, prec, em))
 
     # -- timing -------------------------------------------------------------------
     def enable_timing(self, on=True):

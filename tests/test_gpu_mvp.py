@@ -175,3 +175,36 @@ def test_error_paths():
         op.set_fN(bad)
     with pytest.raises(L_.MarieError):
         L_.Operator(4, 4, 4, (8, 8, 34))         # 34 = 2 * 17: prime factor above 13
+
+
+@pytest.mark.parametrize("prec,variants", [("f64", [4, 8, 16]), ("f32", [8, 16, 32])])
+def test_every_tile_variant(prec, variants):
+    """Each compiled tile shape gives the oracle's product (both fN storage forms)."""
+    from marie_amd import lib as L_
+    L, M, N = 12, 10, 16
+    print("case tile variants", prec, flush=True)
+    rng = np.random.default_rng(21)
+    # parity-consistent operator -> octant storage; random operator -> full storage
+    G = rng.standard_normal((L, M, N, 6)) + 1j * rng.standard_normal((L, M, N, 6))
+    for c in range(6):
+        if oop.GP_COEFF['L'][c] < 0:
+            G[0, :, :, c] = 0
+        if oop.GP_COEFF['M'][c] < 0:
+            G[:, 0, :, c] = 0
+        if oop.GP_COEFF['N'][c] < 0:
+            G[:, :, 0, c] = 0
+    fN_sym = oop.fft_operator(oop.circulant_nop(G))
+    _, Mr, Mc, idx, x = _case(L, M, N, seed=22)
+    fN_rand = _case(L, M, N, seed=23)[0]
+    p = L_.PREC_F64 if prec == "f64" else L_.PREC_F32
+    tol = TOL64 if prec == "f64" else TOL32
+    for fN in (fN_sym, fN_rand):
+        op = _op(L, M, N, None, fN, Mr, Mc, 0.8, idx, prec=p)
+        yr = mvp.mv_AN(x, fN, Mr, Mc, 0.8, 'notransp', idx)
+        for exy in variants:
+            for ez in variants[:2]:
+                op.set_tiles([exy, exy, ez, exy, exy], p)
+                assert _rel(op.apply(x), yr) <= tol, (exy, ez)
+        # round trip of the stored operator (octant expand or full copy)
+        if prec == "f64":
+            assert _rel(op.get_fN(), fN) <= 1e-15
