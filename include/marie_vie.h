@@ -123,16 +123,19 @@ int marie_vie_gmres(marie_vie_op* h, const double* b, double* x, int restart, do
                     int mode, double inner_tol, int* flag, double* relres, int* iter, double* resvec,
                     int resvec_cap, int* nres);
 
-/* Tile autotuner: times every compiled tile variant of the five matvec
- * kernels (elements per thread -> lines per workgroup -> LDS footprint) with
- * HIP events on the handle's stream, `iters` products each, and keeps the
- * fastest per kernel for precision `prec`. emax5 / best_ms5 (optional)
- * receive the choice and its mean kernel times. No counterpart in the
- * reference (MATLAB's fftn plans internally). */
-int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* emax5, double* best_ms5);
+/* Tile autotuner: times every compiled variant of the five matvec kernels
+ * with HIP events on the handle's stream, `iters` products each, and keeps
+ * the fastest per kernel for precision `prec`. A variant is cfg6[0..4], one
+ * value per kernel (fwd_x, fwd_y, z, inv_y, inv_x): v > 0 selects the
+ * LDS-stage FFT with v elements per thread, v < 0 the register-resident FFT
+ * with -v points per thread (power-of-two sides); cfg6[5] = 1 makes one
+ * workgroup of the z kernel process all mirror columns of an octant-stored
+ * fN. cfg6 / best_ms5 (optional) receive the choice and its mean kernel
+ * times. No counterpart in the reference (MATLAB's fftn plans internally). */
+int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* cfg6, double* best_ms5);
 
-/* Set the tile variant of each of the five kernels explicitly. */
-int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* emax5);
+/* Set the kernel variants explicitly (cfg6 as above). */
+int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* cfg6);
 
 /* Per-kernel timing with HIP events on the handle's stream. When enabled,
  * every apply records an event between its five kernels and accumulates

@@ -9,7 +9,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libmarie_vie.so')
-SOURCES = ['vie_capi.hip', 'vie_mvp.hip', 'vie_krylov.hip', 'vie_assembly.hip']
+SOURCES = ['vie_capi.hip', 'vie_mvp.hip', 'vie_mvp_reg_f64.hip', 'vie_mvp_reg_f32.hip', 'vie_krylov.hip',
+           'vie_assembly.hip']
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
@@ -18,7 +19,7 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
          '-Wno-unused-result', '-Wno-unknown-warning-option']
 
 
-def build(verbose=False, jobs=4):
+def build(verbose=False, jobs=6):
     objs = []
     procs = []
     for s in SOURCES:

@@ -330,7 +330,7 @@ int marie_vie_get_timing(marie_vie_op* h, double* mean_ms5, int64_t* calls) {
 
 }  // extern "C"
 
-extern "C" int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* emax5, double* best_ms5) {
+extern "C" int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* cfg6, double* best_ms5) {
     CHECK_H(h);
     if ((prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || !(h->precisions & prec) || iters < 1) {
         set_error("invalid autotune arguments");
@@ -342,20 +342,21 @@ extern "C" int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* ema
     }
     int st = autotune(h, prec, iters, best_ms5);
     if (st) return st;
-    if (emax5) {
+    if (cfg6) {
         const int* em = prec == MARIE_PREC_F64 ? h->emax64 : h->emax32;
-        for (int k = 0; k < 5; ++k) emax5[k] = em[k];
+        for (int k = 0; k < 5; ++k) cfg6[k] = em[k];
+        cfg6[5] = 0;
     }
     return MARIE_OK;
 }
 
-extern "C" int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* emax5) {
+extern "C" int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* cfg6) {
     CHECK_H(h);
-    if ((prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || !emax5) {
+    if ((prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || !cfg6) {
         set_error("invalid tile arguments");
         return MARIE_ERR_ARG;
     }
     int* em = prec == MARIE_PREC_F64 ? h->emax64 : h->emax32;
-    for (int k = 0; k < 5; ++k) em[k] = emax5[k];
+    for (int k = 0; k < 5; ++k) em[k] = cfg6[k];
     return MARIE_OK;
 }

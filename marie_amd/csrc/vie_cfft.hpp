@@ -1,0 +1,116 @@
+// Register-resident Stockham FFT with a compile-time plan (power-of-two P).
+//
+// Same data movement as vie_regfft.hpp (thread t of Tl = P/E owns butterflies
+// jj = t + Tl*q of each stage, slot v[q*R + r] holds input jj + r*P/R; LDS only
+// between stages), but the line length P, the points per thread E and the
+// radix sequence are template constants, so every slot index, twiddle index
+// and LDS address is a shift/add of t and the butterflies are straight-line
+// code on fixed registers.
+#pragma once
+#include "vie_regfft.hpp"
+
+namespace marie {
+
+struct CPlanData {
+    int n;
+    int r[12];
+};
+
+// greedy radices from {16, 8, 4, 2} that divide E and what is left of P
+constexpr CPlanData make_cplan(int P, int E) {
+    CPlanData d{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    int rem = P;
+    while (rem > 1 && d.n < 12) {
+        int R = 16;
+        while (R > 1 && (R > E || E % R != 0 || rem % R != 0)) R >>= 1;
+        d.r[d.n] = R;
+        d.n += 1;
+        rem /= R;
+    }
+    return d;
+}
+
+constexpr int cplan_radix(int P, int E, bool rev, int s) {
+    return rev ? make_cplan(P, E).r[make_cplan(P, E).n - 1 - s] : make_cplan(P, E).r[s];
+}
+
+// product of the radices before stage s
+constexpr int cplan_ns(int P, int E, bool rev, int s) {
+    int ns = 1;
+    for (int i = 0; i < s; ++i) ns *= cplan_radix(P, E, rev, i);
+    return ns;
+}
+
+constexpr int cplan_first(int P, int E, bool rev) { return cplan_radix(P, E, rev, 0); }
+constexpr int cplan_last(int P, int E, bool rev) { return cplan_radix(P, E, rev, make_cplan(P, E).n - 1); }
+
+template <typename T, int P, int E, int R, int Ns, bool INV>
+__device__ __forceinline__ void c_bfly(c2<T> (&v)[E], const c2<T>* __restrict__ tw, int t) {
+    constexpr int Tl = P / E;
+    constexpr int Q = E / R;
+    constexpr int ts = P / (Ns * R);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if constexpr (Ns > 1) {
+            const int k = (t + Tl * q) & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const c2<T> w = tw[r * k * ts];
+                v[q * R + r] = INV ? cmulc(v[q * R + r], w) : cmul(v[q * R + r], w);
+            }
+        }
+        rdft<T, INV, R>(&v[q * R]);
+    }
+}
+
+// outputs of a stage (radix Rp, sub-size Nsp) -> inputs of the next (radix R)
+template <typename T, int P, int E, int Rp, int Nsp, int R>
+__device__ __forceinline__ void c_exchange(c2<T> (&v)[E], c2<T>* lds, int sp, int off, int t) {
+    constexpr int Tl = P / E;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int q = m / Rp, r = m % Rp;
+        const int jj = t + Tl * q;
+        const int k = jj & (Nsp - 1);
+        const int o = (jj - k) * Rp + k + r * Nsp;
+        lds[o * sp + off] = v[m];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int q = m / R, r = m % R;
+        v[m] = lds[((t + Tl * q) + r * (P / R)) * sp + off];
+    }
+}
+
+template <typename T, int P, int E, bool INV, bool REV, int S>
+__device__ __forceinline__ void c_fft_rec(c2<T> (&v)[E], c2<T>* lds, int sp, int off, const c2<T>* tw, int t) {
+    constexpr int n = make_cplan(P, E).n;
+    if constexpr (S < n) {
+        constexpr int R = cplan_radix(P, E, REV, S);
+        constexpr int Ns = cplan_ns(P, E, REV, S);
+        if constexpr (S > 0) {
+            constexpr int Rp = cplan_radix(P, E, REV, S - 1);
+            c_exchange<T, P, E, Rp, Ns / Rp, R>(v, lds, sp, off, t);
+        }
+        c_bfly<T, P, E, R, Ns, INV>(v, tw, t);
+        c_fft_rec<T, P, E, INV, REV, S + 1>(v, lds, sp, off, tw, t);
+    }
+}
+
+// Full transform (REV: radices in reverse order, used so an inverse can start
+// from the layout a forward transform ends in).
+template <typename T, int P, int E, bool INV, bool REV>
+__device__ __forceinline__ void c_fft(c2<T> (&v)[E], c2<T>* lds, int sp, int off, const c2<T>* tw, int t) {
+    c_fft_rec<T, P, E, INV, REV, 0>(v, lds, sp, off, tw, t);
+}
+
+// position held by slot m in a first-stage input / last-stage output layout of radix R
+template <int P, int E, int R>
+__device__ __forceinline__ int c_pos(int m, int t) {
+    constexpr int Tl = P / E;
+    return t + Tl * ((m / R) + (E / R) * (m % R));
+}
+
+}  // namespace marie

@@ -16,8 +16,11 @@
 // discards, so the intermediates are 2n (A) and 4n (B) complex per component
 // instead of 8n.
 #include "vie_handle.hpp"
+#include "vie_reg.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -538,71 +541,96 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     const int* em = f64 ? h->emax64 : h->emax32;
     const int dxy = f64 ? 16 : 32, dz = f64 ? 8 : 16;
     const int txmax = Variants<T>::txmax;
-    const int e0 = effective_emax(em[0], h->plan[0], dxy), e1 = effective_emax(em[1], h->plan[1], dxy);
-    const int e2 = effective_emax(em[2], h->plan[2], dz), e3 = effective_emax(em[3], h->plan[1], dxy);
-    const int e4 = effective_emax(em[4], h->plan[0], dxy);
-    if (Px > e0 * NT || Px > e4 * NT || Py > e1 * NT || Py > e3 * NT || Pz > e2 * NT) {
-        set_error("circulant side exceeds the single-tile FFT limit of the selected tile variant");
-        return ERR_SIZE;
+    // per pass: cfg > 0 -> LDS-stage engine with cfg elements per thread;
+    //           cfg < 0 -> register engine with -cfg points per thread (2^k sides)
+    bool reg[5];
+    const int Pp[5] = {Px, Py, Pz, Py, Px};
+    for (int k = 0; k < 5; ++k) reg[k] = em[k] < 0 && reg_supported<T>(Pp[k], -em[k]);
+    const int dflt[5] = {dxy, dxy, dz, dxy, dxy};
+    const AxisPlan* apl[5] = {&h->plan[0], &h->plan[1], &h->plan[2], &h->plan[1], &h->plan[0]};
+    int e[5];
+    for (int k = 0; k < 5; ++k) {
+        e[k] = effective_emax(em[k] > 0 ? em[k] : dflt[k], *apl[k], dflt[k]);
+        if (!reg[k] && Pp[k] > e[k] * NT) {
+            set_error("circulant side exceeds the single-tile FFT limit of the selected tile variant");
+            return ERR_SIZE;
+        }
     }
     const int64_t nline = (int64_t)M * N;
     const bool tm = h->timing;
     if (tm) MARIE_HIP(hipEventRecord(h->ev[0], s));
+    const int conj_in = (op == OP_AN && transp) ? 1 : 0;
 
     // forward x
-    {
-        const int lg = pick_lg(Px, e0 * NT, 16);
+    if (reg[0]) {
+        reg_fwd_x<T>(Px, -em[0], s, x, A, h->pos, Mc, twx, L, M, N, h->nidx, conj_in, ncomp);
+    } else {
+        const int lg = pick_lg(Px, e[0] * NT, 16);
         const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
         const size_t sm = ((size_t)Px << lg) * sizeof(c2<T>);
-        MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e0, RS, hipLaunchKernelGGL((k_fwd_x<T, EM, RS>), g, dim3(NT), sm, s,
-                   x, A, h->pos, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, (op == OP_AN && transp) ? 1 : 0)));
-        MARIE_HIP(hipGetLastError());
+        MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e[0], RS, hipLaunchKernelGGL((k_fwd_x<T, EM, RS>), g, dim3(NT), sm,
+                   s, x, A, h->pos, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, conj_in)));
     }
+    MARIE_HIP(hipGetLastError());
     if (tm) MARIE_HIP(hipEventRecord(h->ev[1], s));
     // forward y: A[c][k][j<M][i] -> B[c][k][j<Py][i]
-    {
-        const int lg = pick_lg(Py, e1 * NT, txmax);
+    if (reg[1]) {
+        reg_strided<T>(false, Py, -em[1], s, A, B, twy, Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
+                       (int64_t)Py * Px, M, Py, (int64_t)N * M * Px, (int64_t)N * Py * Px, N, ncomp);
+    } else {
+        const int lg = pick_lg(Py, e[1] * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, ncomp);
         const size_t sm = ((size_t)Py << lg) * sizeof(c2<T>);
-        MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e1, RS, hipLaunchKernelGGL((k_strided<T, EM, false, RS>), g, dim3(NT),
-                   sm, s, A, B, twy, h->plan[1], Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px, (int64_t)Py * Px, M,
-                   Py, lg, (int64_t)N * M * Px, (int64_t)N * Py * Px)));
-        MARIE_HIP(hipGetLastError());
+        MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e[1], RS, hipLaunchKernelGGL((k_strided<T, EM, false, RS>), g,
+                   dim3(NT), sm, s, A, B, twy, h->plan[1], Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
+                   (int64_t)Py * Px, M, Py, lg, (int64_t)N * M * Px, (int64_t)N * Py * Px)));
     }
+    MARIE_HIP(hipGetLastError());
     if (tm) MARIE_HIP(hipEventRecord(h->ev[2], s));
     // z: fused forward, tensor product, inverse
-    {
-        const int lg = pick_lg(Pz, e2 * NT, txmax);
+    if (reg[2]) {
+        reg_z<T>(h->fN_oct, Pz, -em[2], s, B, fN, twz, Px, Py, N, nrhs);
+    } else {
+        const int lg = pick_lg(Pz, e[2] * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), Py, nrhs);
         const size_t sm = ((size_t)Pz << lg) * sizeof(c2<T>);
         if (h->fN_oct)
-            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e2, RS, hipLaunchKernelGGL((k_z<T, EM, RS, true>), g, dim3(NT), sm, s,
-                       B, fN, twz, h->plan[2], Px, Py, N, lg)));
+            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, true>), g, dim3(NT),
+                       sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg)));
         else
-            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e2, RS, hipLaunchKernelGGL((k_z<T, EM, RS, false>), g, dim3(NT), sm,
-                       s, B, fN, twz, h->plan[2], Px, Py, N, lg)));
-        MARIE_HIP(hipGetLastError());
+            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, false>), g, dim3(NT),
+                       sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg)));
     }
+    MARIE_HIP(hipGetLastError());
     if (tm) MARIE_HIP(hipEventRecord(h->ev[3], s));
     // inverse y: B[c][k][j<Py][i] -> A[c][k][j<M][i]
-    {
-        const int lg = pick_lg(Py, e3 * NT, txmax);
+    if (reg[3]) {
+        reg_strided<T>(true, Py, -em[3], s, B, A, twy, Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
+                       (int64_t)M * Px, Py, M, (int64_t)N * Py * Px, (int64_t)N * M * Px, N, ncomp);
+    } else {
+        const int lg = pick_lg(Py, e[3] * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, ncomp);
         const size_t sm = ((size_t)Py << lg) * sizeof(c2<T>);
-        MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e3, RS, hipLaunchKernelGGL((k_strided<T, EM, true, RS>), g, dim3(NT),
-                   sm, s, B, A, twy, h->plan[1], Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px, (int64_t)M * Px, Py,
-                   M, lg, (int64_t)N * Py * Px, (int64_t)N * M * Px)));
-        MARIE_HIP(hipGetLastError());
+        MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e[3], RS, hipLaunchKernelGGL((k_strided<T, EM, true, RS>), g,
+                   dim3(NT), sm, s, B, A, twy, h->plan[1], Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
+                   (int64_t)M * Px, Py, M, lg, (int64_t)N * Py * Px, (int64_t)N * M * Px)));
     }
+    MARIE_HIP(hipGetLastError());
     if (tm) MARIE_HIP(hipEventRecord(h->ev[4], s));
     // inverse x + crop + epilogue
     {
-        const int lg = pick_lg(Px, e4 * NT, 16);
-        const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
-        const size_t sm = ((size_t)Px << lg) * sizeof(c2<T>);
         const T scale = T(1.0 / ((double)Px * (double)Py * (double)Pz));
-        MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e4, RS, hipLaunchKernelGGL((k_inv_x<T, EM, RS>), g, dim3(NT), sm, s,
-                   A, y, x, h->pos, Mr, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, op, transp, (T)h->Gram, scale)));
+        if (reg[4]) {
+            reg_inv_x<T>(Px, -em[4], s, A, y, x, h->pos, Mr, Mc, twx, L, M, N, h->nidx, op, transp, (T)h->Gram, scale,
+                         ncomp);
+        } else {
+            const int lg = pick_lg(Px, e[4] * NT, 16);
+            const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
+            const size_t sm = ((size_t)Px << lg) * sizeof(c2<T>);
+            MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e[4], RS, hipLaunchKernelGGL((k_inv_x<T, EM, RS>), g, dim3(NT),
+                       sm, s, A, y, x, h->pos, Mr, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, op, transp,
+                       (T)h->Gram, scale)));
+        }
         MARIE_HIP(hipGetLastError());
     }
     if (tm) {
@@ -652,14 +680,25 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
     int pick[5] = {em[0], em[1], em[2], em[3], em[4]};
     const bool was = h->timing;
     int st = OK;
-    const int trials = nxy > nz ? nxy : nz;
-    for (int v = 0; v < trials && st == OK; ++v) {
-        const int exy = vxy[v < nxy ? v : nxy - 1];
-        const int ez = vz[v < nz ? v : nz - 1];
-        const int cand[5] = {exy, exy, ez, exy, exy};
-        // skip variants too small for the line lengths
+    // candidate configurations: the LDS-stage engine's tile variants, then the
+    // register engine's points-per-thread variants (power-of-two sides only)
+    int cands[8][5];
+    int nc = 0;
+    for (int v = 0; v < nxy; ++v) {
+        const int exy = vxy[v], ez = vz[v < nz ? v : nz - 1];
         if (h->Px > exy * NT || h->Py > exy * NT || h->Pz > ez * NT) continue;
-        for (int k = 0; k < 5; ++k) em[k] = cand[k];
+        const int c[5] = {exy, exy, ez, exy, exy};
+        for (int k = 0; k < 5; ++k) cands[nc][k] = c[k];
+        ++nc;
+    }
+    const int regE[3] = {4, 8, 16};
+    for (int v = 0; v < 3; ++v)
+        for (int k = 0; k < 5; ++k) cands[nc + v][k] = -regE[v];
+    nc += 3;
+    const bool verbose = getenv("MARIE_TUNE_VERBOSE") != nullptr;
+    for (int ci = 0; ci < nc && st == OK; ++ci) {
+        const int cj = ci;
+        for (int k = 0; k < 5; ++k) em[k] = cands[cj][k];
         h->timing = false;
         st = apply(h, OP_AN, 0, prec, x, y, 1, h->stream);      // warm-up
         if (st) break;
@@ -669,8 +708,15 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
         for (int it = 0; it < iters && st == OK; ++it) st = apply(h, OP_AN, 0, prec, x, y, 1, h->stream);
         for (int k = 0; k < 5 && st == OK; ++k) {
             const double ms = h->kernel_ms[k] / (h->kernel_calls ? h->kernel_calls : 1);
-            if (ms < best[k]) { best[k] = ms; pick[k] = cand[k]; }
+            if (ms < best[k]) {
+                best[k] = ms;
+                pick[k] = cands[cj][k];
+            }
         }
+        if (verbose)
+            fprintf(stderr, "[marie tune] prec %d cfg [%d %d %d %d %d] : %.4f %.4f %.4f %.4f %.4f ms\n",
+                    prec, em[0], em[1], em[2], em[3], em[4], h->kernel_ms[0] / iters, h->kernel_ms[1] / iters,
+                    h->kernel_ms[2] / iters, h->kernel_ms[3] / iters, h->kernel_ms[4] / iters);
     }
     for (int k = 0; k < 5; ++k) em[k] = pick[k];
     h->timing = was;

@@ -207,14 +207,16 @@ class Operator:
 
     # -- tiles ------------------------------------------------------------------
     def autotune(self, prec=PREC_F64, iters=5):
-        """Pick the fastest tile variant per kernel; returns (emax5, best_ms5)."""
-        em = (ctypes.c_int * 5)()
+        """Pick the fastest variant per kernel; returns (cfg6, best_ms5)."""
+        em = (ctypes.c_int * 6)()
         ms = (ctypes.c_double * 5)()
         check(load().marie_vie_autotune(self._h, prec, iters, em, ms))
         return list(em), list(ms)
 
-    def set_tiles(self, emax5, prec=PREC_F64):
-        em = (ctypes.c_int * 5)(*emax5)
+    def set_tiles(self, cfg, prec=PREC_F64):
+        """cfg: five per-kernel variants, optionally a sixth z-grouping flag."""
+        cfg = list(cfg) + [0] * (6 - len(cfg))
+        em = (ctypes.c_int * 6)(*cfg)
         check(load().marie_vie_set_tiles(self._h, prec, em))
 
     # -- timing -------------------------------------------------------------------

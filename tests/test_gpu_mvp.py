@@ -177,11 +177,19 @@ def test_error_paths():
         L_.Operator(4, 4, 4, (8, 8, 34))         # 34 = 2 * 17: prime factor above 13
 
 
-@pytest.mark.parametrize("prec,variants", [("f64", [4, 8, 16]), ("f32", [8, 16, 32])])
-def test_every_tile_variant(prec, variants):
-    """Each compiled tile shape gives the oracle's product (both fN storage forms)."""
+@pytest.mark.parametrize("prec,variants,shape", [
+    ("f64", [4, 8, 16, -4, -8, -16], (12, 10, 16)),   # LDS engine (+ register engine on the 2^k side)
+    ("f32", [8, 16, 32, -4, -8, -16], (12, 10, 16)),
+    ("f64", [-4, -8, -16, 8], (16, 8, 32)),          # register engine on every pass
+    ("f32", [-4, -8, -16, 16], (16, 8, 32)),
+])
+def test_every_tile_variant(prec, variants, shape):
+    """Each compiled tile shape gives the oracle's product (both fN storage forms).
+
+    Positive values select the LDS-stage engine (elements per thread), negative
+    values the register engine (points per thread) where the side is 2^k."""
     from marie_amd import lib as L_
-    L, M, N = 12, 10, 16
+    L, M, N = shape
     print("case tile variants", prec, flush=True)
     rng = np.random.default_rng(21)
     # parity-consistent operator -> octant storage; random operator -> full storage
@@ -202,9 +210,12 @@ def test_every_tile_variant(prec, variants):
         op = _op(L, M, N, None, fN, Mr, Mc, 0.8, idx, prec=p)
         yr = mvp.mv_AN(x, fN, Mr, Mc, 0.8, 'notransp', idx)
         for exy in variants:
-            for ez in variants[:2]:
-                op.set_tiles([exy, exy, ez, exy, exy], p)
-                assert _rel(op.apply(x), yr) <= tol, (exy, ez)
+            for ez in variants:
+              for zg in (0, 1):
+                op.set_tiles([exy, exy, ez, exy, exy, zg], p)
+                assert _rel(op.apply(x), yr) <= tol, (exy, ez, zg)
+                assert _rel(op.apply(x, L_.OP_AN, True),
+                            mvp.mv_AN(x, fN, Mr, Mc, 0.8, 'transp', idx)) <= tol, (exy, ez, zg)
         # round trip of the stored operator (octant expand or full copy)
         if prec == "f64":
             assert _rel(op.get_fN(), fN) <= 1e-15
