@@ -110,8 +110,11 @@ int marie_vie_apply_device(marie_vie_op* h, int op, int transp, int prec, const 
  * true residual ||b - A x||/||b||; resvec relative to ||b|| (capacity
  * `resvec_cap`, count in *nres); iter = {inner, outer}. Device buffers
  * b_dev, x_dev (complex128, nidx each; x_dev holds x0 on entry).
- * `mode`: 0 = plain FP64; 1 = complex64 matvecs inside FP64 GMRES with
- * iterative refinement on the FP64 residual (inner tolerance `inner_tol`). */
+ * `mode`: 0 = plain FP64; 1 = complex64 matvecs inside FP64 GMRES (FP64
+ * Krylov basis) with iterative refinement on the FP64 true residual; 2 = as 1
+ * with the Krylov basis stored in complex64 (dot products, norms, Givens
+ * rotations and the refinement residual still in FP64). Each refinement pass
+ * solves to max(inner_tol, 0.5*tol*||b||/||r||) relative to its residual. */
 int marie_vie_gmres_device(marie_vie_op* h, const void* b_dev, void* x_dev, int restart, double tol,
                            int maxit, int mode, double inner_tol, int* flag, double* relres,
                            int* iter, double* resvec, int resvec_cap, int* nres);

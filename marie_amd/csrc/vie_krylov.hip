@@ -25,7 +25,6 @@ constexpr int KT = 256;      // threads per block
 constexpr int KG = 1024;     // max blocks of a reduction
 constexpr int KV = 4;        // basis vectors per pass of the multi-dot
 
-__device__ __forceinline__ double2 ld2(const c2<double>* p) { return *reinterpret_cast<const double2*>(p); }
 
 template <int W>
 __device__ __forceinline__ void block_reduce(double (&v)[W], double* sh) {
@@ -53,28 +52,41 @@ __device__ __forceinline__ void block_reduce(double (&v)[W], double* sh) {
     __syncthreads();
 }
 
-// partial[(i) * G + blk] = sum over the block's range of conj(V_i) * w, i in [i0, i0+nv)
-__global__ __launch_bounds__(KT) void k_mdot(const c2<double>* __restrict__ V, int64_t ldv, int i0, int nv,
-                                             const c2<double>* __restrict__ w, int64_t n,
-                                             c2<double>* __restrict__ partial, int G) {
+// Vectors of the Krylov space are stored as c2<TV> (TV = double, or float for
+// the complex64-basis mode); every product and sum accumulates in double.
+template <typename TV> __device__ __forceinline__ double2 ldc(const c2<TV>* p) {
+    const c2<TV> v = *p;
+    return double2{(double)v.x, (double)v.y};
+}
+
+// partial[i * G + blk] = block sum of conj(V_i) * w over its range, i < k
+// (groups of KV basis vectors per sweep over the block's range of w)
+template <typename TV>
+__global__ __launch_bounds__(KT) void k_mdot(const c2<TV>* __restrict__ V, int64_t ldv, int k,
+                                             const c2<TV>* __restrict__ w, int64_t n, c2<double>* __restrict__ partial,
+                                             int G) {
     __shared__ double sh[KT / 64 * 2 * KV];
-    double acc[2 * KV];
+    for (int i0 = 0; i0 < k; i0 += KV) {
+        const int nv = (k - i0) < KV ? (k - i0) : KV;
+        double acc[2 * KV];
 #pragma unroll
-    for (int q = 0; q < 2 * KV; ++q) acc[q] = 0.0;
-    for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)G * KT) {
-        const double2 wv = ld2(w + e);
+        for (int q = 0; q < 2 * KV; ++q) acc[q] = 0.0;
+        for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)G * KT) {
+            const double2 wv = ldc(w + e);
 #pragma unroll
-        for (int q = 0; q < KV; ++q) {
-            if (q < nv) {
-                const double2 a = ld2(V + (i0 + q) * ldv + e);
-                acc[2 * q] += a.x * wv.x + a.y * wv.y;
-                acc[2 * q + 1] += a.x * wv.y - a.y * wv.x;
+            for (int q = 0; q < KV; ++q) {
+                if (q < nv) {
+                    const double2 a = ldc(V + (i0 + q) * ldv + e);
+                    acc[2 * q] += a.x * wv.x + a.y * wv.y;
+                    acc[2 * q + 1] += a.x * wv.y - a.y * wv.x;
+                }
             }
         }
+        block_reduce<2 * KV>(acc, sh);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < nv; ++q)
+                partial[(int64_t)(i0 + q) * G + blockIdx.x] = c2<double>{acc[2 * q], acc[2 * q + 1]};
     }
-    block_reduce<2 * KV>(acc, sh);
-    if (threadIdx.x == 0)
-        for (int q = 0; q < nv; ++q) partial[(int64_t)(i0 + q) * G + blockIdx.x] = c2<double>{acc[2 * q], acc[2 * q + 1]};
 }
 
 // out[i] = sum_b partial[i*G + b], fixed order; one block per i
@@ -91,11 +103,11 @@ __global__ __launch_bounds__(KT) void k_reduce(const c2<double>* __restrict__ pa
     if (threadIdx.x == 0) out[blockIdx.x] = c2<double>{acc[0], acc[1]};
 }
 
-// y = beta*y + sum_{i<k} V_i c_i ; optional partial ||y||^2 into npart[blk]
-__global__ __launch_bounds__(KT) void k_vcomb(c2<double>* __restrict__ y, double beta,
-                                              const c2<double>* __restrict__ V, int64_t ldv, int k,
-                                              const c2<double>* __restrict__ coef, double csign, int64_t n,
-                                              double* __restrict__ npart, int G) {
+// y = beta*y + csign * sum_{i<k} V_i c_i ; optional partial ||y||^2 into npart[blk]
+template <typename TY, typename TV>
+__global__ __launch_bounds__(KT) void k_vcomb(c2<TY>* __restrict__ y, double beta, const c2<TV>* __restrict__ V,
+                                              int64_t ldv, int k, const c2<double>* __restrict__ coef, double csign,
+                                              int64_t n, double* __restrict__ npart, int G) {
     __shared__ c2<double> cs[80];
     __shared__ double sh[KT / 64];
     for (int i = threadIdx.x; i < k; i += KT) cs[i] = coef[i];
@@ -104,18 +116,20 @@ __global__ __launch_bounds__(KT) void k_vcomb(c2<double>* __restrict__ y, double
     for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)G * KT) {
         double2 acc = {0.0, 0.0};
         if (beta != 0.0) {
-            const double2 yv = ld2(y + e);
+            const double2 yv = ldc(y + e);
             acc.x = beta * yv.x;
             acc.y = beta * yv.y;
         }
         for (int i = 0; i < k; ++i) {
-            const double2 a = ld2(V + i * ldv + e);
+            const double2 a = ldc(V + i * ldv + e);
             const c2<double> c = cs[i];
             acc.x += csign * (a.x * c.x - a.y * c.y);
             acc.y += csign * (a.x * c.y + a.y * c.x);
         }
-        y[e] = c2<double>{acc.x, acc.y};
-        nrm[0] += acc.x * acc.x + acc.y * acc.y;
+        const c2<TY> out = {(TY)acc.x, (TY)acc.y};
+        y[e] = out;
+        const double ox = (double)out.x, oy = (double)out.y;
+        nrm[0] += ox * ox + oy * oy;
     }
     if (npart) {
         block_reduce<1>(nrm, sh);
@@ -123,12 +137,13 @@ __global__ __launch_bounds__(KT) void k_vcomb(c2<double>* __restrict__ y, double
     }
 }
 
-__global__ __launch_bounds__(KT) void k_norm2(const c2<double>* __restrict__ v, int64_t n,
-                                              double* __restrict__ npart, int G) {
+template <typename TV>
+__global__ __launch_bounds__(KT) void k_norm2(const c2<TV>* __restrict__ v, int64_t n, double* __restrict__ npart,
+                                              int G) {
     __shared__ double sh[KT / 64];
     double acc[1] = {0.0};
     for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)G * KT) {
-        const double2 a = ld2(v + e);
+        const double2 a = ldc(v + e);
         acc[0] += a.x * a.x + a.y * a.y;
     }
     block_reduce<1>(acc, sh);
@@ -143,39 +158,31 @@ __global__ __launch_bounds__(KT) void k_reduce_real(const double* __restrict__ p
     if (threadIdx.x == 0) out[0] = acc[0];
 }
 
-// z = a*x + b*y (real scalars)
-__global__ void k_axpby(c2<double>* __restrict__ z, double a, const c2<double>* __restrict__ x, double b,
-                        const c2<double>* __restrict__ y, int64_t n) {
+// z = a*x + b*y (real scalars), any storage types
+template <typename TZ, typename TX, typename TY>
+__global__ void k_axpby(c2<TZ>* __restrict__ z, double a, const c2<TX>* __restrict__ x, double b,
+                        const c2<TY>* __restrict__ y, int64_t n) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-        const double2 xv = ld2(x + e), yv = ld2(y + e);
-        z[e] = c2<double>{a * xv.x + b * yv.x, a * xv.y + b * yv.y};
+        const double2 xv = ldc(x + e), yv = ldc(y + e);
+        z[e] = c2<TZ>{(TZ)(a * xv.x + b * yv.x), (TZ)(a * xv.y + b * yv.y)};
     }
-}
-
-__global__ void k_f64_to_f32(const c2<double>* __restrict__ s, c2<float>* __restrict__ d, int64_t n) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-        d[e] = c2<float>{(float)s[e].x, (float)s[e].y};
-}
-
-__global__ void k_f32_to_f64(const c2<float>* __restrict__ s, c2<double>* __restrict__ d, int64_t n) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
-        d[e] = c2<double>{(double)s[e].x, (double)s[e].y};
 }
 
 // ------------------------------------------------------------------ host side
 struct Ws {
-    c2<double>* V = nullptr;     // (m+1) x n
-    c2<double>* w = nullptr;     // n
-    c2<double>* r = nullptr;     // n
-    c2<double>* t = nullptr;     // n
-    c2<float>* xs = nullptr;     // n (mixed precision staging)
+    void* V = nullptr;           // (m+1) x n of c2<TV>
+    void* w = nullptr;           // n of c2<TV>
+    void* rT = nullptr;          // n of c2<TV> (inner residual)
+    c2<double>* r = nullptr;     // n, FP64 true residual
+    c2<double>* t = nullptr;     // n, FP64 scratch / correction
+    c2<float>* xs = nullptr;     // n (complex64 staging for FP64-basis mixed mode)
     c2<float>* ys = nullptr;
     c2<double>* part = nullptr;  // (m+1) x KG
-    c2<double>* h = nullptr;     // m+1
+    c2<double>* h = nullptr;     // m+2
     double* npart = nullptr;     // KG
     double* nout = nullptr;      // 1
     void free_all() {
-        void* p[] = {V, w, r, t, xs, ys, part, h, npart, nout};
+        void* p[] = {V, w, rT, r, t, xs, ys, part, h, npart, nout};
         for (void* q : p)
             if (q) (void)hipFree(q);
     }
@@ -187,7 +194,6 @@ struct Ctx {
     int64_t n;
     int G;
     Ws ws;
-    int mode;
 };
 
 static int grid_for(int64_t n) {
@@ -197,8 +203,8 @@ static int grid_for(int64_t n) {
     return (int)g;
 }
 
-static int norm(Ctx& c, const c2<double>* v, double* out) {
-    hipLaunchKernelGGL(k_norm2, dim3(c.G), dim3(KT), 0, c.s, v, c.n, c.ws.npart, c.G);
+template <typename TV> static int norm(Ctx& c, const c2<TV>* v, double* out) {
+    hipLaunchKernelGGL(k_norm2<TV>, dim3(c.G), dim3(KT), 0, c.s, v, c.n, c.ws.npart, c.G);
     hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, c.ws.nout);
     MARIE_HIP(hipGetLastError());
     double v2 = 0.0;
@@ -208,29 +214,41 @@ static int norm(Ctx& c, const c2<double>* v, double* out) {
     return OK;
 }
 
-// y = A x in FP64 (mode 0) or through complex64 (mode 1)
-static int matvec(Ctx& c, const c2<double>* x, c2<double>* y, bool f32) {
-    if (!f32) return apply(c.h, OP_AN, 0, PREC_F64, x, y, 1, c.s);
-    const int g = grid_for(c.n);
-    hipLaunchKernelGGL(k_f64_to_f32, dim3(g), dim3(KT), 0, c.s, x, c.ws.xs, c.n);
-    int st = apply(c.h, OP_AN, 0, PREC_F32, c.ws.xs, c.ws.ys, 1, c.s);
-    if (st) return st;
-    hipLaunchKernelGGL(k_f32_to_f64, dim3(g), dim3(KT), 0, c.s, c.ws.ys, y, c.n);
-    MARIE_HIP(hipGetLastError());
-    return OK;
+// y = A x with vectors stored as TV; `f32op` selects the complex64 operator
+template <typename TV> static int matvec(Ctx& c, const c2<TV>* x, c2<TV>* y, bool f32op) {
+    if constexpr (sizeof(TV) == 4) {
+        return apply(c.h, OP_AN, 0, PREC_F32, x, y, 1, c.s);
+    } else {
+        if (!f32op) return apply(c.h, OP_AN, 0, PREC_F64, x, y, 1, c.s);
+        const int g = grid_for(c.n);
+        hipLaunchKernelGGL((k_axpby<float, double, double>), dim3(g), dim3(KT), 0, c.s, c.ws.xs, 1.0, x, 0.0, x, c.n);
+        int st = apply(c.h, OP_AN, 0, PREC_F32, c.ws.xs, c.ws.ys, 1, c.s);
+        if (st) return st;
+        hipLaunchKernelGGL((k_axpby<double, float, float>), dim3(g), dim3(KT), 0, c.s, y, 1.0, c.ws.ys, 0.0, c.ws.ys,
+                           c.n);
+        MARIE_HIP(hipGetLastError());
+        return OK;
+    }
 }
 
-// One restart cycle of pgmres_iter on A z = r (r = current residual, device).
-// Updates x in place and r to the implicit residual. Returns iterations in *kout
-// and appends |LS residual| values to resv.
-static int gmres_cycle(Ctx& c, c2<double>* x, c2<double>* r, int m, double abstol, bool f32, int* kout,
+// One restart cycle of pgmres_iter on A z = r (r: current residual, stored as TV).
+// Accumulates the update into the FP64 vector x and leaves the implicit
+// residual V (rhs - H y) in r. Appends |LS residual| values to resv.
+template <typename TV>
+static int gmres_cycle(Ctx& c, c2<double>* x, c2<TV>* r, int m, double abstol, bool f32op, int* kout,
                        std::vector<double>& resv) {
     const int64_t n = c.n;
+    c2<TV>* V = reinterpret_cast<c2<TV>*>(c.ws.V);
+    c2<TV>* w = reinterpret_cast<c2<TV>*>(c.ws.w);
     double beta = 0.0;
-    int st = norm(c, r, &beta);
+    int st = norm<TV>(c, r, &beta);
     if (st) return st;
+    if (beta == 0.0) {
+        *kout = 0;
+        return OK;
+    }
     // V0 = r / beta
-    hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.V, 1.0 / beta, r, 0.0, r, n);
+    hipLaunchKernelGGL((k_axpby<TV, TV, TV>), dim3(c.G), dim3(KT), 0, c.s, V, 1.0 / beta, r, 0.0, r, n);
     std::vector<cd> H((size_t)(m + 1) * m, cd(0, 0));   // column-major, original Hessenberg
     std::vector<cd> R((size_t)(m + 1) * m, cd(0, 0));   // rotated
     std::vector<double> cs(m);
@@ -238,31 +256,27 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<double>* r, int m, double absto
     std::vector<cd> g(m + 1, cd(0, 0));
     g[0] = cd(beta, 0);
     std::vector<cd> hcol(m + 1);
+    std::vector<c2<double>> hh(m + 1);
     int k = 0;
     for (k = 1; k <= m; ++k) {
-        c2<double>* vk = c.ws.V + (int64_t)(k - 1) * n;
-        st = matvec(c, vk, c.ws.w, f32);
+        st = matvec<TV>(c, V + (int64_t)(k - 1) * n, w, f32op);
         if (st) return st;
-        // h = V(:,1:k)^H w  (CGS, pgmres_iter.m:106)
-        for (int i0 = 0; i0 < k; i0 += KV) {
-            const int nv = (k - i0) < KV ? (k - i0) : KV;
-            hipLaunchKernelGGL(k_mdot, dim3(c.G), dim3(KT), 0, c.s, c.ws.V, n, i0, nv, c.ws.w, n, c.ws.part, c.G);
-        }
+        // h = V(:,1:k)^H w  (classical Gram-Schmidt, pgmres_iter.m:106)
+        hipLaunchKernelGGL(k_mdot<TV>, dim3(c.G), dim3(KT), 0, c.s, V, n, k, w, n, c.ws.part, c.G);
         hipLaunchKernelGGL(k_reduce, dim3(k), dim3(KT), 0, c.s, c.ws.part, c.G, c.ws.h);
-        // w = w - V h, with ||w||^2 partials (pgmres_iter.m:107, :109)
-        hipLaunchKernelGGL(k_vcomb, dim3(c.G), dim3(KT), 0, c.s, c.ws.w, 1.0, c.ws.V, n, k, c.ws.h, -1.0, n,
+        // w = w - V h with ||w||^2 partials (pgmres_iter.m:107, :109)
+        hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, w, 1.0, V, n, k, c.ws.h, -1.0, n,
                            c.ws.npart, c.G);
         hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, c.ws.nout);
         MARIE_HIP(hipGetLastError());
-        std::vector<c2<double>> hh(k);
         double w2 = 0.0;
         MARIE_HIP(hipMemcpyAsync(hh.data(), c.ws.h, k * sizeof(c2<double>), hipMemcpyDeviceToHost, c.s));
         MARIE_HIP(hipMemcpyAsync(&w2, c.ws.nout, sizeof(double), hipMemcpyDeviceToHost, c.s));
         MARIE_HIP(hipStreamSynchronize(c.s));
         const double hk1 = std::sqrt(w2);
         // V(:,k+1) = w / H(k+1,k)  (pgmres_iter.m:110)
-        hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.V + (int64_t)k * n, 1.0 / hk1, c.ws.w, 0.0,
-                           c.ws.w, n);
+        hipLaunchKernelGGL((k_axpby<TV, TV, TV>), dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)k * n,
+                           hk1 > 0.0 ? 1.0 / hk1 : 0.0, w, 0.0, w, n);
         for (int i = 0; i < k; ++i) hcol[i] = cd(hh[i].x, hh[i].y);
         hcol[k] = cd(hk1, 0);
         for (int i = 0; i <= k; ++i) H[(size_t)(k - 1) * (m + 1) + i] = hcol[i];
@@ -282,9 +296,9 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<double>* r, int m, double absto
                 cs[k - 1] = 0.0;
                 sn[k - 1] = std::conj(b) / nb;
             } else {
-                const double t = std::hypot(na, nb);
-                cs[k - 1] = na / t;
-                sn[k - 1] = (a / na) * std::conj(b) / t;
+                const double tt = std::hypot(na, nb);
+                cs[k - 1] = na / tt;
+                sn[k - 1] = (a / na) * std::conj(b) / tt;
             }
             hcol[k - 1] = cs[k - 1] * a + sn[k - 1] * b;
             hcol[k] = cd(0, 0);
@@ -295,31 +309,31 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<double>* r, int m, double absto
         for (int i = 0; i < k; ++i) R[(size_t)(k - 1) * (m + 1) + i] = hcol[i];
         const double res = std::abs(g[k]);
         resv.push_back(res);
-        if (res < abstol) break;
+        if (res < abstol || hk1 == 0.0) break;
         if (k == m) break;
     }
     if (k > m) k = m;
     // y = R^{-1} g (upper triangular k x k)
     std::vector<cd> y(k);
     for (int i = k - 1; i >= 0; --i) {
-        cd s = g[i];
-        for (int j = i + 1; j < k; ++j) s -= R[(size_t)j * (m + 1) + i] * y[j];
-        y[i] = s / R[(size_t)i * (m + 1) + i];
+        cd sacc = g[i];
+        for (int j = i + 1; j < k; ++j) sacc -= R[(size_t)j * (m + 1) + i] * y[j];
+        y[i] = sacc / R[(size_t)i * (m + 1) + i];
     }
     // LS residual vector res = rhs - Hc y (pgmres_iter.m:119) -> r = V(:,1:k+1) res (:150)
-    std::vector<cd> resv_vec(k + 1, cd(0, 0));
-    resv_vec[0] = cd(beta, 0);
+    std::vector<cd> rv(k + 1, cd(0, 0));
+    rv[0] = cd(beta, 0);
     for (int j = 0; j < k; ++j)
-        for (int i = 0; i <= j + 1; ++i) resv_vec[i] -= H[(size_t)j * (m + 1) + i] * y[j];
+        for (int i = 0; i <= j + 1; ++i) rv[i] -= H[(size_t)j * (m + 1) + i] * y[j];
     std::vector<c2<double>> coef(k + 1);
     for (int i = 0; i < k; ++i) coef[i] = c2<double>{y[i].real(), y[i].imag()};
     MARIE_HIP(hipMemcpyAsync(c.ws.h, coef.data(), k * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
-    hipLaunchKernelGGL(k_vcomb, dim3(c.G), dim3(KT), 0, c.s, x, 1.0, c.ws.V, n, k, c.ws.h, 1.0, n,
+    hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, V, n, k, c.ws.h, 1.0, n,
                        (double*)nullptr, c.G);
     MARIE_HIP(hipStreamSynchronize(c.s));
-    for (int i = 0; i <= k; ++i) coef[i] = c2<double>{resv_vec[i].real(), resv_vec[i].imag()};
+    for (int i = 0; i <= k; ++i) coef[i] = c2<double>{rv[i].real(), rv[i].imag()};
     MARIE_HIP(hipMemcpyAsync(c.ws.h, coef.data(), (k + 1) * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
-    hipLaunchKernelGGL(k_vcomb, dim3(c.G), dim3(KT), 0, c.s, r, 0.0, c.ws.V, n, k + 1, c.ws.h, 1.0, n,
+    hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, r, 0.0, V, n, k + 1, c.ws.h, 1.0, n,
                        (double*)nullptr, c.G);
     MARIE_HIP(hipGetLastError());
     MARIE_HIP(hipStreamSynchronize(c.s));
@@ -327,8 +341,9 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<double>* r, int m, double absto
     return OK;
 }
 
-static int alloc_ws(Ctx& c, int m, bool f32) {
+static int alloc_ws(Ctx& c, int m, int mode) {
     const int64_t n = c.n;
+    const size_t ev = mode == 2 ? sizeof(c2<float>) : sizeof(c2<double>);
     auto A = [&](void** p, size_t bytes) -> int {
         if (hipMalloc(p, bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -338,17 +353,67 @@ static int alloc_ws(Ctx& c, int m, bool f32) {
         }
         return OK;
     };
-    int st = A((void**)&c.ws.V, (size_t)(m + 1) * n * sizeof(c2<double>));
-    if (!st) st = A((void**)&c.ws.w, n * sizeof(c2<double>));
+    int st = A(&c.ws.V, (size_t)(m + 1) * n * ev);
+    if (!st) st = A(&c.ws.w, n * ev);
+    if (!st && mode != 0) st = A(&c.ws.rT, n * ev);
     if (!st) st = A((void**)&c.ws.r, n * sizeof(c2<double>));
     if (!st) st = A((void**)&c.ws.t, n * sizeof(c2<double>));
-    if (!st && f32) st = A((void**)&c.ws.xs, n * sizeof(c2<float>));
-    if (!st && f32) st = A((void**)&c.ws.ys, n * sizeof(c2<float>));
+    if (!st && mode == 1) st = A((void**)&c.ws.xs, n * sizeof(c2<float>));
+    if (!st && mode == 1) st = A((void**)&c.ws.ys, n * sizeof(c2<float>));
     if (!st) st = A((void**)&c.ws.part, (size_t)(m + 1) * KG * sizeof(c2<double>));
     if (!st) st = A((void**)&c.ws.h, (m + 2) * sizeof(c2<double>));
     if (!st) st = A((void**)&c.ws.npart, KG * sizeof(double));
     if (!st) st = A((void**)&c.ws.nout, sizeof(double));
     return st;
+}
+
+// Iterative refinement around inner GMRES cycles with a complex64 operator,
+// basis stored as TV; the true residual b - A x is recomputed in FP64.
+template <typename TV>
+static int refine(Ctx& c, const c2<double>* b, c2<double>* x, int m, double tol, int maxit, double inner_tol,
+                  double bnorm, std::vector<double>& resvec, int* p_out, int* outit_out) {
+    c2<TV>* rT = reinterpret_cast<c2<TV>*>(c.ws.rT);
+    int outit = 0, p = 0, st = OK;
+    double rcur = resvec.back() * bnorm;
+    while (outit < maxit && rcur > tol * bnorm) {
+        ++outit;
+        // inner relative target: just enough to reach tol (with a 10 % margin for
+        // the complex64 operator's error), at least inner_tol
+        double rel = 0.9 * tol * bnorm / rcur;
+        if (inner_tol > rel) rel = inner_tol;
+        if (rel > 0.5) rel = 0.5;
+        MARIE_HIP(hipMemsetAsync(c.ws.t, 0, c.n * sizeof(c2<double>), c.s));
+        hipLaunchKernelGGL((k_axpby<TV, double, double>), dim3(c.G), dim3(KT), 0, c.s, rT, 1.0, c.ws.r, 0.0, c.ws.r,
+                           c.n);
+        std::vector<double> rv;
+        double inner_res = rcur;
+        int cycles = 0;
+        while (inner_res > rel * rcur && cycles < maxit) {
+            ++cycles;
+            std::vector<double> rvc;
+            st = gmres_cycle<TV>(c, c.ws.t, rT, m, rel * rcur, true, &p, rvc);
+            if (st) return st;
+            for (double v : rvc) rv.push_back(v);
+            if (rvc.empty()) break;
+            inner_res = rvc.back();
+        }
+        // x += d ; r = b - A64 x
+        hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, x, 1.0, c.ws.t,
+                           c.n);
+        st = matvec<double>(c, x, c.ws.t, false);
+        if (st) return st;
+        hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, c.ws.r, 1.0, b, -1.0,
+                           c.ws.t, c.n);
+        MARIE_HIP(hipGetLastError());
+        st = norm<double>(c, c.ws.r, &rcur);
+        if (st) return st;
+        // resvec: the inner LS residuals scaled to ||b||, then the refined true residual
+        for (double v : rv) resvec.push_back(v / bnorm);
+        resvec.push_back(rcur / bnorm);
+    }
+    *p_out = p;
+    *outit_out = outit;
+    return OK;
 }
 
 int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxit, int mode, double inner_tol,
@@ -358,13 +423,11 @@ int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxi
     c.s = h->stream;
     c.n = h->nidx;
     c.G = grid_for(c.n);
-    c.mode = mode;
-    const bool mixed = mode == 1;
-    int st = alloc_ws(c, m, mixed);
+    int st = alloc_ws(c, m, mode);
     if (st) { c.ws.free_all(); return st; }
     auto done = [&](int code) { (void)hipStreamSynchronize(c.s); c.ws.free_all(); return code; };
     double bnorm = 0.0;
-    st = norm(c, b, &bnorm);
+    st = norm<double>(c, b, &bnorm);
     if (st) return done(st);
     if (bnorm == 0.0) {
         // zero right-hand side: x = 0 (jvie2solver.m:30-33 fast return)
@@ -375,78 +438,47 @@ int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxi
     }
     // r = b - A x0 (x0 may be zero)
     double xn = 0.0;
-    st = norm(c, x, &xn);
+    st = norm<double>(c, x, &xn);
     if (st) return done(st);
     if (xn > 0.0) {
-        st = matvec(c, x, c.ws.t, false);
+        st = matvec<double>(c, x, c.ws.t, false);
         if (st) return done(st);
-        hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.r, 1.0, b, -1.0, c.ws.t, c.n);
+        hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, c.ws.r, 1.0, b, -1.0,
+                           c.ws.t, c.n);
     } else {
         MARIE_HIP(hipMemcpyAsync(c.ws.r, b, c.n * sizeof(c2<double>), hipMemcpyDeviceToDevice, c.s));
     }
     double rn = 0.0;
-    st = norm(c, c.ws.r, &rn);
+    st = norm<double>(c, c.ws.r, &rn);
     if (st) return done(st);
     resvec.assign(1, rn / bnorm);
     int outit = 0, p = 0;
-    if (!mixed) {
+    if (mode == 0) {
         // pgmres.m:153-167
         while (resvec.back() > tol && outit < maxit) {
             ++outit;
             std::vector<double> rv;
-            st = gmres_cycle(c, x, c.ws.r, m, tol * bnorm, false, &p, rv);
+            st = gmres_cycle<double>(c, x, c.ws.r, m, tol * bnorm, false, &p, rv);
             if (st) return done(st);
             for (double v : rv) resvec.push_back(v / bnorm);
+            if (rv.empty()) break;
         }
+    } else if (mode == 1) {
+        st = refine<double>(c, b, x, m, tol, maxit, inner_tol, bnorm, resvec, &p, &outit);
+        if (st) return done(st);
     } else {
-        // iterative refinement: true FP64 residual, complex64 correction solves
-        while (outit < maxit) {
-            ++outit;
-            // correction solve A d = r to relative inner tolerance, d accumulated in t
-            double rcur = 0.0;
-            st = norm(c, c.ws.r, &rcur);
-            if (st) return done(st);
-            double rel = inner_tol;
-            const double need = 0.5 * tol * bnorm / rcur;   // just enough to reach tol
-            if (need > rel) rel = need;
-            if (rel > 0.5) rel = 0.5;
-            MARIE_HIP(hipMemsetAsync(c.ws.t, 0, c.n * sizeof(c2<double>), c.s));
-            MARIE_HIP(hipMemcpyAsync(c.ws.w, c.ws.r, c.n * sizeof(c2<double>), hipMemcpyDeviceToDevice, c.s));
-            std::vector<double> rv;
-            // the inner cycles restart on their implicit residual, kept in r
-            double inner_res = rcur;
-            int cycles = 0;
-            while (inner_res > rel * rcur && cycles < maxit) {
-                ++cycles;
-                std::vector<double> rvc;
-                st = gmres_cycle(c, c.ws.t, c.ws.r, m, rel * rcur, true, &p, rvc);
-                if (st) return done(st);
-                for (double v : rvc) rv.push_back(v);
-                inner_res = rvc.empty() ? 0.0 : rvc.back();
-            }
-            // x += d ; r = b - A64 x
-            hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, x, 1.0, x, 1.0, c.ws.t, c.n);
-            st = matvec(c, x, c.ws.t, false);
-            if (st) return done(st);
-            hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.r, 1.0, b, -1.0, c.ws.t, c.n);
-            MARIE_HIP(hipGetLastError());
-            double rn2 = 0.0;
-            st = norm(c, c.ws.r, &rn2);
-            if (st) return done(st);
-            // resvec: the inner LS residuals scaled to ||b||, then the refined true residual
-            for (double v : rv) resvec.push_back(v / bnorm);
-            resvec.push_back(rn2 / bnorm);
-            if (rn2 / bnorm <= tol) break;
-        }
+        st = refine<float>(c, b, x, m, tol, maxit, inner_tol, bnorm, resvec, &p, &outit);
+        if (st) return done(st);
     }
     iter[0] = p;
     iter[1] = outit;
     // true residual (pgmres.m:175)
-    st = matvec(c, x, c.ws.t, false);
+    st = matvec<double>(c, x, c.ws.t, false);
     if (st) return done(st);
-    hipLaunchKernelGGL(k_axpby, dim3(c.G), dim3(KT), 0, c.s, c.ws.t, 1.0, b, -1.0, c.ws.t, c.n);
+    hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, c.ws.t, 1.0, b, -1.0, c.ws.t,
+                       c.n);
     double tn = 0.0;
-    st = norm(c, c.ws.t, &tn);
+    st = norm<double>(c, c.ws.t, &tn);
     if (st) return done(st);
     *relres = tn / bnorm;
     *flag = (*relres <= tol) ? 0 : 1;
@@ -464,12 +496,12 @@ extern "C" int marie_vie_gmres_device(marie_vie_op* hh, const void* b, void* x, 
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
     MARIE_HIP(hipSetDevice(h->device));
     if (!b || !x || !flag || !relres || !iter || restart < 1 || restart > 64 || maxit < 1 || !(tol > 0) ||
-        (mode != 0 && mode != 1)) {
-        set_error("invalid gmres arguments (restart must be 1..64)");
+        (mode < 0 || mode > 2)) {
+        set_error("invalid gmres arguments (restart must be 1..64, mode 0..2)");
         return MARIE_ERR_ARG;
     }
-    if (!(h->precisions & PREC_F64) || (mode == 1 && !(h->precisions & PREC_F32))) {
-        set_error("gmres needs an FP64 operator (and FP32 for mode 1)");
+    if (!(h->precisions & PREC_F64) || (mode != 0 && !(h->precisions & PREC_F32))) {
+        set_error("gmres needs an FP64 operator (and FP32 for modes 1 and 2)");
         return MARIE_ERR_STATE;
     }
     if (!h->have_fN || !h->have_mat || !h->have_idx) {

@@ -127,3 +127,30 @@ def test_gmres_zero_rhs_and_maxit_flag():
     assert fl == 0 and np.all(x == 0)
     x, fl, rel, it, rv = op.gmres(b, restart=2, tol=1e-12, maxit=1)
     assert fl == 1 and it[1] == 1
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_gmres_refined_modes_reach_tolerance(mode):
+    """complex64 matvecs (mode 1: FP64 basis, mode 2: complex64 basis) with FP64
+    refinement: true residual <= tol and the solution within the FP64 solve's
+    own error of a tightly converged FP64 reference."""
+    from marie_amd import lib
+    n, dx, f = 16, 6.25e-3, 298.2e6
+    print("case gmres refined mode", mode, flush=True)
+    r, e_r, Einc, em = _sphere_problem(n, dx, f)
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(Einc, e_r, dx, f, form=2)
+    op = lib.Operator(n, n, n, precisions=lib.PREC_F64 | lib.PREC_F32)
+    op.set_fN(fN)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx)
+    xref, fl, rel, _, _ = op.gmres(b, restart=50, tol=1e-11, maxit=200, mode=0)
+    x64, fl64, rel64, _, _ = op.gmres(b, restart=50, tol=1e-6, maxit=200, mode=0)
+    xm, flm, relm, itm, rvm = op.gmres(b, restart=50, tol=1e-6, maxit=50, mode=mode, inner_tol=1e-7)
+    assert fl == 0 and fl64 == 0 and flm == 0
+    assert relm <= 1e-6
+    e64 = _rel(x64, xref)
+    em_ = _rel(xm, xref)
+    assert em_ <= max(5 * e64, 1e-6), (em_, e64)
+    # resvec ends with the refined FP64 true residual
+    assert abs(rvm[-1] - relm) <= 1e-9
