@@ -59,6 +59,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--profile-only', action='store_true',
                     help='assemble and run the timed FP64 matvecs only (for rocprofv3 runs)')
+    ap.add_argument('--profile-solve', action='store_true',
+                    help='assemble, then one FP64 and one FP32-refined solve to 1e-6 (for rocprofv3 runs)')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='run a plain 1 GiB device copy five times (scale factor for FETCH/WRITE_SIZE)')
     args = ap.parse_args()
@@ -134,6 +136,18 @@ def main():
         tiles[name] = em
         log('tiles %s %s  (%s ms)' % (name, em, ' '.join('%.3f' % m for m in ms)))
 
+    if args.profile_solve:
+        b = torch.from_numpy(np.ascontiguousarray(Vrhs)).to(dev)
+        for mode in (0, 2):
+            xs = torch.zeros_like(b)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            flag, relres, it, rv = op.gmres_device(b.data_ptr(), xs.data_ptr(), restart=50, tol=1e-6, maxit=200,
+                                                   mode=mode, inner_tol=1e-7)
+            torch.cuda.synchronize()
+            log('mode %d solve %.3f s, %d iterations, relres %.2e' % (mode, time.perf_counter() - t,
+                                                                     len(rv) - 1, relres))
+        return
     K, W = args.steps, args.warmup
     T64 = timed(lib.PREC_F64, K, W)
     mv64 = K / T64

@@ -78,7 +78,10 @@ int marie_vie_get_fN(marie_vie_op* h, double* fN);
 
 /* Assemble fN on the device for voxel size dx and wavenumber ko
  * (getOPERATORS(r,f,'N'), getOPERATORS.m:47-48, :74-96; assembly_nop.m,
- * circulant_nop.m, fft_operator.m). The circulant must be 2L x 2M x 2N.
+ * circulant_nop.m, fft_operator.m). The reference's circulant is 2L x 2M x 2N;
+ * any side >= 2L-1 embeds the same Toeplitz operator (the Toeplitz block at
+ * the start, its reflection at the end, zeros between) and gives the same
+ * product, which lets callers choose sides with small prime factors.
  * `singular` = 10 doubles (re,im of I_ST, I_EAc, I_EAo, I_VAc, I_VAo as
  * returned by singular_st/ea/va.m) or NULL for the library's own
  * machine-precision self-terms. */

@@ -548,8 +548,10 @@ __global__ void k_to_f32(const c2<double>* __restrict__ s, c2<float>* __restrict
 
 int assemble_N(Op* h, double dx, double ko, const double* singular) {
     const int L = h->L, M = h->M, N = h->N;
-    if (h->Px != 2 * L || h->Py != 2 * M || h->Pz != 2 * N) {
-        set_error("assembly builds the 2L x 2M x 2N circulant of getOPERATORS");
+    // any circulant side >= 2L-1 embeds the same Toeplitz operator (circulant_nop.m
+    // uses exactly 2L); other sides let the caller pick FFT-friendly factors
+    if (h->Px < 2 * L - 1 || h->Py < 2 * M - 1 || h->Pz < 2 * N - 1) {
+        set_error("assembly needs a circulant of at least (2L-1) x (2M-1) x (2N-1)");
         return ERR_ARG;
     }
     if (L < 5 || M < 5 || N < 5) {

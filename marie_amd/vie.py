@@ -10,6 +10,8 @@ Same names, argument meaning and return values as the reference functions
   PlaneWave_Excitation(r, k, omega_mu, Eo)                      .../PlaneWave_Excitation.m:1
   JVIE_Solver(Einc, e_r, r, freq, fN, tol, form, solver, ...)   src_vie/src_solver/JVIE_Solver.m:1
   EMconstants(freq)                                             src_vie/src_solver/EMconstants.m
+  E_field_Nop(J, fN, Gram, freq, Einc)                          src_vie/src_fields/src_op/E_field_Nop.m:1
+  local_SAR / absorbed_power                                    src_vie/src_solver/Scat_Solver.m:179-228
 
 Every product runs in libmarie_vie.so on the GPU. `fN` may be the host array
 returned by getOPERATORS or an already-resident `lib.Operator`; device
@@ -90,8 +92,12 @@ def mv_L(JIn0, fN, r, Gram, idx, GPU_flag=1):
 
 
 def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operator=False,
-                 precisions=lib.PREC_F64):
+                 precisions=lib.PREC_F64, circulant_shape=None):
     """fN (2L x 2M x 2N x 6, complex128) assembled on the device.
+
+    circulant_shape=(Px, Py, Pz), each side >= 2L-1, embeds the same Toeplitz
+    operator in another circulant (e.g. 2*3*11 = 66 instead of 2*43 = 86 for
+    L = 43); the matvec result is the same.
 
     `singular` 'DIRECTFN' / 'DEMCEM' select the reference's self-term method by
     name; both are served by the library's own machine-precision quadrature.
@@ -109,11 +115,47 @@ def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operat
     dx = r[1, 0, 0, 0] - r[0, 0, 0, 0]
     ko = 2 * np.pi / C0 * f                    # getOPERATORS.m:48
     sv = None if isinstance(singular, str) else np.asarray(singular, np.complex128)
-    o = lib.Operator(L, M, N, precisions=precisions)
+    o = lib.Operator(L, M, N, circulant_shape, precisions=precisions)
     o.assemble_N(dx, ko, sv)
     if return_operator:
         return o
     return o.get_fN()
+
+
+def E_field_Nop(J, fN, Gram, freq, Einc=None, GPU_flag=1):
+    """Total field Einc + (N J - Gram J)/(Gram j omega eo) (E_field_Nop.m:49-84).
+
+    J is L x M x N x 3 (the currents JVIE_Solver returns); the product is
+    mv_L over every voxel (idx = 1:3LMN, E_field_Nop.m:75-76).
+    """
+    _require_gpu(GPU_flag)
+    J = np.asarray(J, complex)
+    if Einc is None:
+        Einc = np.zeros_like(J)
+    if not np.any(J):
+        return np.asarray(Einc, complex)
+    em = EMconstants(freq)
+    L, M, N, _ = J.shape
+    op = _operator(fN, (L, M, N))
+    op.set_material(np.ones((L, M, N)), np.zeros((L, M, N)), Gram)
+    op.set_index(np.arange(1, 3 * L * M * N + 1), one_based=True)
+    E = op.apply(J.ravel(order='F'), lib.OP_L).reshape(L, M, N, 3, order='F')
+    return Einc + E / (Gram * 1j * em['omega'] * em['eo'])
+
+
+def local_SAR(J, sigma_e, rho):
+    """|J|^2 / (2 sigma rho) per voxel, 0 where 2 sigma rho <= 1e-10 (Scat_Solver.m:179-186)."""
+    denom = 2 * np.asarray(sigma_e, float) * np.asarray(rho, float)
+    S = np.sum(np.abs(J) ** 2, axis=3)
+    out = np.zeros_like(S)
+    ok = np.abs(denom) > 1e-10
+    out[ok] = S[ok] / denom[ok]
+    return out
+
+
+def absorbed_power(E, J, Gram):
+    """Pabs = 0.5 Re sum(E . conj J) Gram (Scat_Solver.m:226-228)."""
+    return 0.5 * np.real(np.sum(E * np.conj(J))) * Gram
 
 
 def PlaneWave_Excitation(r, k, omega_mu, Eo):

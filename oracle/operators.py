@@ -243,21 +243,28 @@ def assembly_nop(L, M, N, ko, dx, method='own', ref=None):
     return G / (4.0 * np.pi)
 
 
-def circulant_nop(G):
-    """Embed the Toeplitz octant in a 2L x 2M x 2N circulant (circulant_nop.m:1-43)."""
+def circulant_nop(G, P=None):
+    """Embed the Toeplitz octant in a 2L x 2M x 2N circulant (circulant_nop.m:1-43).
+
+    With P = (Px, Py, Pz), Px >= 2L-1, the same embedding in a larger or
+    smaller circulant: the block, its signed reflection ending at Px-1, zeros
+    between (any such circulant contains the same Toeplitz operator).
+    """
     L, M, N, _ = G.shape
-    C = np.zeros((2 * L, 2 * M, 2 * N, 6), complex)
+    Px, Py, Pz = P or (2 * L, 2 * M, 2 * N)
+    C = np.zeros((Px, Py, Pz, 6), complex)
     rL = slice(L - 1, 0, -1)
     rM = slice(M - 1, 0, -1)
     rN = slice(N - 1, 0, -1)
+    xL, xM, xN = slice(Px - L + 1, Px), slice(Py - M + 1, Py), slice(Pz - N + 1, Pz)
     C[:L, :M, :N] = G
-    C[L + 1:, :M, :N] = G[rL, :, :] * GP_COEFF['L']
-    C[:L, M + 1:, :N] = G[:, rM, :] * GP_COEFF['M']
-    C[:L, :M, N + 1:] = G[:, :, rN] * GP_COEFF['N']
-    C[L + 1:, M + 1:, :N] = G[rL, rM, :] * GP_COEFF['LM']
-    C[L + 1:, :M, N + 1:] = G[rL, :, rN] * GP_COEFF['LN']
-    C[:L, M + 1:, N + 1:] = G[:, rM, rN] * GP_COEFF['MN']
-    C[L + 1:, M + 1:, N + 1:] = G[rL, rM, rN]
+    C[xL, :M, :N] = G[rL, :, :] * GP_COEFF['L']
+    C[:L, xM, :N] = G[:, rM, :] * GP_COEFF['M']
+    C[:L, :M, xN] = G[:, :, rN] * GP_COEFF['N']
+    C[xL, xM, :N] = G[rL, rM, :] * GP_COEFF['LM']
+    C[xL, :M, xN] = G[rL, :, rN] * GP_COEFF['LN']
+    C[:L, xM, xN] = G[:, rM, rN] * GP_COEFF['MN']
+    C[xL, xM, xN] = G[rL, rM, rN]
     return C
 
 

@@ -51,6 +51,33 @@ def test_assembly_parity(shape, dx, f):
     assert _rel(got2, ref2) <= 1e-12
 
 
+@pytest.mark.parametrize("P", [(15, 13, 12), (16, 18, 11)])
+def test_assembly_other_circulant(P):
+    """Sides other than 2L (>= 2L-1): same embedding rule, same matvec as 2L."""
+    from marie_amd import lib
+    L, M, N = 8, 7, 6
+    dx, f = 6.25e-3, 298.2e6
+    print("case assembly circulant", P, flush=True)
+    ko = 2 * np.pi / C0 * f
+    op = lib.Operator(L, M, N, P)
+    op.assemble_N(dx, ko)
+    G = oop.assembly_nop(L, M, N, ko, dx)
+    ref = oop.fft_operator(oop.circulant_nop(G, P))
+    assert _rel(op.get_fN(), ref) <= 1e-12
+    op2 = lib.Operator(L, M, N)
+    op2.assemble_N(dx, ko)
+    rng = np.random.default_rng(3)
+    e_r = 1 + rng.random((L, M, N)) * 10 - 1j * rng.random((L, M, N))
+    idx = np.arange(3 * L * M * N)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    ys = []
+    for o in (op, op2):
+        o.set_material(e_r, e_r - 1, dx ** 3)
+        o.set_index(idx)
+        ys.append(o.apply(x))
+    assert _rel(ys[0], ys[1]) <= 1e-12
+
+
 class _Given:
     def __init__(self, sv):
         self.sv = sv
