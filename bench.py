@@ -145,8 +145,8 @@ def main():
             flag, relres, it, rv = op.gmres_device(b.data_ptr(), xs.data_ptr(), restart=50, tol=1e-6, maxit=200,
                                                    mode=mode, inner_tol=1e-7)
             torch.cuda.synchronize()
-            log('mode %d solve %.3f s, %d iterations, relres %.2e' % (mode, time.perf_counter() - t,
-                                                                     len(rv) - 1, relres))
+            log('mode %d solve %.3f s, %d resvec entries, iter %s, relres %.2e' % (
+                mode, time.perf_counter() - t, len(rv) - 1, list(it), relres))
         return
     K, W = args.steps, args.warmup
     T64 = timed(lib.PREC_F64, K, W)

@@ -246,4 +246,34 @@ __device__ void tile_fft(c2<T>* buf, const c2<T>* tw, const AxisPlan& pl, int lg
         }                                                                  \
     } while (0)
 
+// Dispatch order pairing mirror images: 0, 1, n-1, 2, n-2, ..., n/2 (n even).
+__device__ __forceinline__ int mirror_order2(int y, int n) {
+    if (y == 0 || (n & 1)) return y;
+    if (y == n - 1) return n / 2;
+    const int p = (y + 1) >> 1;
+    return (y & 1) ? p : n - p;
+}
+
+// Tile of the z pass for workgroup `lid` of a 1D grid of gx * Py * nrhs tiles
+// (gx column blocks, Py rows, nrhs right-hand sides).
+//
+// The 8 XCDs of the chip each have their own L2, and workgroups are handed to
+// them round-robin (lid % 8). Consecutive logical tiles are given to one XCD:
+// XCD x runs the x-th contiguous range of tiles, in order. Within that order
+// the right-hand sides of a tile come first, then the column block and its
+// mirror (bx, gx-1-bx), then rows in the order 0, 1, Py-1, 2, Py-2, ...; so
+// the (up to 8 nrhs) tiles that read the same octant entries of fN run back
+// to back on one XCD and share its L2. Any lid -> tile map here is a
+// bijection, so the result does not depend on how the hardware dispatches.
+__device__ __forceinline__ void zpass_tile(int lid, int nt, int gx, int Py, int nrhs, int& bx, int& j, int& rhs) {
+    const int xcd = lid & 7, s = lid >> 3;
+    const int q = nt >> 3, r = nt & 7;
+    const int lam = xcd * q + (xcd < r ? xcd : r) + s;
+    rhs = lam % nrhs;
+    const int u = lam / nrhs;
+    const int v = u % gx;
+    bx = (v & 1) ? gx - 1 - (v >> 1) : (v >> 1);
+    j = mirror_order2(u / gx, Py);
+}
+
 }  // namespace marie

@@ -11,6 +11,14 @@
 // reference's from-scratch solve in exact arithmetic); the small dense
 // algebra runs on the host in FP64, the O(n) vector work on the device with
 // deterministic two-stage reductions (no atomics).
+//
+// The Arnoldi step needs nothing from the host: h, ||w|| and the next basis
+// vector stay on the device, and each Hessenberg column is copied to pinned
+// host memory behind an event. The host queues step k+1 before it waits for
+// column k, so the device never idles on the host's Givens update; when
+// column k meets the tolerance, the already queued step k+1 is discarded
+// (its only outputs are basis column k+1 and Hessenberg column k+1, which
+// the cycle's update never reads).
 #include "../../include/marie_vie.h"
 #include "vie_handle.hpp"
 
@@ -23,7 +31,7 @@ namespace marie {
 using cd = std::complex<double>;
 constexpr int KT = 256;      // threads per block
 constexpr int KG = 1024;     // max blocks of a reduction
-constexpr int KV = 4;        // basis vectors per pass of the multi-dot
+constexpr int KV = 8;        // basis vectors per pass of the multi-dot
 
 
 template <int W>
@@ -71,7 +79,22 @@ __global__ __launch_bounds__(KT) void k_mdot(const c2<TV>* __restrict__ V, int64
         double acc[2 * KV];
 #pragma unroll
         for (int q = 0; q < 2 * KV; ++q) acc[q] = 0.0;
-        for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)G * KT) {
+        const int64_t st = (int64_t)G * KT;
+        int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x;
+        // two elements per thread per trip: 2 * (nv + 1) independent loads in flight
+        for (; e + st < n; e += 2 * st) {
+            const double2 w0 = ldc(w + e), w1 = ldc(w + e + st);
+#pragma unroll
+            for (int q = 0; q < KV; ++q) {
+                if (q < nv) {
+                    const c2<TV>* col = V + (i0 + q) * ldv + e;
+                    const double2 a = ldc(col), b = ldc(col + st);
+                    acc[2 * q] += a.x * w0.x + a.y * w0.y + b.x * w1.x + b.y * w1.y;
+                    acc[2 * q + 1] += a.x * w0.y - a.y * w0.x + b.x * w1.y - b.y * w1.x;
+                }
+            }
+        }
+        if (e < n) {
             const double2 wv = ldc(w + e);
 #pragma unroll
             for (int q = 0; q < KV; ++q) {
@@ -158,6 +181,27 @@ __global__ __launch_bounds__(KT) void k_reduce_real(const double* __restrict__ p
     if (threadIdx.x == 0) out[0] = acc[0];
 }
 
+// *out = {sqrt(sum_b p[b]), 0}: the subdiagonal entry H(k+1,k) = ||w||
+__global__ __launch_bounds__(KT) void k_norm_entry(const double* __restrict__ p, int G, c2<double>* __restrict__ out) {
+    __shared__ double sh[KT / 64];
+    double acc[1] = {0.0};
+    for (int b = threadIdx.x; b < G; b += KT) acc[0] += p[b];
+    block_reduce<1>(acc, sh);
+    if (threadIdx.x == 0) *out = c2<double>{sqrt(acc[0]), 0.0};
+}
+
+// v = w / nrm->x (zero when the norm is zero: breakdown)
+template <typename TV>
+__global__ void k_scale_by_entry(c2<TV>* __restrict__ v, const c2<TV>* __restrict__ w,
+                                 const c2<double>* __restrict__ nrm, int64_t n) {
+    const double d = nrm->x;
+    const double a = d > 0.0 ? 1.0 / d : 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double2 x = ldc(w + e);
+        v[e] = c2<TV>{(TV)(a * x.x), (TV)(a * x.y)};
+    }
+}
+
 // z = a*x + b*y (real scalars), any storage types
 template <typename TZ, typename TX, typename TY>
 __global__ void k_axpby(c2<TZ>* __restrict__ z, double a, const c2<TX>* __restrict__ x, double b,
@@ -178,13 +222,18 @@ struct Ws {
     c2<float>* xs = nullptr;     // n (complex64 staging for FP64-basis mixed mode)
     c2<float>* ys = nullptr;
     c2<double>* part = nullptr;  // (m+1) x KG
-    c2<double>* h = nullptr;     // m+2
+    c2<double>* h = nullptr;     // (m+1) x (m+1): Hessenberg columns, then update coefficients
     double* npart = nullptr;     // KG
     double* nout = nullptr;      // 1
+    c2<double>* hp = nullptr;    // pinned host mirror of h
+    hipEvent_t ev[2] = {nullptr, nullptr};
     void free_all() {
         void* p[] = {V, w, rT, r, t, xs, ys, part, h, npart, nout};
         for (void* q : p)
             if (q) (void)hipFree(q);
+        if (hp) (void)hipHostFree(hp);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -249,37 +298,43 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<TV>* r, int m, double abstol, b
     }
     // V0 = r / beta
     hipLaunchKernelGGL((k_axpby<TV, TV, TV>), dim3(c.G), dim3(KT), 0, c.s, V, 1.0 / beta, r, 0.0, r, n);
-    std::vector<cd> H((size_t)(m + 1) * m, cd(0, 0));   // column-major, original Hessenberg
-    std::vector<cd> R((size_t)(m + 1) * m, cd(0, 0));   // rotated
+    const int ld = m + 1;                                // Hessenberg column stride (device and pinned)
+    std::vector<cd> H((size_t)ld * m, cd(0, 0));         // column-major, original Hessenberg
+    std::vector<cd> R((size_t)ld * m, cd(0, 0));         // rotated
     std::vector<double> cs(m);
     std::vector<cd> sn(m);
     std::vector<cd> g(m + 1, cd(0, 0));
     g[0] = cd(beta, 0);
     std::vector<cd> hcol(m + 1);
-    std::vector<c2<double>> hh(m + 1);
-    int k = 0;
-    for (k = 1; k <= m; ++k) {
-        st = matvec<TV>(c, V + (int64_t)(k - 1) * n, w, f32op);
-        if (st) return st;
+
+    // Arnoldi step k (1-based): w = A V(:,k), Hessenberg column k-1 on the
+    // device, V(:,k+1) = w / H(k+1,k), column copied to pinned memory + event
+    auto enqueue = [&](int k) -> int {
+        int st2 = matvec<TV>(c, V + (int64_t)(k - 1) * n, w, f32op);
+        if (st2) return st2;
+        c2<double>* hc = c.ws.h + (size_t)(k - 1) * ld;
         // h = V(:,1:k)^H w  (classical Gram-Schmidt, pgmres_iter.m:106)
         hipLaunchKernelGGL(k_mdot<TV>, dim3(c.G), dim3(KT), 0, c.s, V, n, k, w, n, c.ws.part, c.G);
-        hipLaunchKernelGGL(k_reduce, dim3(k), dim3(KT), 0, c.s, c.ws.part, c.G, c.ws.h);
+        hipLaunchKernelGGL(k_reduce, dim3(k), dim3(KT), 0, c.s, c.ws.part, c.G, hc);
         // w = w - V h with ||w||^2 partials (pgmres_iter.m:107, :109)
-        hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, w, 1.0, V, n, k, c.ws.h, -1.0, n,
+        hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, w, 1.0, V, n, k, hc, -1.0, n,
                            c.ws.npart, c.G);
-        hipLaunchKernelGGL(k_reduce_real, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, c.ws.nout);
-        MARIE_HIP(hipGetLastError());
-        double w2 = 0.0;
-        MARIE_HIP(hipMemcpyAsync(hh.data(), c.ws.h, k * sizeof(c2<double>), hipMemcpyDeviceToHost, c.s));
-        MARIE_HIP(hipMemcpyAsync(&w2, c.ws.nout, sizeof(double), hipMemcpyDeviceToHost, c.s));
-        MARIE_HIP(hipStreamSynchronize(c.s));
-        const double hk1 = std::sqrt(w2);
+        hipLaunchKernelGGL(k_norm_entry, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, hc + k);
         // V(:,k+1) = w / H(k+1,k)  (pgmres_iter.m:110)
-        hipLaunchKernelGGL((k_axpby<TV, TV, TV>), dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)k * n,
-                           hk1 > 0.0 ? 1.0 / hk1 : 0.0, w, 0.0, w, n);
-        for (int i = 0; i < k; ++i) hcol[i] = cd(hh[i].x, hh[i].y);
-        hcol[k] = cd(hk1, 0);
-        for (int i = 0; i <= k; ++i) H[(size_t)(k - 1) * (m + 1) + i] = hcol[i];
+        hipLaunchKernelGGL(k_scale_by_entry<TV>, dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)k * n, w, hc + k, n);
+        MARIE_HIP(hipGetLastError());
+        MARIE_HIP(hipMemcpyAsync(c.ws.hp + (size_t)(k - 1) * ld, hc, (k + 1) * sizeof(c2<double>),
+                                 hipMemcpyDeviceToHost, c.s));
+        MARIE_HIP(hipEventRecord(c.ws.ev[k & 1], c.s));
+        return OK;
+    };
+    // Givens update with column k-1 (Arnoldi step k); true when the cycle stops there
+    auto process = [&](int k, bool* stop) -> int {
+        MARIE_HIP(hipEventSynchronize(c.ws.ev[k & 1]));
+        const c2<double>* hh = c.ws.hp + (size_t)(k - 1) * ld;
+        for (int i = 0; i <= k; ++i) hcol[i] = cd(hh[i].x, hh[i].y);
+        const double hk1 = hh[k].x;
+        for (int i = 0; i <= k; ++i) H[(size_t)(k - 1) * ld + i] = hcol[i];
         // apply previous rotations, then a new one zeroing hcol[k]
         for (int i = 0; i < k - 1; ++i) {
             const cd a = hcol[i], b = hcol[i + 1];
@@ -306,34 +361,53 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<TV>* r, int m, double abstol, b
             g[k - 1] = cs[k - 1] * gk;
             g[k] = -std::conj(sn[k - 1]) * gk;
         }
-        for (int i = 0; i < k; ++i) R[(size_t)(k - 1) * (m + 1) + i] = hcol[i];
+        for (int i = 0; i < k; ++i) R[(size_t)(k - 1) * ld + i] = hcol[i];
         const double res = std::abs(g[k]);
         resv.push_back(res);
-        if (res < abstol || hk1 == 0.0) break;
-        if (k == m) break;
+        *stop = (res < abstol || hk1 == 0.0);
+        return OK;
+    };
+    int k = 0;
+    bool stop = false;
+    for (int kk = 1; kk <= m; ++kk) {
+        int st2 = enqueue(kk);
+        if (st2) return st2;
+        if (kk >= 2) {
+            st2 = process(kk - 1, &stop);
+            if (st2) return st2;
+            if (stop) { k = kk - 1; break; }
+        }
     }
-    if (k > m) k = m;
+    if (!stop) {
+        int st2 = process(m, &stop);
+        if (st2) return st2;
+        k = m;
+    }
     // y = R^{-1} g (upper triangular k x k)
     std::vector<cd> y(k);
     for (int i = k - 1; i >= 0; --i) {
         cd sacc = g[i];
-        for (int j = i + 1; j < k; ++j) sacc -= R[(size_t)j * (m + 1) + i] * y[j];
-        y[i] = sacc / R[(size_t)i * (m + 1) + i];
+        for (int j = i + 1; j < k; ++j) sacc -= R[(size_t)j * ld + i] * y[j];
+        y[i] = sacc / R[(size_t)i * ld + i];
     }
     // LS residual vector res = rhs - Hc y (pgmres_iter.m:119) -> r = V(:,1:k+1) res (:150)
     std::vector<cd> rv(k + 1, cd(0, 0));
     rv[0] = cd(beta, 0);
     for (int j = 0; j < k; ++j)
-        for (int i = 0; i <= j + 1; ++i) rv[i] -= H[(size_t)j * (m + 1) + i] * y[j];
-    std::vector<c2<double>> coef(k + 1);
-    for (int i = 0; i < k; ++i) coef[i] = c2<double>{y[i].real(), y[i].imag()};
-    MARIE_HIP(hipMemcpyAsync(c.ws.h, coef.data(), k * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
-    hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, V, n, k, c.ws.h, 1.0, n,
-                       (double*)nullptr, c.G);
+        for (int i = 0; i <= j + 1; ++i) rv[i] -= H[(size_t)j * ld + i] * y[j];
+    // coefficients through the two slots after the m Hessenberg columns
     MARIE_HIP(hipStreamSynchronize(c.s));
-    for (int i = 0; i <= k; ++i) coef[i] = c2<double>{rv[i].real(), rv[i].imag()};
-    MARIE_HIP(hipMemcpyAsync(c.ws.h, coef.data(), (k + 1) * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
-    hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, r, 0.0, V, n, k + 1, c.ws.h, 1.0, n,
+    c2<double>* cy = c.ws.hp + (size_t)m * ld;
+    c2<double>* cr = c.ws.hp + (size_t)m * ld + ld;
+    for (int i = 0; i < k; ++i) cy[i] = c2<double>{y[i].real(), y[i].imag()};
+    for (int i = 0; i <= k; ++i) cr[i] = c2<double>{rv[i].real(), rv[i].imag()};
+    c2<double>* dy = c.ws.h + (size_t)m * ld;
+    c2<double>* dr = c.ws.h + (size_t)m * ld + ld;
+    MARIE_HIP(hipMemcpyAsync(dy, cy, k * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
+    MARIE_HIP(hipMemcpyAsync(dr, cr, (k + 1) * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
+    hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, V, n, k, dy, 1.0, n,
+                       (double*)nullptr, c.G);
+    hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, r, 0.0, V, n, k + 1, dr, 1.0, n,
                        (double*)nullptr, c.G);
     MARIE_HIP(hipGetLastError());
     MARIE_HIP(hipStreamSynchronize(c.s));
@@ -361,14 +435,30 @@ static int alloc_ws(Ctx& c, int m, int mode) {
     if (!st && mode == 1) st = A((void**)&c.ws.xs, n * sizeof(c2<float>));
     if (!st && mode == 1) st = A((void**)&c.ws.ys, n * sizeof(c2<float>));
     if (!st) st = A((void**)&c.ws.part, (size_t)(m + 1) * KG * sizeof(c2<double>));
-    if (!st) st = A((void**)&c.ws.h, (m + 2) * sizeof(c2<double>));
+    // Hessenberg columns (m x (m+1)) + the two coefficient vectors of the update
+    const size_t nh = (size_t)(m + 2) * (m + 1);
+    if (!st) st = A((void**)&c.ws.h, nh * sizeof(c2<double>));
     if (!st) st = A((void**)&c.ws.npart, KG * sizeof(double));
     if (!st) st = A((void**)&c.ws.nout, sizeof(double));
+    if (!st && hipHostMalloc((void**)&c.ws.hp, nh * sizeof(c2<double>), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c.ws.hp = nullptr;
+        set_error("pinned host allocation failed");
+        st = ERR_ALLOC;
+    }
+    for (int e = 0; e < 2 && !st; ++e)
+        if (hipEventCreateWithFlags(&c.ws.ev[e], hipEventDisableTiming) != hipSuccess) {
+            set_error("hipEventCreate failed");
+            st = ERR_HIP;
+        }
     return st;
 }
 
-// Iterative refinement around inner GMRES cycles with a complex64 operator,
-// basis stored as TV; the true residual b - A x is recomputed in FP64.
+// Iterative refinement with one inner GMRES cycle per step: the cycle runs the
+// complex64 operator with its basis stored as TV on the current FP64 residual,
+// then x is updated and the true residual b - A x recomputed in FP64. So the
+// residual is replaced by the FP64 one at every restart, and the complex64
+// operator's error enters relative to the current residual, not to ||b||.
 template <typename TV>
 static int refine(Ctx& c, const c2<double>* b, c2<double>* x, int m, double tol, int maxit, double inner_tol,
                   double bnorm, std::vector<double>& resvec, int* p_out, int* outit_out) {
@@ -377,8 +467,9 @@ static int refine(Ctx& c, const c2<double>* b, c2<double>* x, int m, double tol,
     double rcur = resvec.back() * bnorm;
     while (outit < maxit && rcur > tol * bnorm) {
         ++outit;
-        // inner relative target: just enough to reach tol (with a 10 % margin for
-        // the complex64 operator's error), at least inner_tol
+        // the cycle stops at the global target (with a 10 % margin for the
+        // complex64 operator's error), at most inner_tol relative, and reduces
+        // the residual by at least a factor 2
         double rel = 0.9 * tol * bnorm / rcur;
         if (inner_tol > rel) rel = inner_tol;
         if (rel > 0.5) rel = 0.5;
@@ -386,17 +477,8 @@ static int refine(Ctx& c, const c2<double>* b, c2<double>* x, int m, double tol,
         hipLaunchKernelGGL((k_axpby<TV, double, double>), dim3(c.G), dim3(KT), 0, c.s, rT, 1.0, c.ws.r, 0.0, c.ws.r,
                            c.n);
         std::vector<double> rv;
-        double inner_res = rcur;
-        int cycles = 0;
-        while (inner_res > rel * rcur && cycles < maxit) {
-            ++cycles;
-            std::vector<double> rvc;
-            st = gmres_cycle<TV>(c, c.ws.t, rT, m, rel * rcur, true, &p, rvc);
-            if (st) return st;
-            for (double v : rvc) rv.push_back(v);
-            if (rvc.empty()) break;
-            inner_res = rvc.back();
-        }
+        st = gmres_cycle<TV>(c, c.ws.t, rT, m, rel * rcur, true, &p, rv);
+        if (st) return st;
         // x += d ; r = b - A64 x
         hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, x, 1.0, c.ws.t,
                            c.n);

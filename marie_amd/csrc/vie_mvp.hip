@@ -114,15 +114,15 @@ __global__ __launch_bounds__(NT) void k_strided(const c2<T>* __restrict__ src, c
 template <typename T, int EMAX, int RSET, bool OCT>
 __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __restrict__ fN,
                                           const c2<T>* __restrict__ tw, AxisPlan pl, int Px, int Py,
-                                          int N, int lg) {
+                                          int N, int lg, int nrhs) {
     extern __shared__ unsigned char smem_[];
     c2<T>* buf = reinterpret_cast<c2<T>*>(smem_);
     const int TX = 1 << lg;
     const int msk = TX - 1;
     const int P = pl.P;
-    const int i0 = blockIdx.x * TX;
-    const int j = blockIdx.y;
-    const int rhs = blockIdx.z;
+    int bx, j, rhs;
+    zpass_tile((int)blockIdx.x, (int)gridDim.x, (Px + TX - 1) / TX, Py, nrhs, bx, j, rhs);
+    const int i0 = bx * TX;
     const int64_t plane = (int64_t)Py * Px;
     const int64_t cstride = plane * N;
     const int nx = min(TX, Px - i0);
@@ -592,14 +592,14 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
         reg_z<T>(h->fN_oct, Pz, -em[2], s, B, fN, twz, Px, Py, N, nrhs);
     } else {
         const int lg = pick_lg(Pz, e[2] * NT, txmax);
-        const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), Py, nrhs);
+        const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg) * Py * nrhs);
         const size_t sm = ((size_t)Pz << lg) * sizeof(c2<T>);
         if (h->fN_oct)
             MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, true>), g, dim3(NT),
-                       sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg)));
+                       sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
         else
             MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, false>), g, dim3(NT),
-                       sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg)));
+                       sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
     }
     MARIE_HIP(hipGetLastError());
     if (tm) MARIE_HIP(hipEventRecord(h->ev[3], s));
@@ -654,6 +654,12 @@ int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs,
     if (!(h->precisions & prec)) {
         set_error("requested precision was not enabled at create");
         return ERR_STATE;
+    }
+    // the z pass runs one workgroup per (column tile, row, rhs) in a 1D grid;
+    // keep the grid's work-item count within 32 bits for the smallest tile
+    if ((int64_t)h->Px * h->Py * nrhs * NT >= (int64_t)1 << 32) {
+        set_error("too many right-hand sides for one product on this grid");
+        return ERR_ARG;
     }
     int st = ensure_work(h, nrhs);
     if (st) return st;
