@@ -116,8 +116,11 @@ int marie_vie_apply_device(marie_vie_op* h, int op, int transp, int prec, const 
  * `mode`: 0 = plain FP64; 1 = complex64 matvecs inside FP64 GMRES (FP64
  * Krylov basis) with iterative refinement on the FP64 true residual; 2 = as 1
  * with the Krylov basis stored in complex64 (dot products, norms, Givens
- * rotations and the refinement residual still in FP64). Each refinement pass
- * solves to max(inner_tol, 0.5*tol*||b||/||r||) relative to its residual. */
+ * rotations and the refinement residual still in FP64). In modes 1 and 2 each
+ * refinement step is one GMRES(restart) cycle on the FP64 residual r, stopped
+ * when its least-squares residual reaches max(inner_tol*||r||,
+ * 0.9*tol*||b||) (at most 0.5*||r||); x and r = b - A x are then updated in
+ * FP64, so the residual is replaced by the true one at every restart. */
 int marie_vie_gmres_device(marie_vie_op* h, const void* b_dev, void* x_dev, int restart, double tol,
                            int maxit, int mode, double inner_tol, int* flag, double* relres,
                            int* iter, double* resvec, int resvec_cap, int* nres);
@@ -129,15 +132,42 @@ int marie_vie_gmres(marie_vie_op* h, const double* b, double* x, int restart, do
                     int mode, double inner_tol, int* flag, double* relres, int* iter, double* resvec,
                     int resvec_cap, int* nres);
 
+/* Block GMRES for `nrhs` right-hand sides of the operator (device buffers B,
+ * X: nrhs stacked compact vectors of nidx complex128 each; X holds X0 on
+ * entry). The reference solves excitation ports one pgmres at a time
+ * (VSIE_Solver.m:387-407, fVIEsolver_Assembly.m:185-197); here one block
+ * Krylov space serves all columns: restarted block GMRES with `restart`
+ * block steps per cycle, two-pass block Gram-Schmidt, shifted CholeskyQR3
+ * for the block QR, a Householder least-squares update per step, and the
+ * FP64 true residual B - A X at every restart. Column q converges when
+ * ||b_q - A x_q|| <= tol ||b_q|| (flag[q], relres[q] as pgmres.m:175).
+ * mode 0 = complex128 products; mode 1 = complex64 products, FP64 basis and
+ * FP64 residual replacement (each cycle stops at max(inner_tol*||r_q||,
+ * 0.9*tol*||b_q||) per column). use_graph = 1 replays each block step's
+ * multi-column product from a HIP graph captured on the cycle that first
+ * reaches it (not on the legacy default stream). iter = {block steps of the
+ * last cycle, cycles}; resvec gets the worst column's relative residual per
+ * step and, after each cycle, its true value. Limits: nrhs <= 64,
+ * (restart+1)*nrhs <= 2048 basis vectors. */
+int marie_vie_block_gmres_device(marie_vie_op* h, const void* B_dev, void* X_dev, int nrhs, int restart,
+                                 double tol, int maxit, int mode, double inner_tol, int use_graph,
+                                 int* flag, double* relres, int* iter, double* resvec, int resvec_cap,
+                                 int* nres);
+
+/* Same with host buffers B, X (nrhs stacked compact vectors, complex128). */
+int marie_vie_block_gmres(marie_vie_op* h, const double* B, double* X, int nrhs, int restart, double tol,
+                          int maxit, int mode, double inner_tol, int use_graph, int* flag, double* relres,
+                          int* iter, double* resvec, int resvec_cap, int* nres);
+
 /* Tile autotuner: times every compiled variant of the five matvec kernels
  * with HIP events on the handle's stream, `iters` products each, and keeps
  * the fastest per kernel for precision `prec`. A variant is cfg6[0..4], one
  * value per kernel (fwd_x, fwd_y, z, inv_y, inv_x): v > 0 selects the
  * LDS-stage FFT with v elements per thread, v < 0 the register-resident FFT
- * with -v points per thread (power-of-two sides); cfg6[5] = 1 makes one
- * workgroup of the z kernel process all mirror columns of an octant-stored
- * fN. cfg6 / best_ms5 (optional) receive the choice and its mean kernel
- * times. No counterpart in the reference (MATLAB's fftn plans internally). */
+ * with -v points per thread (power-of-two sides); cfg6[5] is reserved and
+ * returned as 0. cfg6 / best_ms5 (optional) receive the choice and its mean
+ * kernel times. No counterpart in the reference (MATLAB's fftn plans
+ * internally). */
 int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* cfg6, double* best_ms5);
 
 /* Set the kernel variants explicitly (cfg6 as above). */

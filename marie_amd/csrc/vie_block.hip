@@ -1,0 +1,634 @@
+// Restarted block GMRES for s right-hand sides of one operator (coil ports,
+// excitations): one Krylov space of block vectors, one s-column product per
+// block step (the fN reads of the z pass are shared by the s columns).
+//
+// The reference solves ports one at a time (VSIE_Solver.m:387-407,
+// fVIEsolver_Assembly.m:185-197 build one pgmres per column); this is the
+// batched counterpart, with per-column flag/relres semantics of pgmres.m:175.
+//
+// Per cycle, on R = B - A X (FP64):
+//   R = V_0 S_0                       shifted CholeskyQR3 (Gram on device, s x s Cholesky on host)
+//   block step j: W = A V_j           multi-RHS product (complex128, or complex64 in mode 1)
+//                 H_{0:j,j} = V^H W, W -= V H      block classical Gram-Schmidt, two passes
+//                 W = V_{j+1} H_{j+1,j}            shifted CholeskyQR3
+//   least squares min || E_1 S_0 - Hbar Y ||  by Householder QR of the block
+//   Hessenberg matrix, updated one block column per step on the host;
+//   X += V Y; the true residual B - A X (FP64) starts the next cycle.
+// The tall-skinny products V^H W, V H and W R^-1 are plain complex GEMMs
+// and go to rocBLAS; the multi-RHS product is this library's FFT path, and
+// may be replayed from HIP graphs captured per block step.
+#include "../../include/marie_vie.h"
+#include "vie_handle.hpp"
+
+#include <rocblas/rocblas.h>
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+namespace marie {
+
+using cd = std::complex<double>;
+constexpr int BT = 256;
+constexpr int BG = 1024;
+
+namespace {
+
+template <typename TO, typename TI>
+__global__ void k_convert(c2<TO>* __restrict__ o, const c2<TI>* __restrict__ in, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const c2<TI> v = in[e];
+        o[e] = c2<TO>{(TO)v.x, (TO)v.y};
+    }
+}
+
+// r = b - t
+__global__ void k_sub(c2<double>* __restrict__ r, const c2<double>* __restrict__ b, const c2<double>* __restrict__ t,
+                      int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        r[e] = c2<double>{b[e].x - t[e].x, b[e].y - t[e].y};
+}
+
+// x += d
+__global__ void k_add(c2<double>* __restrict__ x, const c2<double>* __restrict__ d, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        x[e] = c2<double>{x[e].x + d[e].x, x[e].y + d[e].y};
+}
+
+// part[col * G + blk] = block sum of |v_col|^2 (grid: G x s)
+__global__ __launch_bounds__(BT) void k_colnorm2(const c2<double>* __restrict__ v, int64_t n,
+                                                 double* __restrict__ part, int G) {
+    __shared__ double sh[BT / 64];
+    const c2<double>* col = v + (int64_t)blockIdx.y * n;
+    double s = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)BT + threadIdx.x; e < n; e += (int64_t)G * BT)
+        s += col[e].x * col[e].x + col[e].y * col[e].y;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < BT / 64; ++w) t += sh[w];
+        part[(int64_t)blockIdx.y * G + blockIdx.x] = t;
+    }
+}
+
+// out[col] = sum_b part[col * G + b], fixed order (one block per column)
+__global__ __launch_bounds__(BT) void k_colsum(const double* __restrict__ part, int G, double* __restrict__ out) {
+    __shared__ double sh[BT / 64];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += BT) s += part[(int64_t)blockIdx.x * G + b];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < BT / 64; ++w) t += sh[w];
+        out[blockIdx.x] = t;
+    }
+}
+
+int grid_of(int64_t n) {
+    int64_t g = (n + BT * 4 - 1) / (BT * 4);
+    return (int)(g < 1 ? 1 : (g > BG ? BG : g));
+}
+
+struct BlockWs {
+    c2<double>* V = nullptr;     // n x (m+1)s
+    c2<double>* W = nullptr;     // n x s
+    c2<double>* T = nullptr;     // n x s (CholeskyQR staging, products)
+    c2<double>* R = nullptr;     // n x s (true residual)
+    c2<double>* D = nullptr;     // n x s (cycle correction)
+    c2<float>* Xs = nullptr;     // n x s complex64 staging (mode 1)
+    c2<float>* Ys = nullptr;
+    c2<double>* Hd = nullptr;    // (m+1)s x s block column / Gram / small factors
+    c2<double>* Sd = nullptr;    // (m+1)s x s small matrix upload
+    double* part = nullptr;      // s x BG
+    double* nrm = nullptr;       // s
+    std::vector<hipGraphExec_t> graphs;
+    std::vector<hipGraph_t> gsrc;
+    void free_all() {
+        void* p[] = {V, W, T, R, D, Xs, Ys, Hd, Sd, part, nrm};
+        for (void* q : p)
+            if (q) (void)hipFree(q);
+        for (auto g : graphs)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (auto g : gsrc)
+            if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+struct BCtx {
+    Op* h;
+    hipStream_t s;
+    rocblas_handle blas;
+    int64_t n;
+    int ns;                      // block size s
+    int G;
+    BlockWs ws;
+};
+
+#define MARIE_BLAS(call)                                                        \
+    do {                                                                        \
+        rocblas_status b_ = (call);                                             \
+        if (b_ != rocblas_status_success) {                                     \
+            ::marie::set_error(std::string(#call) + ": " + rocblas_status_to_string(b_)); \
+            return ::marie::ERR_HIP;                                            \
+        }                                                                       \
+    } while (0)
+
+inline rocblas_double_complex* zc(c2<double>* p) { return reinterpret_cast<rocblas_double_complex*>(p); }
+inline const rocblas_double_complex* zc(const c2<double>* p) {
+    return reinterpret_cast<const rocblas_double_complex*>(p);
+}
+
+// C (m x nn, ldc) = alpha * op(A) * B + beta * C, column-major
+int zgemm(BCtx& c, bool conjA, int m, int nn, int64_t k, cd alpha, const c2<double>* A, int64_t lda,
+          const c2<double>* B, int64_t ldb, cd beta, c2<double>* C, int64_t ldc) {
+    const rocblas_double_complex al(alpha.real(), alpha.imag()), be(beta.real(), beta.imag());
+    MARIE_BLAS(rocblas_zgemm(c.blas, conjA ? rocblas_operation_conjugate_transpose : rocblas_operation_none,
+                             rocblas_operation_none, m, nn, (rocblas_int)k, &al, zc(A), (rocblas_int)lda, zc(B),
+                             (rocblas_int)ldb, &be, zc(C), (rocblas_int)ldc));
+    return OK;
+}
+
+// per-column 2-norms of an n x s block (host result)
+int colnorms(BCtx& c, const c2<double>* v, std::vector<double>& out) {
+    hipLaunchKernelGGL(k_colnorm2, dim3(c.G, c.ns), dim3(BT), 0, c.s, v, c.n, c.ws.part, c.G);
+    hipLaunchKernelGGL(k_colsum, dim3(c.ns), dim3(BT), 0, c.s, c.ws.part, c.G, c.ws.nrm);
+    MARIE_HIP(hipGetLastError());
+    out.assign(c.ns, 0.0);
+    MARIE_HIP(hipMemcpyAsync(out.data(), c.ws.nrm, c.ns * sizeof(double), hipMemcpyDeviceToHost, c.s));
+    MARIE_HIP(hipStreamSynchronize(c.s));
+    for (double& x : out) x = std::sqrt(x);
+    return OK;
+}
+
+// Hermitian positive definite s x s (column-major) -> upper R with A = R^H R;
+// false when A is not numerically positive definite
+bool cholesky_upper(const std::vector<cd>& A, int s, std::vector<cd>& R) {
+    R.assign((size_t)s * s, cd(0, 0));
+    double amax = 0.0;
+    for (int i = 0; i < s; ++i) amax = std::max(amax, A[(size_t)i * s + i].real());
+    for (int j = 0; j < s; ++j) {
+        cd d = A[(size_t)j * s + j];
+        for (int k = 0; k < j; ++k) d -= std::conj(R[(size_t)j * s + k]) * R[(size_t)j * s + k];
+        if (!(d.real() > 1e-28 * amax) || !std::isfinite(d.real())) return false;
+        const double rjj = std::sqrt(d.real());
+        R[(size_t)j * s + j] = rjj;
+        for (int i = j + 1; i < s; ++i) {
+            cd v = A[(size_t)i * s + j];   // A(j, i)
+            for (int k = 0; k < j; ++k) v -= std::conj(R[(size_t)j * s + k]) * R[(size_t)i * s + k];
+            R[(size_t)i * s + j] = v / rjj;   // R(j, i)
+        }
+    }
+    return true;
+}
+
+// inverse of an upper triangular s x s matrix (column-major)
+std::vector<cd> upper_inverse(const std::vector<cd>& R, int s) {
+    std::vector<cd> X((size_t)s * s, cd(0, 0));
+    for (int j = 0; j < s; ++j) {
+        X[(size_t)j * s + j] = 1.0 / R[(size_t)j * s + j];
+        for (int i = j - 1; i >= 0; --i) {
+            cd v = 0;
+            for (int k = i + 1; k <= j; ++k) v += R[(size_t)k * s + i] * X[(size_t)j * s + k];
+            X[(size_t)j * s + i] = -v / R[(size_t)i * s + i];
+        }
+    }
+    return X;
+}
+
+// Shifted CholeskyQR3: W = out Rf with out orthonormal (out != W). The first
+// pass factors the Gram matrix plus a small shift (11 (n s + s(s+1)) u per
+// unit diagonal), which always succeeds and leaves a basis of condition
+// number O(u^-1/2); two plain passes then restore orthonormality. Each pass
+// equilibrates the Gram matrix to unit diagonal first, so columns of very
+// different norms (a nearly converged right-hand side next to others) lose
+// nothing. Returns ERR_STATE (no error text) for a zero or non-finite column.
+int cholqr(BCtx& c, const c2<double>* W, c2<double>* out, std::vector<cd>& Rf) {
+    const int s = c.ns;
+    std::vector<cd> g((size_t)s * s);
+    std::vector<std::vector<cd>> Rp(3);
+    const double shift = 11.0 * ((double)c.n * s + (double)s * (s + 1)) * 1.1102230246251565e-16 * s;
+    const c2<double>* src = W;
+    c2<double>* dsts[3] = {out, c.ws.T, out};
+    for (int pass = 0; pass < 3; ++pass) {
+        int st = zgemm(c, true, s, s, c.n, 1.0, src, c.n, src, c.n, 0.0, c.ws.Hd, s);
+        if (st) return st;
+        MARIE_HIP(hipMemcpyAsync(g.data(), c.ws.Hd, (size_t)s * s * sizeof(c2<double>), hipMemcpyDeviceToHost, c.s));
+        MARIE_HIP(hipStreamSynchronize(c.s));
+        std::vector<double> d(s);
+        for (int i = 0; i < s; ++i) {
+            d[i] = std::sqrt(g[(size_t)i * s + i].real());
+            if (!(d[i] > 0.0) || !std::isfinite(d[i])) return ERR_STATE;
+        }
+        for (int jj = 0; jj < s; ++jj)
+            for (int i = 0; i < s; ++i) g[(size_t)jj * s + i] /= d[i] * d[jj];
+        if (pass == 0)
+            for (int i = 0; i < s; ++i) g[(size_t)i * s + i] += shift;
+        std::vector<cd>& R = Rp[pass];
+        if (!cholesky_upper(g, s, R)) return ERR_STATE;
+        for (int jj = 0; jj < s; ++jj)
+            for (int i = 0; i <= jj; ++i) R[(size_t)jj * s + i] *= d[jj];
+        const std::vector<cd> Ri = upper_inverse(R, s);
+        MARIE_HIP(hipMemcpyAsync(c.ws.Sd, Ri.data(), (size_t)s * s * sizeof(c2<double>), hipMemcpyHostToDevice,
+                                 c.s));
+        st = zgemm(c, false, (int)c.n, s, s, 1.0, src, c.n, c.ws.Sd, s, 0.0, dsts[pass], c.n);
+        if (st) return st;
+        MARIE_HIP(hipStreamSynchronize(c.s));   // Ri (host memory) goes out of scope
+        src = dsts[pass];
+    }
+    // Rf = R3 R2 R1
+    auto mul_upper = [s](const std::vector<cd>& A, const std::vector<cd>& B) {
+        std::vector<cd> C((size_t)s * s, cd(0, 0));
+        for (int j = 0; j < s; ++j)
+            for (int i = 0; i <= j; ++i) {
+                cd v = 0;
+                for (int k = i; k <= j; ++k) v += A[(size_t)k * s + i] * B[(size_t)j * s + k];
+                C[(size_t)j * s + i] = v;
+            }
+        return C;
+    };
+    Rf = mul_upper(Rp[2], mul_upper(Rp[1], Rp[0]));
+    return OK;
+}
+
+// W = A V_j through the FFT path, in complex128 or (mode 1) complex64
+int block_matvec(BCtx& c, const c2<double>* Vj, c2<double>* W, bool f32, int j, bool use_graph) {
+    const int64_t ns = c.n * c.ns;
+    auto body = [&]() -> int {
+        if (!f32) return apply(c.h, OP_AN, 0, PREC_F64, Vj, W, c.ns, c.s);
+        const int g = grid_of(ns);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(BT), 0, c.s, c.ws.Xs, Vj, ns);
+        int st = apply(c.h, OP_AN, 0, PREC_F32, c.ws.Xs, c.ws.Ys, c.ns, c.s);
+        if (st) return st;
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(BT), 0, c.s, W, c.ws.Ys, ns);
+        MARIE_HIP(hipGetLastError());
+        return OK;
+    };
+    if (!use_graph || j < 0) return body();
+    if ((int)c.ws.graphs.size() <= j) {
+        c.ws.graphs.resize(j + 1, nullptr);
+        c.ws.gsrc.resize(j + 1, nullptr);
+    }
+    if (!c.ws.graphs[j]) {
+        MARIE_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+        const int st = body();
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(c.s, &gr);
+        if (st) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return st;
+        }
+        if (e != hipSuccess) {
+            set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            return ERR_HIP;
+        }
+        c.ws.gsrc[j] = gr;
+        MARIE_HIP(hipGraphInstantiate(&c.ws.graphs[j], gr, nullptr, nullptr, 0));
+    }
+    MARIE_HIP(hipGraphLaunch(c.ws.graphs[j], c.s));
+    return OK;
+}
+
+int alloc_block(BCtx& c, int m, int mode) {
+    const int64_t n = c.n, s = c.ns;
+    auto A = [&](void** p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            set_error("out of device memory for the block Krylov workspace");
+            return ERR_ALLOC;
+        }
+        return OK;
+    };
+    int st = A((void**)&c.ws.V, (size_t)(m + 1) * s * n * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.W, (size_t)s * n * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.T, (size_t)s * n * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.R, (size_t)s * n * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.D, (size_t)s * n * sizeof(c2<double>));
+    if (!st && mode == 1) st = A((void**)&c.ws.Xs, (size_t)s * n * sizeof(c2<float>));
+    if (!st && mode == 1) st = A((void**)&c.ws.Ys, (size_t)s * n * sizeof(c2<float>));
+    if (!st) st = A((void**)&c.ws.Hd, (size_t)(m + 1) * s * s * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.Sd, (size_t)(m + 1) * s * s * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.part, (size_t)s * BG * sizeof(double));
+    if (!st) st = A((void**)&c.ws.nrm, (size_t)s * sizeof(double));
+    return st;
+}
+
+// Householder QR of the block Hessenberg matrix, one block column at a time.
+struct BlockLS {
+    int s, m, ld;                      // ld = (m+1)s rows
+    std::vector<cd> H;                 // ld x ms, column-major (original)
+    std::vector<cd> Rq;                // transformed copy
+    std::vector<cd> G;                 // ld x s right-hand side (transformed)
+    std::vector<std::vector<cd>> hv;   // reflector of column c (length s+1), rows c..c+s
+    std::vector<double> hb;            // 2 / (v^H v), 0 for an identity reflector
+    void init(int s_, int m_, const std::vector<cd>& S0) {
+        s = s_; m = m_; ld = (m + 1) * s;
+        H.assign((size_t)ld * m * s, cd(0, 0));
+        Rq = H;
+        G.assign((size_t)ld * s, cd(0, 0));
+        for (int q = 0; q < s; ++q)
+            for (int i = 0; i <= q; ++i) G[(size_t)q * ld + i] = S0[(size_t)q * s + i];
+        hv.assign((size_t)m * s, std::vector<cd>());
+        hb.assign((size_t)m * s, 0.0);
+    }
+    void reflect(int c, cd* col) const {        // apply reflector c to a column of length ld
+        if (hb[c] == 0.0) return;
+        const std::vector<cd>& v = hv[c];
+        cd d = 0;
+        for (int i = 0; i <= s; ++i) d += std::conj(v[i]) * col[c + i];
+        d *= hb[c];
+        for (int i = 0; i <= s; ++i) col[c + i] -= v[i] * d;
+    }
+    // block column j: Hcol is ld x s (rows beyond (j+2)s zero); returns per-column LS residuals
+    void add_block(int j, const std::vector<cd>& Hcol, std::vector<double>& res) {
+        for (int q = 0; q < s; ++q) {
+            const int c = j * s + q;
+            cd* hc = &H[(size_t)c * ld];
+            cd* rc = &Rq[(size_t)c * ld];
+            for (int i = 0; i < ld; ++i) hc[i] = rc[i] = Hcol[(size_t)q * ld + i];
+            for (int p = 0; p < c; ++p) reflect(p, rc);
+            // new reflector zeroing rows c+1..c+s of this column
+            std::vector<cd> v(s + 1);
+            double nx = 0.0;
+            for (int i = 0; i <= s; ++i) {
+                v[i] = rc[c + i];
+                nx += std::norm(v[i]);
+            }
+            nx = std::sqrt(nx);
+            double tail = nx * nx - std::norm(v[0]);
+            if (nx == 0.0 || tail <= 0.0) {
+                hb[c] = 0.0;
+            } else {
+                const double a0 = std::abs(v[0]);
+                const cd ph = a0 > 0 ? v[0] / a0 : cd(1, 0);
+                v[0] += ph * nx;
+                double vv = 0.0;
+                for (int i = 0; i <= s; ++i) vv += std::norm(v[i]);
+                hv[c] = v;
+                hb[c] = 2.0 / vv;
+                reflect(c, rc);
+                for (int i = 1; i <= s; ++i) rc[c + i] = 0;
+                for (int q2 = 0; q2 < s; ++q2) reflect(c, &G[(size_t)q2 * ld]);
+            }
+        }
+        res.assign(s, 0.0);
+        for (int q = 0; q < s; ++q) {
+            double r2 = 0.0;
+            for (int i = (j + 1) * s; i < (j + 2) * s; ++i) r2 += std::norm(G[(size_t)q * ld + i]);
+            res[q] = std::sqrt(r2);
+        }
+    }
+    // Y (Js x s, column-major) = R^-1 G(0:Js, :)
+    std::vector<cd> solve(int J) const {
+        const int K = J * s;
+        std::vector<cd> Y((size_t)K * s, cd(0, 0));
+        for (int q = 0; q < s; ++q)
+            for (int i = K - 1; i >= 0; --i) {
+                cd v = G[(size_t)q * ld + i];
+                for (int k = i + 1; k < K; ++k) v -= Rq[(size_t)k * ld + i] * Y[(size_t)q * K + k];
+                Y[(size_t)q * K + i] = v / Rq[(size_t)i * ld + i];
+            }
+        return Y;
+    }
+};
+
+// One cycle on the current residual R: D = V Y with the block Krylov space of R.
+// Returns the number of block steps, or -1 with *st set on error.
+int block_cycle(BCtx& c, int m, const std::vector<double>& target, bool f32, bool use_graph, int* steps,
+                std::vector<double>& hist, const std::vector<double>& bn) {
+    const int s = c.ns;
+    const int64_t n = c.n;
+    std::vector<cd> S0;
+    int st = cholqr(c, c.ws.R, c.ws.V, S0);
+    if (st == ERR_STATE) {
+        set_error("the block of residuals is rank deficient (dependent right-hand sides)");
+        return ERR_ARG;
+    }
+    if (st) return st;
+    BlockLS ls;
+    ls.init(s, m, S0);
+    const int ld = (m + 1) * s;
+    std::vector<cd> Hcol((size_t)ld * s), Ht((size_t)ld * s);
+    std::vector<double> res;
+    int J = 0;
+    for (int j = 0; j < m; ++j) {
+        c2<double>* Vj = c.ws.V + (int64_t)j * s * n;
+        st = block_matvec(c, Vj, c.ws.W, f32, j, use_graph);
+        if (st) return st;
+        const int P = (j + 1) * s;
+        std::fill(Hcol.begin(), Hcol.end(), cd(0, 0));
+        for (int pass = 0; pass < 2; ++pass) {   // block CGS, two passes
+            st = zgemm(c, true, P, s, n, 1.0, c.ws.V, n, c.ws.W, n, 0.0, c.ws.Hd, P);
+            if (st) return st;
+            st = zgemm(c, false, (int)n, s, P, -1.0, c.ws.V, n, c.ws.Hd, P, 1.0, c.ws.W, n);
+            if (st) return st;
+            MARIE_HIP(hipMemcpyAsync(Ht.data(), c.ws.Hd, (size_t)P * s * sizeof(c2<double>), hipMemcpyDeviceToHost,
+                                     c.s));
+            MARIE_HIP(hipStreamSynchronize(c.s));
+            for (int q = 0; q < s; ++q)
+                for (int i = 0; i < P; ++i) Hcol[(size_t)q * ld + i] += Ht[(size_t)q * P + i];
+        }
+        std::vector<cd> Rs;
+        st = cholqr(c, c.ws.W, c.ws.V + (int64_t)(j + 1) * s * n, Rs);
+        const bool breakdown = (st == ERR_STATE);
+        if (st && !breakdown) return st;
+        if (!breakdown)
+            for (int q = 0; q < s; ++q)
+                for (int i = 0; i <= q; ++i) Hcol[(size_t)q * ld + P + i] = Rs[(size_t)q * s + i];
+        ls.add_block(j, Hcol, res);
+        J = j + 1;
+        double worst = 0.0;
+        bool done = true;
+        for (int q = 0; q < s; ++q) {
+            worst = std::max(worst, res[q] / bn[q]);
+            if (res[q] > target[q]) done = false;
+        }
+        hist.push_back(worst);
+        if (done || breakdown) break;
+    }
+    // D = V(:, 0:Js) Y
+    const std::vector<cd> Y = ls.solve(J);
+    MARIE_HIP(hipMemcpyAsync(c.ws.Sd, Y.data(), Y.size() * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
+    st = zgemm(c, false, (int)n, s, (int64_t)J * s, 1.0, c.ws.V, n, c.ws.Sd, (int64_t)J * s, 0.0, c.ws.D, n);
+    if (st) return st;
+    MARIE_HIP(hipStreamSynchronize(c.s));
+    *steps = J;
+    return OK;
+}
+
+}  // namespace
+
+int ensure_blas(Op* h) {
+    if (h->blas) return OK;
+    rocblas_handle b = nullptr;
+    if (rocblas_create_handle(&b) != rocblas_status_success) {
+        set_error("rocblas_create_handle failed");
+        return ERR_HIP;
+    }
+    h->blas = b;
+    return OK;
+}
+
+void release_blas(Op* h) {
+    if (h->blas) (void)rocblas_destroy_handle(reinterpret_cast<rocblas_handle>(h->blas));
+    h->blas = nullptr;
+}
+
+int block_gmres(Op* h, const c2<double>* B, c2<double>* X, int s, int m, double tol, int maxit, int mode,
+                double inner_tol, bool use_graph, int* flags, double* relres, int* iter, std::vector<double>& hist) {
+    int st = ensure_blas(h);
+    if (st) return st;
+    BCtx c;
+    c.h = h;
+    c.s = h->stream;
+    c.blas = reinterpret_cast<rocblas_handle>(h->blas);
+    c.n = h->nidx;
+    c.ns = s;
+    c.G = grid_of(c.n);
+    if (rocblas_set_stream(c.blas, c.s) != rocblas_status_success) {
+        set_error("rocblas_set_stream failed");
+        return ERR_HIP;
+    }
+    if (!c.s) use_graph = false;   // the legacy default stream cannot be captured
+    st = alloc_block(c, m, mode);
+    auto done = [&](int code) {
+        (void)hipStreamSynchronize(c.s);
+        c.ws.free_all();
+        return code;
+    };
+    if (st) return done(st);
+    st = ensure_work(h, s);
+    if (st) return done(st);
+    const int64_t ns = c.n * s;
+    const int g = grid_of(ns);
+    std::vector<double> bn, rn;
+    st = colnorms(c, B, bn);
+    if (st) return done(st);
+    for (int q = 0; q < s; ++q)
+        if (bn[q] == 0.0) {
+            set_error("a right-hand side column is zero");
+            return done(ERR_ARG);
+        }
+    // R = B - A X
+    st = apply(h, OP_AN, 0, PREC_F64, X, c.ws.T, s, c.s);
+    if (st) return done(st);
+    hipLaunchKernelGGL(k_sub, dim3(g), dim3(BT), 0, c.s, c.ws.R, B, c.ws.T, ns);
+    MARIE_HIP(hipGetLastError());
+    st = colnorms(c, c.ws.R, rn);
+    if (st) return done(st);
+    auto worst_rel = [&]() {
+        double w = 0.0;
+        for (int q = 0; q < s; ++q) w = std::max(w, rn[q] / bn[q]);
+        return w;
+    };
+    hist.assign(1, worst_rel());
+    int cycles = 0, steps = 0;
+    while (worst_rel() > tol && cycles < maxit) {
+        ++cycles;
+        std::vector<double> target(s);
+        for (int q = 0; q < s; ++q) {
+            // stop the cycle at the global target (10 % margin in mode 1), at most
+            // inner_tol relative to the cycle's residual in mode 1
+            double t = (mode == 1 ? 0.9 : 1.0) * tol * bn[q];
+            if (mode == 1) t = std::max(t, inner_tol * rn[q]);
+            target[q] = std::min(t, 0.5 * rn[q]);
+        }
+        st = block_cycle(c, m, target, mode == 1, use_graph, &steps, hist, bn);
+        if (st) return done(st);
+        hipLaunchKernelGGL(k_add, dim3(g), dim3(BT), 0, c.s, X, c.ws.D, ns);
+        st = apply(h, OP_AN, 0, PREC_F64, X, c.ws.T, s, c.s);
+        if (st) return done(st);
+        hipLaunchKernelGGL(k_sub, dim3(g), dim3(BT), 0, c.s, c.ws.R, B, c.ws.T, ns);
+        MARIE_HIP(hipGetLastError());
+        st = colnorms(c, c.ws.R, rn);
+        if (st) return done(st);
+        hist.push_back(worst_rel());
+    }
+    for (int q = 0; q < s; ++q) {
+        relres[q] = rn[q] / bn[q];
+        flags[q] = relres[q] <= tol ? 0 : 1;
+    }
+    iter[0] = steps;
+    iter[1] = cycles;
+    return done(OK);
+}
+
+}  // namespace marie
+
+using namespace marie;
+
+extern "C" int marie_vie_block_gmres_device(marie_vie_op* hh, const void* B, void* X, int nrhs, int restart,
+                                            double tol, int maxit, int mode, double inner_tol, int use_graph,
+                                            int* flag, double* relres, int* iter, double* resvec, int resvec_cap,
+                                            int* nres) {
+    Op* h = reinterpret_cast<Op*>(hh);
+    if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
+    MARIE_HIP(hipSetDevice(h->device));
+    if (!B || !X || !flag || !relres || !iter || nrhs < 1 || nrhs > 64 || restart < 1 ||
+        (int64_t)(restart + 1) * nrhs > 2048 || maxit < 1 || !(tol > 0) || (mode != 0 && mode != 1)) {
+        set_error("invalid block gmres arguments (nrhs 1..64, (restart+1)*nrhs <= 2048, mode 0 or 1)");
+        return MARIE_ERR_ARG;
+    }
+    if (!(h->precisions & PREC_F64) || (mode == 1 && !(h->precisions & PREC_F32))) {
+        set_error("block gmres needs an FP64 operator (and FP32 for mode 1)");
+        return MARIE_ERR_STATE;
+    }
+    if (!h->have_fN || !h->have_mat || !h->have_idx) {
+        set_error("operator, index or material not set");
+        return MARIE_ERR_STATE;
+    }
+    if (h->nidx == 0) {
+        for (int q = 0; q < nrhs; ++q) { flag[q] = 0; relres[q] = 0.0; }
+        iter[0] = iter[1] = 0;
+        if (nres) *nres = 0;
+        return MARIE_OK;
+    }
+    std::vector<double> hist;
+    const int st = block_gmres(h, (const c2<double>*)B, (c2<double>*)X, nrhs, restart, tol, maxit, mode, inner_tol,
+                               use_graph != 0, flag, relres, iter, hist);
+    if (st) return st;
+    if (resvec && resvec_cap > 0)
+        for (int i = 0; i < (int)hist.size() && i < resvec_cap; ++i) resvec[i] = hist[i];
+    if (nres) *nres = (int)hist.size();
+    return MARIE_OK;
+}
+
+extern "C" int marie_vie_block_gmres(marie_vie_op* hh, const double* B, double* X, int nrhs, int restart, double tol,
+                                     int maxit, int mode, double inner_tol, int use_graph, int* flag, double* relres,
+                                     int* iter, double* resvec, int resvec_cap, int* nres) {
+    Op* h = reinterpret_cast<Op*>(hh);
+    if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
+    MARIE_HIP(hipSetDevice(h->device));
+    if (!B || !X || nrhs < 1) { set_error("null block or nrhs < 1"); return MARIE_ERR_ARG; }
+    const size_t n = (size_t)h->nidx * nrhs;
+    for (size_t i = 0; i < 2 * n; ++i)
+        if (!std::isfinite(B[i]) || !std::isfinite(X[i])) {
+            set_error("right-hand sides or initial guesses have non-finite entries");
+            return MARIE_ERR_NONFINITE;
+        }
+    void *db = nullptr, *dx = nullptr;
+    if (n) {
+        if (hipMalloc(&db, n * sizeof(c2<double>)) != hipSuccess || hipMalloc(&dx, n * sizeof(c2<double>)) != hipSuccess) {
+            (void)hipGetLastError();
+            if (db) (void)hipFree(db);
+            set_error("out of device memory");
+            return MARIE_ERR_ALLOC;
+        }
+        MARIE_HIP(hipMemcpy(db, B, n * sizeof(c2<double>), hipMemcpyHostToDevice));
+        MARIE_HIP(hipMemcpy(dx, X, n * sizeof(c2<double>), hipMemcpyHostToDevice));
+    }
+    int st = marie_vie_block_gmres_device(hh, db, dx, nrhs, restart, tol, maxit, mode, inner_tol, use_graph, flag,
+                                          relres, iter, resvec, resvec_cap, nres);
+    if (st == MARIE_OK && n && hipMemcpy(X, dx, n * sizeof(c2<double>), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("D2H copy failed");
+        st = MARIE_ERR_HIP;
+    }
+    if (db) (void)hipFree(db);
+    if (dx) (void)hipFree(dx);
+    return st;
+}

@@ -43,6 +43,10 @@ SIGNATURES = [
                                     ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_gmres', _I, [_P, _P, _P, _I, _D, _I, _I, _D, ctypes.POINTER(_I), _DP,
                              ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_block_gmres_device', _I, [_P, _P, _P, _I, _I, _D, _I, _I, _D, _I, ctypes.POINTER(_I), _DP,
+                                          ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_block_gmres', _I, [_P, _P, _P, _I, _I, _D, _I, _I, _D, _I, ctypes.POINTER(_I), _DP,
+                                   ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_autotune', _I, [_P, _I, _I, ctypes.POINTER(_I), _DP]),
     ('marie_vie_set_tiles', _I, [_P, _I, ctypes.POINTER(_I)]),
     ('marie_vie_enable_timing', _I, [_P, _I]),
@@ -190,6 +194,42 @@ class Operator:
                                             maxit, mode, inner_tol, ctypes.byref(flag), ctypes.byref(relres), it,
                                             resvec.ctypes.data_as(_DP), resvec_cap, ctypes.byref(nres)))
         return flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    def block_gmres_device(self, B_ptr, X_ptr, nrhs, restart=20, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4,
+                           use_graph=False, resvec_cap=20000):
+        """Block GMRES on device buffers; returns (flags, relres, [steps, cycles], resvec)."""
+        flag = (ctypes.c_int * nrhs)()
+        relres = (ctypes.c_double * nrhs)()
+        it = (ctypes.c_int * 2)()
+        resvec = np.zeros(resvec_cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_block_gmres_device(self._h, ctypes.c_void_p(B_ptr), ctypes.c_void_p(X_ptr), nrhs,
+                                                  restart, tol, maxit, mode, inner_tol, 1 if use_graph else 0, flag,
+                                                  relres, it, resvec.ctypes.data_as(_DP), resvec_cap,
+                                                  ctypes.byref(nres)))
+        return (np.array(flag[:]), np.array(relres[:]), [it[0], it[1]],
+                resvec[:min(nres.value, resvec_cap)].copy())
+
+    def block_gmres(self, B, X0=None, restart=20, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4, use_graph=False,
+                    resvec_cap=20000):
+        """Host-buffer block GMRES; B is nidx x nrhs. Returns (X, flags, relres, [steps, cycles], resvec)."""
+        B = np.asarray(B, np.complex128)
+        if B.ndim == 1:
+            B = B[:, None]
+        nrhs = B.shape[1]
+        Bc = np.ascontiguousarray(B.T)
+        Xc = (np.zeros_like(Bc) if X0 is None else
+              np.ascontiguousarray(np.asarray(X0, np.complex128).reshape(B.shape).T).copy())
+        flag = (ctypes.c_int * nrhs)()
+        relres = (ctypes.c_double * nrhs)()
+        it = (ctypes.c_int * 2)()
+        resvec = np.zeros(resvec_cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_block_gmres(self._h, _ptr(Bc), _ptr(Xc), nrhs, restart, tol, maxit, mode, inner_tol,
+                                           1 if use_graph else 0, flag, relres, it, resvec.ctypes.data_as(_DP),
+                                           resvec_cap, ctypes.byref(nres)))
+        return (Xc.T.copy(), np.array(flag[:]), np.array(relres[:]), [it[0], it[1]],
+                resvec[:min(nres.value, resvec_cap)].copy())
 
     def gmres(self, b, x0=None, restart=50, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4, resvec_cap=20000):
         """Host-buffer GMRES; returns (x, flag, relres, [inner, outer], resvec)."""
