@@ -137,26 +137,33 @@ int marie_vie_gmres(marie_vie_op* h, const double* b, double* x, int restart, do
  * entry). The reference solves excitation ports one pgmres at a time
  * (VSIE_Solver.m:387-407, fVIEsolver_Assembly.m:185-197); here one block
  * Krylov space serves all columns: restarted block GMRES with `restart`
- * block steps per cycle, two-pass block Gram-Schmidt, shifted CholeskyQR3
+ * block steps per cycle, block classical Gram-Schmidt, CholeskyQR2 (shifted
+ * CholeskyQR3 for an ill-conditioned block)
  * for the block QR, a Householder least-squares update per step, and the
  * FP64 true residual B - A X at every restart. Column q converges when
  * ||b_q - A x_q|| <= tol ||b_q|| (flag[q], relres[q] as pgmres.m:175).
  * mode 0 = complex128 products; mode 1 = complex64 products, FP64 basis and
  * FP64 residual replacement (each cycle stops at max(inner_tol*||r_q||,
- * 0.9*tol*||b_q||) per column). use_graph = 1 replays each block step's
- * multi-column product from a HIP graph captured on the cycle that first
- * reaches it (not on the legacy default stream). iter = {block steps of the
+ * 0.9*tol*||b_q||) per column). `options` is a bitmask:
+ * MARIE_BLOCK_GRAPH replays each block step's multi-column product from a
+ * HIP graph captured on the cycle that first reaches it (not on the legacy
+ * default stream); MARIE_BLOCK_CGS2 orthogonalises with two classical
+ * Gram-Schmidt passes instead of the one of pgmres_iter.m:106-107 (twice the
+ * block-vector traffic, orthogonality kept to working precision). iter =
+ * {block steps of the
  * last cycle, cycles}; resvec gets the worst column's relative residual per
  * step and, after each cycle, its true value. Limits: nrhs <= 64,
  * (restart+1)*nrhs <= 2048 basis vectors. */
+#define MARIE_BLOCK_GRAPH 1
+#define MARIE_BLOCK_CGS2 2
 int marie_vie_block_gmres_device(marie_vie_op* h, const void* B_dev, void* X_dev, int nrhs, int restart,
-                                 double tol, int maxit, int mode, double inner_tol, int use_graph,
+                                 double tol, int maxit, int mode, double inner_tol, int options,
                                  int* flag, double* relres, int* iter, double* resvec, int resvec_cap,
                                  int* nres);
 
 /* Same with host buffers B, X (nrhs stacked compact vectors, complex128). */
 int marie_vie_block_gmres(marie_vie_op* h, const double* B, double* X, int nrhs, int restart, double tol,
-                          int maxit, int mode, double inner_tol, int use_graph, int* flag, double* relres,
+                          int maxit, int mode, double inner_tol, int options, int* flag, double* relres,
                           int* iter, double* resvec, int resvec_cap, int* nres);
 
 /* Tile autotuner: times every compiled variant of the five matvec kernels

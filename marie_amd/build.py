@@ -35,8 +35,7 @@ def build(verbose=False, jobs=6):
     for p in procs:
         _wait(p)
     tmp = LIB + '.tmp'
-    cmd = [HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp] + objs + ['-L/opt/rocm/lib', '-lrocblas',
-                                                                                  '-Wl,-rpath,/opt/rocm/lib']
+    cmd = [HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC', '-o', tmp] + objs
     if verbose:
         print(' '.join(cmd), flush=True)
     out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)

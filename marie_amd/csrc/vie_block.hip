@@ -9,21 +9,26 @@
 // Per cycle, on R = B - A X (FP64):
 //   R = V_0 S_0                       shifted CholeskyQR3 (Gram on device, s x s Cholesky on host)
 //   block step j: W = A V_j           multi-RHS product (complex128, or complex64 in mode 1)
-//                 H_{0:j,j} = V^H W, W -= V H      block classical Gram-Schmidt, two passes
+//                 H_{0:j,j} = V^H W, W -= V H      block classical Gram-Schmidt, one pass
+//                                                  as pgmres_iter.m:106-107 (or two)
 //                 W = V_{j+1} H_{j+1,j}            shifted CholeskyQR3
 //   least squares min || E_1 S_0 - Hbar Y ||  by Householder QR of the block
 //   Hessenberg matrix, updated one block column per step on the host;
 //   X += V Y; the true residual B - A X (FP64) starts the next cycle.
-// The tall-skinny products V^H W, V H and W R^-1 are plain complex GEMMs
-// and go to rocBLAS; the multi-RHS product is this library's FFT path, and
-// may be replayed from HIP graphs captured per block step.
+// The tall-skinny products (V^H W with K = n ~ 10^6 rows, and V C with a
+// small C) are this file's kernels: a library GEMM tiles the small P x s
+// output and leaves most of the chip idle on K; here k_vhw splits the rows
+// over workgroups (register-tiled 4 x 4 per thread, operands staged in LDS)
+// and k_vc streams V once per output tile. The multi-RHS product is this
+// library's FFT path, and may be replayed from HIP graphs captured per block
+// step.
 #include "../../include/marie_vie.h"
 #include "vie_handle.hpp"
 
-#include <rocblas/rocblas.h>
-
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -94,6 +99,154 @@ int grid_of(int64_t n) {
     return (int)(g < 1 ? 1 : (g > BG ? BG : g));
 }
 
+// ---- tall-skinny products ----------------------------------------------------
+// C (P x Q, partial per row block) = V^H W over rows; V: P columns of length n
+// (stride ldv), W: Q columns (stride ldw). Workgroup (pt, qt, rb): output tile
+// of VP = 64 NI columns of V by VQ = 16 columns of W, rows in chunks of VR = 8
+// taken rb, rb + G, ...; thread (tp, tq) accumulates p = p0 + tp + 64 i (i < NI),
+// q = q0 + tq + 4 j (j < 4) over all its chunks. Partials go to
+// part[(q P + p) G + rb].
+constexpr int VQ = 16, VR = 8;
+
+template <int NI>
+__global__ __launch_bounds__(BT) void k_vhw(const c2<double>* __restrict__ V, int64_t ldv, int P,
+                                            const c2<double>* __restrict__ W, int64_t ldw, int Q, int64_t n,
+                                            c2<double>* __restrict__ part, int G) {
+    constexpr int VP = 64 * NI;
+    __shared__ c2<double> Vs[VR][VP + 1];
+    __shared__ c2<double> Ws[VR][VQ + 1];
+    const int ptn = (P + VP - 1) / VP;
+    const int pt = blockIdx.x % ptn, qt = blockIdx.x / ptn;
+    const int rb = blockIdx.y;
+    const int p0 = pt * VP, q0 = qt * VQ;
+    const int tq = threadIdx.x & 3, tp = threadIdx.x >> 2;
+    double ar[NI][4] = {}, ai[NI][4] = {};
+    const int64_t nchunk = (n + VR - 1) / VR;
+    const int pv = P - p0 < VP ? P - p0 : VP;     // live columns of this tile
+    // the next chunk is loaded into registers while the current one is used
+    constexpr int LV = VR * VP / BT;              // V elements staged per thread
+    c2<double> vreg[LV], wreg = {0.0, 0.0};
+    auto fetch = [&](int64_t ch) {
+        const int64_t r0 = ch * VR;
+#pragma unroll
+        for (int k = 0; k < LV; ++k) {
+            const int i = threadIdx.x + BT * k;
+            const int r = i % VR, p = i / VR;
+            const int64_t e = r0 + r;
+            vreg[k] = (p < pv && e < n) ? V[(int64_t)(p0 + p) * ldv + e] : c2<double>{0.0, 0.0};
+        }
+        if (threadIdx.x < VR * VQ) {
+            const int r = threadIdx.x % VR, q = threadIdx.x / VR;
+            const int64_t e = r0 + r;
+            wreg = (q0 + q < Q && e < n) ? W[(int64_t)(q0 + q) * ldw + e] : c2<double>{0.0, 0.0};
+        }
+    };
+    if (rb < nchunk) fetch(rb);
+    for (int64_t ch = rb; ch < nchunk; ch += G) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < LV; ++k) {
+            const int i = threadIdx.x + BT * k;
+            Vs[i % VR][i / VR] = vreg[k];
+        }
+        if (threadIdx.x < VR * VQ) Ws[threadIdx.x % VR][threadIdx.x / VR] = wreg;
+        __syncthreads();
+        if (ch + G < nchunk) fetch(ch + G);
+        if (tp >= pv) continue;
+#pragma unroll 2
+        for (int r = 0; r < VR; ++r) {
+            c2<double> v[NI], w[4];
+#pragma unroll
+            for (int i = 0; i < NI; ++i) v[i] = Vs[r][tp + 64 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w[j] = Ws[r][tq + 4 * j];
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    // conj(v) * w (columns beyond pv read stale LDS and are not stored)
+                    ar[i][j] += v[i].x * w[j].x + v[i].y * w[j].y;
+                    ai[i][j] += v[i].x * w[j].y - v[i].y * w[j].x;
+                }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int p = p0 + tp + 64 * i, q = q0 + tq + 4 * j;
+            if (tp + 64 * i < pv && q < Q) part[((int64_t)q * P + p) * G + rb] = c2<double>{ar[i][j], ai[i][j]};
+        }
+}
+
+// C[q * ldc + p] = sum_rb part[(q P + p) G + rb]: one wave per output, lanes
+// over rb in a fixed order, then a fixed shuffle tree
+__global__ __launch_bounds__(BT) void k_vhw_reduce(const c2<double>* __restrict__ part, int P, int Q, int G,
+                                                   c2<double>* __restrict__ C, int64_t ldc) {
+    const int o = blockIdx.x * (BT / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (o >= P * Q) return;
+    const c2<double>* src = part + (int64_t)o * G;
+    double x = 0.0, y = 0.0;
+    for (int rb = lane; rb < G; rb += 64) {
+        x += src[rb].x;
+        y += src[rb].y;
+    }
+    for (int s = 32; s > 0; s >>= 1) {
+        x += __shfl_down(x, s, 64);
+        y += __shfl_down(y, s, 64);
+    }
+    if (lane == 0) {
+        const int p = o % P, q = o / P;
+        C[(int64_t)q * ldc + p] = c2<double>{x, y};
+    }
+}
+
+// out(:, q) = beta * out(:, q) + alpha * sum_p V(:, p) C(p, q); thread per row,
+// 16 output columns per workgroup, C streamed through LDS 64 rows at a time
+constexpr int CQ = 16, CP = 64;
+__global__ __launch_bounds__(BT) void k_vc(c2<double>* __restrict__ out, int64_t ldo, const c2<double>* __restrict__ V,
+                                           int64_t ldv, int P, const c2<double>* __restrict__ C, int64_t ldc, int Q,
+                                           int64_t n, double alpha, double beta) {
+    __shared__ c2<double> Cs[CP][CQ];
+    const int q0 = blockIdx.y * CQ;
+    const int64_t e = blockIdx.x * (int64_t)BT + threadIdx.x;
+    double ax[CQ] = {}, ay[CQ] = {};
+    for (int pc = 0; pc < P; pc += CP) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < CP * CQ; i += BT) {
+            const int p = i / CQ, q = i % CQ;
+            Cs[p][q] = (pc + p < P && q0 + q < Q) ? C[(int64_t)(q0 + q) * ldc + pc + p] : c2<double>{0.0, 0.0};
+        }
+        __syncthreads();
+        if (e < n) {
+            const int np = P - pc < CP ? P - pc : CP;
+#pragma unroll 4
+            for (int p = 0; p < np; ++p) {
+                const c2<double> v = V[(int64_t)(pc + p) * ldv + e];
+#pragma unroll
+                for (int q = 0; q < CQ; ++q) {
+                    const c2<double> cc = Cs[p][q];
+                    ax[q] += v.x * cc.x - v.y * cc.y;
+                    ay[q] += v.x * cc.y + v.y * cc.x;
+                }
+            }
+        }
+    }
+    if (e >= n) return;
+#pragma unroll
+    for (int q = 0; q < CQ; ++q) {
+        if (q0 + q >= Q) break;
+        c2<double>* o = out + (int64_t)(q0 + q) * ldo + e;
+        double ox = alpha * ax[q], oy = alpha * ay[q];
+        if (beta != 0.0) {
+            ox += beta * o->x;
+            oy += beta * o->y;
+        }
+        *o = c2<double>{ox, oy};
+    }
+}
+
 struct BlockWs {
     c2<double>* V = nullptr;     // n x (m+1)s
     c2<double>* W = nullptr;     // n x s
@@ -106,10 +259,11 @@ struct BlockWs {
     c2<double>* Sd = nullptr;    // (m+1)s x s small matrix upload
     double* part = nullptr;      // s x BG
     double* nrm = nullptr;       // s
+    c2<double>* vpart = nullptr; // k_vhw partials (VPART complex)
     std::vector<hipGraphExec_t> graphs;
     std::vector<hipGraph_t> gsrc;
     void free_all() {
-        void* p[] = {V, W, T, R, D, Xs, Ys, Hd, Sd, part, nrm};
+        void* p[] = {V, W, T, R, D, Xs, Ys, Hd, Sd, part, nrm, vpart};
         for (void* q : p)
             if (q) (void)hipFree(q);
         for (auto g : graphs)
@@ -122,34 +276,46 @@ struct BlockWs {
 struct BCtx {
     Op* h;
     hipStream_t s;
-    rocblas_handle blas;
     int64_t n;
     int ns;                      // block size s
     int G;
     BlockWs ws;
 };
 
-#define MARIE_BLAS(call)                                                        \
-    do {                                                                        \
-        rocblas_status b_ = (call);                                             \
-        if (b_ != rocblas_status_success) {                                     \
-            ::marie::set_error(std::string(#call) + ": " + rocblas_status_to_string(b_)); \
-            return ::marie::ERR_HIP;                                            \
-        }                                                                       \
-    } while (0)
-
-inline rocblas_double_complex* zc(c2<double>* p) { return reinterpret_cast<rocblas_double_complex*>(p); }
-inline const rocblas_double_complex* zc(const c2<double>* p) {
-    return reinterpret_cast<const rocblas_double_complex*>(p);
+// row blocks of k_vhw: about 1024 workgroups in all, 64..512 per output tile;
+// the partials then take at most VPART complex values
+constexpr size_t VPART = (size_t)1024 * 256 * VQ;
+int vhw_blocks(int tiles) {
+    int g = 1024 / tiles;
+    return g < 64 ? 64 : (g > 512 ? 512 : g);
 }
 
-// C (m x nn, ldc) = alpha * op(A) * B + beta * C, column-major
-int zgemm(BCtx& c, bool conjA, int m, int nn, int64_t k, cd alpha, const c2<double>* A, int64_t lda,
-          const c2<double>* B, int64_t ldb, cd beta, c2<double>* C, int64_t ldc) {
-    const rocblas_double_complex al(alpha.real(), alpha.imag()), be(beta.real(), beta.imag());
-    MARIE_BLAS(rocblas_zgemm(c.blas, conjA ? rocblas_operation_conjugate_transpose : rocblas_operation_none,
-                             rocblas_operation_none, m, nn, (rocblas_int)k, &al, zc(A), (rocblas_int)lda, zc(B),
-                             (rocblas_int)ldb, &be, zc(C), (rocblas_int)ldc));
+// C (P x Q, ld ldc) = V^H W over n rows
+int vhw(BCtx& c, const c2<double>* V, int P, const c2<double>* W, int Q, c2<double>* C, int64_t ldc) {
+    const int ni = P <= 64 ? 1 : 4;
+    const int vp = 64 * ni;
+    const int tiles = ((P + vp - 1) / vp) * ((Q + VQ - 1) / VQ);
+    const int G = vhw_blocks(tiles);
+    if ((size_t)G * P * Q > VPART) {
+        set_error("block too large for the multi-dot partials");
+        return ERR_SIZE;
+    }
+    if (ni == 1)
+        hipLaunchKernelGGL(k_vhw<1>, dim3(tiles, G), dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
+    else
+        hipLaunchKernelGGL(k_vhw<4>, dim3(tiles, G), dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
+    hipLaunchKernelGGL(k_vhw_reduce, dim3((P * Q + BT / 64 - 1) / (BT / 64)), dim3(BT), 0, c.s, c.ws.vpart, P, Q, G,
+                       C, ldc);
+    MARIE_HIP(hipGetLastError());
+    return OK;
+}
+
+// out (n x Q) = beta * out + alpha * V (n x P) C (P x Q, ld ldc)
+int vc(BCtx& c, c2<double>* out, const c2<double>* V, int P, const c2<double>* C, int64_t ldc, int Q, double alpha,
+       double beta) {
+    const dim3 g((unsigned)((c.n + BT - 1) / BT), (unsigned)((Q + CQ - 1) / CQ));
+    hipLaunchKernelGGL(k_vc, g, dim3(BT), 0, c.s, out, c.n, V, c.n, P, C, ldc, Q, c.n, alpha, beta);
+    MARIE_HIP(hipGetLastError());
     return OK;
 }
 
@@ -167,14 +333,14 @@ int colnorms(BCtx& c, const c2<double>* v, std::vector<double>& out) {
 
 // Hermitian positive definite s x s (column-major) -> upper R with A = R^H R;
 // false when A is not numerically positive definite
-bool cholesky_upper(const std::vector<cd>& A, int s, std::vector<cd>& R) {
+bool cholesky_upper(const std::vector<cd>& A, int s, std::vector<cd>& R, double min_pivot = 1e-28) {
     R.assign((size_t)s * s, cd(0, 0));
     double amax = 0.0;
     for (int i = 0; i < s; ++i) amax = std::max(amax, A[(size_t)i * s + i].real());
     for (int j = 0; j < s; ++j) {
         cd d = A[(size_t)j * s + j];
         for (int k = 0; k < j; ++k) d -= std::conj(R[(size_t)j * s + k]) * R[(size_t)j * s + k];
-        if (!(d.real() > 1e-28 * amax) || !std::isfinite(d.real())) return false;
+        if (!(d.real() > min_pivot * amax) || !std::isfinite(d.real())) return false;
         const double rjj = std::sqrt(d.real());
         R[(size_t)j * s + j] = rjj;
         for (int i = j + 1; i < s; ++i) {
@@ -200,13 +366,16 @@ std::vector<cd> upper_inverse(const std::vector<cd>& R, int s) {
     return X;
 }
 
-// Shifted CholeskyQR3: W = out Rf with out orthonormal (out != W). The first
-// pass factors the Gram matrix plus a small shift (11 (n s + s(s+1)) u per
-// unit diagonal), which always succeeds and leaves a basis of condition
-// number O(u^-1/2); two plain passes then restore orthonormality. Each pass
+// CholeskyQR: W = out Rf with out orthonormal (out != W). Each pass
 // equilibrates the Gram matrix to unit diagonal first, so columns of very
 // different norms (a nearly converged right-hand side next to others) lose
-// nothing. Returns ERR_STATE (no error text) for a zero or non-finite column.
+// nothing. When the first Gram matrix factors with every pivot above 1e-10
+// (condition number below 1e5) two plain passes suffice (CholeskyQR2);
+// otherwise the first pass factors the Gram matrix plus a small shift
+// (11 (n s + s(s+1)) u per unit diagonal), which always succeeds and leaves a
+// basis of condition number O(u^-1/2), and two plain passes follow (shifted
+// CholeskyQR3). Returns ERR_STATE (no error text) for a zero or non-finite
+// column.
 int cholqr(BCtx& c, const c2<double>* W, c2<double>* out, std::vector<cd>& Rf) {
     const int s = c.ns;
     std::vector<cd> g((size_t)s * s);
@@ -214,8 +383,9 @@ int cholqr(BCtx& c, const c2<double>* W, c2<double>* out, std::vector<cd>& Rf) {
     const double shift = 11.0 * ((double)c.n * s + (double)s * (s + 1)) * 1.1102230246251565e-16 * s;
     const c2<double>* src = W;
     c2<double>* dsts[3] = {out, c.ws.T, out};
-    for (int pass = 0; pass < 3; ++pass) {
-        int st = zgemm(c, true, s, s, c.n, 1.0, src, c.n, src, c.n, 0.0, c.ws.Hd, s);
+    int npass = 3;
+    for (int pass = 0; pass < npass; ++pass) {
+        int st = vhw(c, src, s, src, s, c.ws.Hd, s);
         if (st) return st;
         MARIE_HIP(hipMemcpyAsync(g.data(), c.ws.Hd, (size_t)s * s * sizeof(c2<double>), hipMemcpyDeviceToHost, c.s));
         MARIE_HIP(hipStreamSynchronize(c.s));
@@ -226,16 +396,22 @@ int cholqr(BCtx& c, const c2<double>* W, c2<double>* out, std::vector<cd>& Rf) {
         }
         for (int jj = 0; jj < s; ++jj)
             for (int i = 0; i < s; ++i) g[(size_t)jj * s + i] /= d[i] * d[jj];
-        if (pass == 0)
-            for (int i = 0; i < s; ++i) g[(size_t)i * s + i] += shift;
         std::vector<cd>& R = Rp[pass];
-        if (!cholesky_upper(g, s, R)) return ERR_STATE;
+        if (pass == 0 && cholesky_upper(g, s, R, 1e-10)) {
+            npass = 2;                  // well conditioned: CholeskyQR2
+            dsts[0] = c.ws.T;
+            dsts[1] = out;
+        } else {
+            if (pass == 0)
+                for (int i = 0; i < s; ++i) g[(size_t)i * s + i] += shift;
+            if (!cholesky_upper(g, s, R)) return ERR_STATE;
+        }
         for (int jj = 0; jj < s; ++jj)
             for (int i = 0; i <= jj; ++i) R[(size_t)jj * s + i] *= d[jj];
         const std::vector<cd> Ri = upper_inverse(R, s);
         MARIE_HIP(hipMemcpyAsync(c.ws.Sd, Ri.data(), (size_t)s * s * sizeof(c2<double>), hipMemcpyHostToDevice,
                                  c.s));
-        st = zgemm(c, false, (int)c.n, s, s, 1.0, src, c.n, c.ws.Sd, s, 0.0, dsts[pass], c.n);
+        st = vc(c, dsts[pass], src, s, c.ws.Sd, s, s, 1.0, 0.0);
         if (st) return st;
         MARIE_HIP(hipStreamSynchronize(c.s));   // Ri (host memory) goes out of scope
         src = dsts[pass];
@@ -251,7 +427,7 @@ int cholqr(BCtx& c, const c2<double>* W, c2<double>* out, std::vector<cd>& Rf) {
             }
         return C;
     };
-    Rf = mul_upper(Rp[2], mul_upper(Rp[1], Rp[0]));
+    Rf = npass == 2 ? mul_upper(Rp[1], Rp[0]) : mul_upper(Rp[2], mul_upper(Rp[1], Rp[0]));
     return OK;
 }
 
@@ -315,6 +491,7 @@ int alloc_block(BCtx& c, int m, int mode) {
     if (!st) st = A((void**)&c.ws.Sd, (size_t)(m + 1) * s * s * sizeof(c2<double>));
     if (!st) st = A((void**)&c.ws.part, (size_t)s * BG * sizeof(double));
     if (!st) st = A((void**)&c.ws.nrm, (size_t)s * sizeof(double));
+    if (!st) st = A((void**)&c.ws.vpart, VPART * sizeof(c2<double>));
     return st;
 }
 
@@ -399,7 +576,7 @@ struct BlockLS {
 
 // One cycle on the current residual R: D = V Y with the block Krylov space of R.
 // Returns the number of block steps, or -1 with *st set on error.
-int block_cycle(BCtx& c, int m, const std::vector<double>& target, bool f32, bool use_graph, int* steps,
+int block_cycle(BCtx& c, int m, const std::vector<double>& target, bool f32, bool use_graph, int ncgs, int* steps,
                 std::vector<double>& hist, const std::vector<double>& bn) {
     const int s = c.ns;
     const int64_t n = c.n;
@@ -422,10 +599,10 @@ int block_cycle(BCtx& c, int m, const std::vector<double>& target, bool f32, boo
         if (st) return st;
         const int P = (j + 1) * s;
         std::fill(Hcol.begin(), Hcol.end(), cd(0, 0));
-        for (int pass = 0; pass < 2; ++pass) {   // block CGS, two passes
-            st = zgemm(c, true, P, s, n, 1.0, c.ws.V, n, c.ws.W, n, 0.0, c.ws.Hd, P);
+        for (int pass = 0; pass < ncgs; ++pass) {   // block classical Gram-Schmidt passes
+            st = vhw(c, c.ws.V, P, c.ws.W, s, c.ws.Hd, P);
             if (st) return st;
-            st = zgemm(c, false, (int)n, s, P, -1.0, c.ws.V, n, c.ws.Hd, P, 1.0, c.ws.W, n);
+            st = vc(c, c.ws.W, c.ws.V, P, c.ws.Hd, P, s, -1.0, 1.0);
             if (st) return st;
             MARIE_HIP(hipMemcpyAsync(Ht.data(), c.ws.Hd, (size_t)P * s * sizeof(c2<double>), hipMemcpyDeviceToHost,
                                      c.s));
@@ -454,7 +631,7 @@ int block_cycle(BCtx& c, int m, const std::vector<double>& target, bool f32, boo
     // D = V(:, 0:Js) Y
     const std::vector<cd> Y = ls.solve(J);
     MARIE_HIP(hipMemcpyAsync(c.ws.Sd, Y.data(), Y.size() * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
-    st = zgemm(c, false, (int)n, s, (int64_t)J * s, 1.0, c.ws.V, n, c.ws.Sd, (int64_t)J * s, 0.0, c.ws.D, n);
+    st = vc(c, c.ws.D, c.ws.V, J * s, c.ws.Sd, (int64_t)J * s, s, 1.0, 0.0);
     if (st) return st;
     MARIE_HIP(hipStreamSynchronize(c.s));
     *steps = J;
@@ -463,39 +640,18 @@ int block_cycle(BCtx& c, int m, const std::vector<double>& target, bool f32, boo
 
 }  // namespace
 
-int ensure_blas(Op* h) {
-    if (h->blas) return OK;
-    rocblas_handle b = nullptr;
-    if (rocblas_create_handle(&b) != rocblas_status_success) {
-        set_error("rocblas_create_handle failed");
-        return ERR_HIP;
-    }
-    h->blas = b;
-    return OK;
-}
-
-void release_blas(Op* h) {
-    if (h->blas) (void)rocblas_destroy_handle(reinterpret_cast<rocblas_handle>(h->blas));
-    h->blas = nullptr;
-}
-
 int block_gmres(Op* h, const c2<double>* B, c2<double>* X, int s, int m, double tol, int maxit, int mode,
-                double inner_tol, bool use_graph, int* flags, double* relres, int* iter, std::vector<double>& hist) {
-    int st = ensure_blas(h);
-    if (st) return st;
+                double inner_tol, int options, int* flags, double* relres, int* iter, std::vector<double>& hist) {
+    bool use_graph = (options & MARIE_BLOCK_GRAPH) != 0;
+    const int ncgs = (options & MARIE_BLOCK_CGS2) ? 2 : 1;
     BCtx c;
     c.h = h;
     c.s = h->stream;
-    c.blas = reinterpret_cast<rocblas_handle>(h->blas);
     c.n = h->nidx;
     c.ns = s;
     c.G = grid_of(c.n);
-    if (rocblas_set_stream(c.blas, c.s) != rocblas_status_success) {
-        set_error("rocblas_set_stream failed");
-        return ERR_HIP;
-    }
     if (!c.s) use_graph = false;   // the legacy default stream cannot be captured
-    st = alloc_block(c, m, mode);
+    int st = alloc_block(c, m, mode);
     auto done = [&](int code) {
         (void)hipStreamSynchronize(c.s);
         c.ws.free_all();
@@ -528,6 +684,8 @@ int block_gmres(Op* h, const c2<double>* B, c2<double>* X, int s, int m, double 
     };
     hist.assign(1, worst_rel());
     int cycles = 0, steps = 0;
+    const char* ve = std::getenv("MARIE_VERBOSE");
+    const bool verbose = ve && *ve && *ve != '0';
     while (worst_rel() > tol && cycles < maxit) {
         ++cycles;
         std::vector<double> target(s);
@@ -538,7 +696,7 @@ int block_gmres(Op* h, const c2<double>* B, c2<double>* X, int s, int m, double 
             if (mode == 1) t = std::max(t, inner_tol * rn[q]);
             target[q] = std::min(t, 0.5 * rn[q]);
         }
-        st = block_cycle(c, m, target, mode == 1, use_graph, &steps, hist, bn);
+        st = block_cycle(c, m, target, mode == 1, use_graph, ncgs, &steps, hist, bn);
         if (st) return done(st);
         hipLaunchKernelGGL(k_add, dim3(g), dim3(BT), 0, c.s, X, c.ws.D, ns);
         st = apply(h, OP_AN, 0, PREC_F64, X, c.ws.T, s, c.s);
@@ -548,6 +706,11 @@ int block_gmres(Op* h, const c2<double>* B, c2<double>* X, int s, int m, double 
         st = colnorms(c, c.ws.R, rn);
         if (st) return done(st);
         hist.push_back(worst_rel());
+        if (verbose) {
+            std::fprintf(stderr, "[marie block gmres] cycle %d: %d block steps, worst relres %.3e\n", cycles, steps,
+                         worst_rel());
+            std::fflush(stderr);
+        }
     }
     for (int q = 0; q < s; ++q) {
         relres[q] = rn[q] / bn[q];
@@ -563,7 +726,7 @@ int block_gmres(Op* h, const c2<double>* B, c2<double>* X, int s, int m, double 
 using namespace marie;
 
 extern "C" int marie_vie_block_gmres_device(marie_vie_op* hh, const void* B, void* X, int nrhs, int restart,
-                                            double tol, int maxit, int mode, double inner_tol, int use_graph,
+                                            double tol, int maxit, int mode, double inner_tol, int options,
                                             int* flag, double* relres, int* iter, double* resvec, int resvec_cap,
                                             int* nres) {
     Op* h = reinterpret_cast<Op*>(hh);
@@ -590,7 +753,7 @@ extern "C" int marie_vie_block_gmres_device(marie_vie_op* hh, const void* B, voi
     }
     std::vector<double> hist;
     const int st = block_gmres(h, (const c2<double>*)B, (c2<double>*)X, nrhs, restart, tol, maxit, mode, inner_tol,
-                               use_graph != 0, flag, relres, iter, hist);
+                               options, flag, relres, iter, hist);
     if (st) return st;
     if (resvec && resvec_cap > 0)
         for (int i = 0; i < (int)hist.size() && i < resvec_cap; ++i) resvec[i] = hist[i];
@@ -599,7 +762,7 @@ extern "C" int marie_vie_block_gmres_device(marie_vie_op* hh, const void* B, voi
 }
 
 extern "C" int marie_vie_block_gmres(marie_vie_op* hh, const double* B, double* X, int nrhs, int restart, double tol,
-                                     int maxit, int mode, double inner_tol, int use_graph, int* flag, double* relres,
+                                     int maxit, int mode, double inner_tol, int options, int* flag, double* relres,
                                      int* iter, double* resvec, int resvec_cap, int* nres) {
     Op* h = reinterpret_cast<Op*>(hh);
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
@@ -622,7 +785,7 @@ extern "C" int marie_vie_block_gmres(marie_vie_op* hh, const double* B, double* 
         MARIE_HIP(hipMemcpy(db, B, n * sizeof(c2<double>), hipMemcpyHostToDevice));
         MARIE_HIP(hipMemcpy(dx, X, n * sizeof(c2<double>), hipMemcpyHostToDevice));
     }
-    int st = marie_vie_block_gmres_device(hh, db, dx, nrhs, restart, tol, maxit, mode, inner_tol, use_graph, flag,
+    int st = marie_vie_block_gmres_device(hh, db, dx, nrhs, restart, tol, maxit, mode, inner_tol, options, flag,
                                           relres, iter, resvec, resvec_cap, nres);
     if (st == MARIE_OK && n && hipMemcpy(X, dx, n * sizeof(c2<double>), hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("D2H copy failed");

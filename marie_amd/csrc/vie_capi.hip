@@ -124,7 +124,6 @@ int marie_vie_destroy(marie_vie_op* h) {
         if (b) (void)hipFree(b);
     for (int k = 0; k < 8; ++k)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
-    release_blas(h);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return MARIE_OK;

@@ -75,7 +75,6 @@ struct Op {
     hipEvent_t ev[8] = {};
     double kernel_ms[8] = {};
     int64_t kernel_calls = 0;
-    void* blas = nullptr;           // rocBLAS handle of the block solver (created on first use)
 
     int64_t nvox() const { return (int64_t)L * M * N; }
 };
@@ -87,7 +86,5 @@ int upload_twiddles(Op* h);
 int store_fN(Op* h, void* full64);   // takes ownership of a device complex128 fN
 int load_fN(Op* h, void* out64);     // full complex128 fN into a device buffer
 int autotune(Op* h, int prec, int iters, double* best_ms5);
-int ensure_blas(Op* h);
-void release_blas(Op* h);
 
 }  // namespace marie

@@ -93,6 +93,16 @@ def _cplx(a):
     return np.ascontiguousarray(a.ravel(order='F'))
 
 
+BLOCK_GRAPH = 1
+BLOCK_CGS2 = 2
+
+
+def _block_options(use_graph, cgs_passes):
+    if cgs_passes not in (1, 2):
+        raise ValueError('cgs_passes must be 1 or 2')
+    return (BLOCK_GRAPH if use_graph else 0) | (BLOCK_CGS2 if cgs_passes == 2 else 0)
+
+
 class Operator:
     """A device-resident JVIE operator: fN, material, DOF index and work buffers.
 
@@ -196,7 +206,7 @@ class Operator:
         return flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
 
     def block_gmres_device(self, B_ptr, X_ptr, nrhs, restart=20, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4,
-                           use_graph=False, resvec_cap=20000):
+                           use_graph=False, cgs_passes=1, resvec_cap=20000):
         """Block GMRES on device buffers; returns (flags, relres, [steps, cycles], resvec)."""
         flag = (ctypes.c_int * nrhs)()
         relres = (ctypes.c_double * nrhs)()
@@ -204,14 +214,15 @@ class Operator:
         resvec = np.zeros(resvec_cap)
         nres = ctypes.c_int()
         check(load().marie_vie_block_gmres_device(self._h, ctypes.c_void_p(B_ptr), ctypes.c_void_p(X_ptr), nrhs,
-                                                  restart, tol, maxit, mode, inner_tol, 1 if use_graph else 0, flag,
+                                                  restart, tol, maxit, mode, inner_tol,
+                                                  _block_options(use_graph, cgs_passes), flag,
                                                   relres, it, resvec.ctypes.data_as(_DP), resvec_cap,
                                                   ctypes.byref(nres)))
         return (np.array(flag[:]), np.array(relres[:]), [it[0], it[1]],
                 resvec[:min(nres.value, resvec_cap)].copy())
 
     def block_gmres(self, B, X0=None, restart=20, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4, use_graph=False,
-                    resvec_cap=20000):
+                    cgs_passes=1, resvec_cap=20000):
         """Host-buffer block GMRES; B is nidx x nrhs. Returns (X, flags, relres, [steps, cycles], resvec)."""
         B = np.asarray(B, np.complex128)
         if B.ndim == 1:
@@ -226,7 +237,8 @@ class Operator:
         resvec = np.zeros(resvec_cap)
         nres = ctypes.c_int()
         check(load().marie_vie_block_gmres(self._h, _ptr(Bc), _ptr(Xc), nrhs, restart, tol, maxit, mode, inner_tol,
-                                           1 if use_graph else 0, flag, relres, it, resvec.ctypes.data_as(_DP),
+                                           _block_options(use_graph, cgs_passes), flag, relres, it,
+                                           resvec.ctypes.data_as(_DP),
                                            resvec_cap, ctypes.byref(nres)))
         return (Xc.T.copy(), np.array(flag[:]), np.array(relres[:]), [it[0], it[1]],
                 resvec[:min(nres.value, resvec_cap)].copy())

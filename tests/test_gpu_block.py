@@ -77,6 +77,16 @@ def test_block_matches_per_column_solves(problem, reference, restart):
     assert abs(rv[-1] - relres.max()) <= 1e-12
 
 
+def test_block_two_pass_gram_schmidt(problem, reference):
+    print("case block cgs2", flush=True)
+    op = _op(problem)
+    B = problem[-1]
+    X, flags, relres, it, rv = op.block_gmres(B, restart=10, tol=1e-10, maxit=100, cgs_passes=2)
+    assert np.all(flags == 0) and np.all(relres <= 1e-10), relres
+    for q in range(B.shape[1]):
+        assert _rel(X[:, q], reference[:, q]) <= 1e-8, q
+
+
 def test_block_single_column(problem, reference):
     print("case block single column", flush=True)
     op = _op(problem)
