@@ -132,6 +132,29 @@ int marie_vie_gmres(marie_vie_op* h, const double* b, double* x, int restart, do
                     int mode, double inner_tol, int* flag, double* relres, int* iter, double* resvec,
                     int resvec_cap, int* nres);
 
+/* GMRES on the adjoint system A^H y = g (A^H applied through mv_AN's
+ * 'transp' branch, mv_AN.m:65-98, the product MATLAB's solvers request as
+ * afun(x,'transp'), iterapp.m:42-58). Arguments and semantics as
+ * marie_vie_gmres_device. */
+int marie_vie_gmres_adjoint_device(marie_vie_op* h, const void* g_dev, void* y_dev, int restart, double tol,
+                                   int maxit, int mode, double inner_tol, int* flag, double* relres,
+                                   int* iter, double* resvec, int resvec_cap, int* nres);
+
+/* Per-voxel gradient of a real objective J(x) with respect to eps_r and
+ * sigma (e_r = eps_r - j sigma/(eo omega), Scat_Solver.m:144) for the JVIE
+ * form II system (JVIE_Solver.m:131-156, :242-244: Mr = 1,
+ * Mc = (e_r-1)/e_r, b = j omega eo Gram Mc Einc) loaded in the handle.
+ * x_dev: the solution; einc_dev: Einc on the DOF index (nidx complex128);
+ * lambda_dev: the adjoint state, A^H lambda = dJ/dconj(x); omega_eo = omega*eo.
+ * Writes grad_eps_dev, grad_sigma_dev (device, L*M*N doubles each, voxel
+ * order i + L(j + M k)): dJ/deps_r = 2 Re S_v, dJ/dsigma = 2 Im S_v/(omega eo)
+ * with S_v = sum_c conj(lambda_vc) (1 - Mc_v)^2 (N x + j omega eo Gram Einc)_vc;
+ * voxels outside the DOF index get 0. No counterpart in the reference (it has
+ * no derivatives); this is the adjoint of its forward solve. */
+int marie_vie_material_gradient_device(marie_vie_op* h, const void* lambda_dev, const void* x_dev,
+                                       const void* einc_dev, double omega_eo, double* grad_eps_dev,
+                                       double* grad_sigma_dev);
+
 /* Block GMRES for `nrhs` right-hand sides of the operator (device buffers B,
  * X: nrhs stacked compact vectors of nidx complex128 each; X holds X0 on
  * entry). The reference solves excitation ports one pgmres at a time

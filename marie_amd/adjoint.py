@@ -1,0 +1,133 @@
+"""Differentiable JVIE solve: a torch.autograd.Function over the device solver.
+
+Forward: per-voxel eps_r and sigma (real, the body's voxels) -> the JVIE
+form II currents x for each excitation (JVIE_Solver.m:131-156, :242-244,
+e_r = eps_r - j sigma/(eo omega) as Scat_Solver.m:144), solved with the
+library's FP64 GMRES. Backward: one adjoint solve A^H lambda = g per
+excitation (marie_vie_gmres_adjoint_device, mv_AN's 'transp' branch) and the
+per-voxel reduction marie_vie_material_gradient_device. The Krylov basis is
+not kept between forward and backward: the adjoint solve builds its own
+(recomputation instead of storage).
+
+Voxel vectors are flat in the library's voxel order i + L (j + M k)
+(MATLAB column-major); `to_voxels` / `from_voxels` convert (L, M, N) arrays.
+torch's gradient for a complex output is 2 dJ/dconj(x); the adjoint right-hand
+side is therefore grad / 2.
+"""
+import numpy as np
+import torch
+
+from . import lib, vie
+
+
+def to_voxels(a):
+    """(L, M, N) array -> flat voxel vector (column-major order)."""
+    return np.asarray(a).ravel(order='F')
+
+
+def from_voxels(v, shape):
+    return np.asarray(v).reshape(shape, order='F')
+
+
+class JVIEProblem:
+    """A body (voxel mask), its operator and excitations, at one frequency.
+
+    op: a lib.Operator with fN set (FP64 enabled); body: (L, M, N) bool mask;
+    Einc: (L, M, N, 3) complex array or a list of them (one per excitation);
+    freq in Hz, dx the voxel size.
+    """
+
+    def __init__(self, op, body, Einc, freq, dx, tol=1e-10, restart=50, maxit=200, device='cuda:0'):
+        self.op = op
+        self.shape = (op.L, op.M, op.N)
+        body = np.asarray(body, bool)
+        if body.shape != self.shape:
+            raise ValueError('body mask shape %s != operator grid %s' % (body.shape, self.shape))
+        n = op.L * op.M * op.N
+        idxS = np.nonzero(to_voxels(body))[0]
+        self.idx = np.concatenate([idxS, n + idxS, 2 * n + idxS])       # JVIE_Solver.m:147-148
+        op.set_index(self.idx)
+        self.body = torch.from_numpy(to_voxels(body)).to(device)
+        em = vie.EMconstants(freq)
+        self.omega_eo = em['omega'] * em['eo']
+        self.Gram = dx ** 3
+        E = Einc if isinstance(Einc, (list, tuple)) else [Einc]
+        self.einc = torch.stack([torch.from_numpy(np.ascontiguousarray(np.asarray(e, np.complex128)
+                                                                        .ravel(order='F')[self.idx]))
+                                 for e in E]).to(device)
+        self.vox = torch.from_numpy(self.idx % n).to(device)
+        self.tol, self.restart, self.maxit = tol, restart, maxit
+        self.device = device
+
+    @property
+    def nrhs(self):
+        return self.einc.shape[0]
+
+    def material(self, eps_r, sigma):
+        """Mc = (e_r - 1)/e_r on the body, 0 in air (form II, Mr = 1)."""
+        e_r = torch.complex(eps_r, -sigma / self.omega_eo)
+        Mc = (e_r - 1) / e_r
+        return torch.where(self.body, Mc, torch.zeros_like(Mc))
+
+    def load_material(self, Mc):
+        Mc_h = from_voxels(Mc.detach().cpu().numpy(), self.shape)
+        self.op.set_material(np.ones(self.shape), Mc_h, self.Gram)
+
+    def rhs(self, Mc):
+        """b = j omega eo Gram Mc Einc on the DOFs (JVIE_Solver.m:151-156 in form II)."""
+        return (1j * self.omega_eo * self.Gram) * Mc[self.vox].unsqueeze(0) * self.einc
+
+
+class _Solve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, eps_r, sigma, prob):
+        Mc = prob.material(eps_r, sigma)
+        prob.load_material(Mc)
+        B = prob.rhs(Mc).contiguous()
+        X = torch.zeros_like(B)
+        torch.cuda.synchronize()
+        for q in range(prob.nrhs):
+            flag, relres, it, rv = prob.op.gmres_device(B[q].data_ptr(), X[q].data_ptr(), prob.restart, prob.tol,
+                                                        prob.maxit)
+            if flag != 0:
+                raise lib.MarieError('forward solve did not converge (relres %.3g)' % relres)
+        ctx.prob = prob
+        ctx.save_for_backward(X, Mc)
+        return X
+
+    @staticmethod
+    def backward(ctx, gX):
+        X, Mc = ctx.saved_tensors
+        prob = ctx.prob
+        prob.load_material(Mc)            # another forward may have replaced it
+        n = Mc.numel()
+        ge = torch.zeros(n, dtype=torch.float64, device=X.device)
+        gs = torch.zeros_like(ge)
+        te = torch.empty_like(ge)
+        ts = torch.empty_like(ge)
+        G = (gX / 2).contiguous()
+        for q in range(prob.nrhs):
+            lam = torch.zeros_like(G[q])
+            torch.cuda.synchronize()
+            flag, relres, it, rv = prob.op.gmres_adjoint_device(G[q].data_ptr(), lam.data_ptr(), prob.restart,
+                                                                prob.tol, prob.maxit)
+            if flag != 0:
+                raise lib.MarieError('adjoint solve did not converge (relres %.3g)' % relres)
+            einc = prob.einc[q].contiguous()
+            prob.op.material_gradient_device(lam.data_ptr(), X[q].contiguous().data_ptr(), einc.data_ptr(),
+                                             prob.omega_eo, te.data_ptr(), ts.data_ptr())
+            ge += te
+            gs += ts
+        return ge, gs, None
+
+
+def jvie_solve(eps_r, sigma, prob):
+    """x (nrhs x nidx complex128) for flat voxel eps_r, sigma (float64 CUDA tensors)."""
+    return _Solve.apply(eps_r, sigma, prob)
+
+
+def interior_field(X, eps_r, sigma, prob):
+    """E = J / tau on the body DOFs, tau = j omega eo (e_r - 1) (JVIE_Solver.m:151)."""
+    e_r = torch.complex(eps_r, -sigma / prob.omega_eo)
+    tau = 1j * prob.omega_eo * (e_r - 1)
+    return X / tau[prob.vox].unsqueeze(0)

@@ -242,6 +242,7 @@ struct Ctx {
     hipStream_t s;
     int64_t n;
     int G;
+    int transp = 0;              // 1: solve with A^H (mv_AN's 'transp' branch)
     Ws ws;
 };
 
@@ -266,12 +267,12 @@ template <typename TV> static int norm(Ctx& c, const c2<TV>* v, double* out) {
 // y = A x with vectors stored as TV; `f32op` selects the complex64 operator
 template <typename TV> static int matvec(Ctx& c, const c2<TV>* x, c2<TV>* y, bool f32op) {
     if constexpr (sizeof(TV) == 4) {
-        return apply(c.h, OP_AN, 0, PREC_F32, x, y, 1, c.s);
+        return apply(c.h, OP_AN, c.transp, PREC_F32, x, y, 1, c.s);
     } else {
-        if (!f32op) return apply(c.h, OP_AN, 0, PREC_F64, x, y, 1, c.s);
+        if (!f32op) return apply(c.h, OP_AN, c.transp, PREC_F64, x, y, 1, c.s);
         const int g = grid_for(c.n);
         hipLaunchKernelGGL((k_axpby<float, double, double>), dim3(g), dim3(KT), 0, c.s, c.ws.xs, 1.0, x, 0.0, x, c.n);
-        int st = apply(c.h, OP_AN, 0, PREC_F32, c.ws.xs, c.ws.ys, 1, c.s);
+        int st = apply(c.h, OP_AN, c.transp, PREC_F32, c.ws.xs, c.ws.ys, 1, c.s);
         if (st) return st;
         hipLaunchKernelGGL((k_axpby<double, float, float>), dim3(g), dim3(KT), 0, c.s, y, 1.0, c.ws.ys, 0.0, c.ws.ys,
                            c.n);
@@ -499,9 +500,10 @@ static int refine(Ctx& c, const c2<double>* b, c2<double>* x, int m, double tol,
 }
 
 int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxit, int mode, double inner_tol,
-          int* flag, double* relres, int* iter, std::vector<double>& resvec) {
+          int* flag, double* relres, int* iter, std::vector<double>& resvec, int transp) {
     Ctx c;
     c.h = h;
+    c.transp = transp;
     c.s = h->stream;
     c.n = h->nidx;
     c.G = grid_for(c.n);
@@ -571,9 +573,9 @@ int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxi
 
 using namespace marie;
 
-extern "C" int marie_vie_gmres_device(marie_vie_op* hh, const void* b, void* x, int restart, double tol, int maxit,
-                                      int mode, double inner_tol, int* flag, double* relres, int* iter,
-                                      double* resvec, int resvec_cap, int* nres) {
+static int gmres_entry(marie_vie_op* hh, const void* b, void* x, int restart, double tol, int maxit, int mode,
+                       double inner_tol, int* flag, double* relres, int* iter, double* resvec, int resvec_cap, int* nres,
+                       int transp) {
     Op* h = reinterpret_cast<Op*>(hh);
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
     MARIE_HIP(hipSetDevice(h->device));
@@ -597,7 +599,7 @@ extern "C" int marie_vie_gmres_device(marie_vie_op* hh, const void* b, void* x, 
     }
     std::vector<double> rv;
     int st = gmres(h, (const c2<double>*)b, (c2<double>*)x, restart, tol, maxit, mode, inner_tol, flag, relres,
-                   iter, rv);
+                   iter, rv, transp);
     if (st) return st;
     if (resvec && resvec_cap > 0) {
         const int nc = (int)rv.size() < resvec_cap ? (int)rv.size() : resvec_cap;
@@ -605,6 +607,20 @@ extern "C" int marie_vie_gmres_device(marie_vie_op* hh, const void* b, void* x, 
     }
     if (nres) *nres = (int)rv.size();
     return MARIE_OK;
+}
+
+extern "C" int marie_vie_gmres_device(marie_vie_op* hh, const void* b, void* x, int restart, double tol, int maxit,
+                                      int mode, double inner_tol, int* flag, double* relres, int* iter,
+                                      double* resvec, int resvec_cap, int* nres) {
+    return gmres_entry(hh, b, x, restart, tol, maxit, mode, inner_tol, flag, relres, iter, resvec, resvec_cap, nres,
+                       0);
+}
+
+extern "C" int marie_vie_gmres_adjoint_device(marie_vie_op* hh, const void* g, void* y, int restart, double tol,
+                                              int maxit, int mode, double inner_tol, int* flag, double* relres,
+                                              int* iter, double* resvec, int resvec_cap, int* nres) {
+    return gmres_entry(hh, g, y, restart, tol, maxit, mode, inner_tol, flag, relres, iter, resvec, resvec_cap, nres,
+                       1);
 }
 
 extern "C" int marie_vie_gmres(marie_vie_op* hh, const double* b, double* x, int restart, double tol, int maxit,

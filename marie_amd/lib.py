@@ -43,6 +43,9 @@ SIGNATURES = [
                                     ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_gmres', _I, [_P, _P, _P, _I, _D, _I, _I, _D, ctypes.POINTER(_I), _DP,
                              ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_gmres_adjoint_device', _I, [_P, _P, _P, _I, _D, _I, _I, _D, ctypes.POINTER(_I), _DP,
+                                            ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_material_gradient_device', _I, [_P, _P, _P, _P, _D, _P, _P]),
     ('marie_vie_block_gmres_device', _I, [_P, _P, _P, _I, _I, _D, _I, _I, _D, _I, ctypes.POINTER(_I), _DP,
                                           ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_block_gmres', _I, [_P, _P, _P, _I, _I, _D, _I, _I, _D, _I, ctypes.POINTER(_I), _DP,
@@ -68,6 +71,14 @@ def load(path=LIB_PATH):
     if not os.path.exists(path):
         raise MarieError('libmarie_vie.so is not built: run `python -m marie_amd.build` '
                          '(or __graft_entry__.build())')
+    # A process must hold one HIP runtime. PyTorch ships its own copy
+    # (soname libamdhip64.so.7, which then also serves this library); loaded
+    # after this library, torch would bring up a second runtime that sees no
+    # GPU. So torch, when installed, is imported first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, res, args in SIGNATURES:
         f = getattr(lib, name)
@@ -204,6 +215,27 @@ class Operator:
                                             maxit, mode, inner_tol, ctypes.byref(flag), ctypes.byref(relres), it,
                                             resvec.ctypes.data_as(_DP), resvec_cap, ctypes.byref(nres)))
         return flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    def gmres_adjoint_device(self, g_ptr, y_ptr, restart=50, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4,
+                             resvec_cap=20000):
+        """GMRES on A^H y = g (device buffers); returns (flag, relres, [inner, outer], resvec)."""
+        flag = ctypes.c_int()
+        relres = ctypes.c_double()
+        it = (ctypes.c_int * 2)()
+        resvec = np.zeros(resvec_cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_gmres_adjoint_device(self._h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(y_ptr), restart,
+                                                    tol, maxit, mode, inner_tol, ctypes.byref(flag),
+                                                    ctypes.byref(relres), it, resvec.ctypes.data_as(_DP), resvec_cap,
+                                                    ctypes.byref(nres)))
+        return flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    def material_gradient_device(self, lam_ptr, x_ptr, einc_ptr, omega_eo, grad_eps_ptr, grad_sigma_ptr):
+        """Per-voxel dJ/deps_r, dJ/dsigma into device buffers (L*M*N doubles each)."""
+        check(load().marie_vie_material_gradient_device(self._h, ctypes.c_void_p(lam_ptr), ctypes.c_void_p(x_ptr),
+                                                        ctypes.c_void_p(einc_ptr), omega_eo,
+                                                        ctypes.c_void_p(grad_eps_ptr),
+                                                        ctypes.c_void_p(grad_sigma_ptr)))
 
     def block_gmres_device(self, B_ptr, X_ptr, nrhs, restart=20, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4,
                            use_graph=False, cgs_passes=1, resvec_cap=20000):

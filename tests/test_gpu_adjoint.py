@@ -1,0 +1,153 @@
+"""Adjoint eps_r / sigma gradients on the device.
+
+- the adjoint GMRES (A^H through mv_AN's 'transp' branch) against
+  oracle.krylov.pgmres on the same operator (same residual history);
+- the per-voxel gradient through the torch.autograd harness against the
+  oracle's adjoint gradient (itself pinned by finite differences on the CPU);
+- the harness end to end against central finite differences of device
+  forward solves, for an objective of the interior field (coefficient of
+  variation of |E_x| over a region plus an L2 misfit), where eps_r and sigma
+  also enter explicitly through E = J / tau: relative error <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+from oracle import adjoint as oadj
+from oracle import fields, krylov, mvp
+from oracle import operators as oop
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(n=8, dx=6.25e-3, f=298.2e6, seed=11):
+    rng = np.random.default_rng(seed)
+    x = (np.arange(n) - (n - 1) / 2) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    r = np.stack([X, Y, Z], axis=-1)
+    body = X ** 2 + Y ** 2 + 1.2 * Z ** 2 < (0.45 * n * dx) ** 2
+    eps = np.where(body, 20 + 40 * rng.random(body.shape), 1.0)
+    sig = np.where(body, 0.1 + 1.5 * rng.random(body.shape), 0.0)
+    em = fields.em_constants(f)
+    E1, _ = fields.plane_wave_excitation(r, [0, 0, em['ko']], em['omega_mu'], [1, 0, 0])
+    E2, _ = fields.plane_wave_excitation(r, [em['ko'], 0, 0], em['omega_mu'], [0, 1, 0])
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    return dict(n=n, dx=dx, f=f, body=body, eps=eps, sig=sig, E=[E1, E2], fN=fN, rng=rng, r=r)
+
+
+@pytest.fixture(scope='module')
+def case():
+    return _setup()
+
+
+def _problem(case, nrhs, tol=1e-12):
+    from marie_amd import adjoint, lib
+    n = case['n']
+    op = lib.Operator(n, n, n)
+    op.set_fN(case['fN'])
+    return adjoint.JVIEProblem(op, case['body'], case['E'][:nrhs], case['f'], case['dx'], tol=tol)
+
+
+def _params(case):
+    import torch
+    from marie_amd import adjoint
+    e = torch.tensor(adjoint.to_voxels(case['eps']), dtype=torch.float64, device='cuda:0', requires_grad=True)
+    s = torch.tensor(adjoint.to_voxels(case['sig']), dtype=torch.float64, device='cuda:0', requires_grad=True)
+    return e, s
+
+
+def test_adjoint_gmres_parity(case):
+    from marie_amd import lib
+    print("case adjoint gmres parity", flush=True)
+    em = fields.em_constants(case['f'])
+    e_r = case['eps'] - 1j * case['sig'] / (em['eo'] * em['omega'])
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(case['E'][0], e_r, case['dx'], case['f'], form=2)
+    g = case['rng'].standard_normal(b.size) + 1j * case['rng'].standard_normal(b.size)
+    AH = lambda v: mvp.mv_AN(v, case['fN'], Mr, Mc, Gram, 'transp', idx)
+    yr, flr, rr, itr, rvr = krylov.pgmres(AH, g, 20, 1e-9, 100)
+    n = case['n']
+    op = lib.Operator(n, n, n)
+    op.set_fN(case['fN'])
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx)
+    import torch
+    gd = torch.from_numpy(g).to('cuda:0')
+    yd = torch.zeros_like(gd)
+    fl, rel, it, rv = op.gmres_adjoint_device(gd.data_ptr(), yd.data_ptr(), restart=20, tol=1e-9, maxit=100)
+    assert fl == flr == 0 and it == itr
+    assert np.max(np.abs(rv - rvr)) <= 1e-9
+    y = yd.cpu().numpy()
+    assert np.linalg.norm(y - yr) / np.linalg.norm(yr) <= 1e-8
+
+
+def test_gradient_matches_oracle(case):
+    import torch
+    from marie_amd import adjoint
+    print("case adjoint vs oracle", flush=True)
+    prob = _problem(case, 1)
+    e, s = _params(case)
+    X = adjoint.jvie_solve(e, s, prob)
+    rng = np.random.default_rng(3)
+    w = torch.tensor(rng.random(X.shape[1]), device='cuda:0')
+    c = torch.tensor(rng.standard_normal(X.shape[1]) + 1j * rng.standard_normal(X.shape[1]), device='cuda:0')
+    c = c * (torch.linalg.norm(X.detach()) / torch.linalg.norm(c))
+    J = torch.sum(w * X[0].abs() ** 2) + torch.real(torch.vdot(c, X[0]))
+    J.backward()
+    # oracle: same objective, g = w x + c / 2
+    x, idx, Mc, einc, Gram, em = oadj.forward(case['eps'], case['sig'], case['E'][0], case['dx'], case['f'],
+                                              case['fN'], tol=1e-12)
+    assert np.linalg.norm(X[0].detach().cpu().numpy() - x) / np.linalg.norm(x) <= 1e-9
+    g = w.cpu().numpy() * x + c.cpu().numpy() / 2
+    lam = oadj.adjoint_state(g, case['fN'], Mc, Gram, idx, tol=1e-12)
+    ge, gs = oadj.material_gradient(lam, x, einc, case['fN'], Mc, Gram, idx, em['omega'] * em['eo'])
+    ge, gs = adjoint.to_voxels(ge), adjoint.to_voxels(gs)
+    got_e, got_s = e.grad.cpu().numpy(), s.grad.cpu().numpy()
+    assert np.max(np.abs(got_e - ge)) <= 1e-8 * np.max(np.abs(ge))
+    assert np.max(np.abs(got_s - gs)) <= 1e-8 * np.max(np.abs(gs))
+    body = adjoint.to_voxels(case['body'])
+    assert np.all(got_e[~body] == 0) and np.all(got_s[~body] == 0)
+
+
+def _field_objective(prob, e, s, target, roi):
+    """CV of |E_x| over the region + misfit to a target field (all excitations)."""
+    import torch
+    from marie_amd import adjoint
+    X = adjoint.jvie_solve(e, s, prob)
+    E = adjoint.interior_field(X, e, s, prob)          # nrhs x nidx
+    nb = E.shape[1] // 3
+    ex = E[:, :nb].abs()[:, roi]
+    cv = torch.std(ex, dim=1) / torch.mean(ex, dim=1)
+    misfit = torch.sum((E - target).abs() ** 2) / torch.sum(target.abs() ** 2)
+    return torch.sum(cv) + 0.5 * misfit
+
+
+def test_autograd_matches_finite_differences(case):
+    import torch
+    print("case adjoint finite differences", flush=True)
+    prob = _problem(case, 2)
+    e, s = _params(case)
+    nb = prob.idx.size // 3
+    rng = np.random.default_rng(7)
+    roi = torch.tensor(rng.random(nb) < 0.5, device='cuda:0')
+    with torch.no_grad():
+        from marie_amd import adjoint
+        X0 = adjoint.jvie_solve(e, s, prob)
+        E0 = adjoint.interior_field(X0, e, s, prob)
+        target = E0 * (1 + 0.3 * torch.tensor(rng.standard_normal(E0.shape), device='cuda:0'))
+    J = _field_objective(prob, e, s, target, roi)
+    J.backward()
+    Jv = float(J.detach())
+    ge, gs = e.grad.cpu().numpy(), s.grad.cpu().numpy()
+    body = np.nonzero(adjoint.to_voxels(case['body']))[0]
+    for v in rng.choice(body, 6, replace=False):
+        for which, grad in (('eps', ge), ('sig', gs)):
+            base = (e if which == 'eps' else s).detach()
+            h = 1e-4 * float(base[v])
+            vals = []
+            for sgn in (1, -1):
+                p = base.clone()
+                p[v] += sgn * h
+                with torch.no_grad():
+                    ee, ss = (p, s.detach()) if which == 'eps' else (e.detach(), p)
+                    vals.append(float(_field_objective(prob, ee, ss, target, roi)))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            assert abs(fd - grad[v]) <= 1e-5 * abs(fd) + 1e-9 * abs(Jv), (which, v, fd, grad[v])
