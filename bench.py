@@ -34,11 +34,15 @@ def log(*a):
 
 
 def kernel_bytes(L, M, N, Px, Py, Pz, nidx, s):
-    """Compulsory HBM bytes of each matvec kernel (s = bytes per complex)."""
+    """Compulsory HBM bytes of each matvec kernel (s = bytes per complex).
+
+    fN is counted as stored: an assembled operator has the circulant's
+    per-axis parity and is kept as its octant [6][Pz/2+1][Py/2+1][Px/2+1].
+    """
     n = L * M * N
     A = 3 * Px * M * N * s
     B = 3 * Px * Py * N * s
-    fN = 6 * Px * Py * Pz * s
+    fN = 6 * (Px // 2 + 1) * (Py // 2 + 1) * (Pz // 2 + 1) * s
     pos = 3 * n * 4
     return {
         'fwd_x': pos + nidx * s + A,
@@ -57,6 +61,8 @@ def main():
     ap.add_argument('--size', type=int, default=128)
     ap.add_argument('--no-solve', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-block', action='store_true', help='skip the 16-excitation block solve')
+    ap.add_argument('--no-adjoint', action='store_true', help='skip the adjoint gradient timing')
     ap.add_argument('--profile-only', action='store_true',
                     help='assemble and run the timed FP64 matvecs only (for rocprofv3 runs)')
     ap.add_argument('--profile-solve', action='store_true',
@@ -132,9 +138,9 @@ def main():
     # tile autotuning (part of the set-up, like an FFT plan)
     tiles = {}
     for prec, name in ((lib.PREC_F64, 'fp64'), (lib.PREC_F32, 'fp32')):
-        em, ms = op.autotune(prec, iters=5)
-        tiles[name] = em
-        log('tiles %s %s  (%s ms)' % (name, em, ' '.join('%.3f' % m for m in ms)))
+        cfg, ms = op.autotune(prec, iters=5)
+        tiles[name] = cfg
+        log('tiles %s %s  (%s ms)' % (name, cfg, ' '.join('%.3f' % m for m in ms)))
 
     if args.profile_solve:
         b = torch.from_numpy(np.ascontiguousarray(Vrhs)).to(dev)
@@ -158,6 +164,16 @@ def main():
     T32 = timed(lib.PREC_F32, K, W)
     mv32 = K / T32
     log('fp32 %.1f matvecs/s (%.3f ms)' % (mv32, 1e3 * T32 / K))
+    # host-buffer entry point (marie_vie_apply): x and y cross PCIe every call,
+    # as in the reference's gpuArray path (mv_AN.m:51, :141); reported beside
+    # `value`, never as it
+    xh = x64.cpu().numpy()
+    op.apply(xh)
+    th = time.perf_counter()
+    for _ in range(10):
+        op.apply(xh)
+    mvh = 10 / (time.perf_counter() - th)
+    log('fp64 host-buffer (PCIe-inclusive) %.1f matvecs/s' % mvh)
 
     # per-kernel durations from HIP events on the launch stream
     Px, Py, Pz = op.P
@@ -200,7 +216,10 @@ def main():
             flag, relres, it, rv = op.gmres_device(b.data_ptr(), xs.data_ptr(), restart=50, tol=tol, maxit=200,
                                                    mode=mode, inner_tol=inner_tol)
             torch.cuda.synchronize()
-            return xs, flag, relres, len(rv) - 1, time.perf_counter() - t
+            # resvec holds one entry per inner iteration plus, in the refined
+            # modes, the true residual after each of the it[1] cycles
+            nit = len(rv) - 1 - (it[1] if mode else 0)
+            return xs, flag, relres, nit, time.perf_counter() - t
 
         # tightly converged FP64 solution: the reference for the error of each 1e-6 solve
         xref, _, rel_ref, it_ref, t_ref = solve(0, 1e-10)
@@ -228,6 +247,65 @@ def main():
             'gmres_iters_fp64': res['fp64'][3], 'gmres_iters_fp32_refined': res['fp32_refined'][3],
             'fp64_reference_solve': {'tol': 1e-10, 'iters': it_ref, 'relres': rel_ref, 's': t_ref},
         }
+
+    if not args.no_solve and not args.no_block:
+        # 16 excitations of the same body (plane waves from seeded directions and
+        # polarisations): one block GMRES vs 16 single solves, all to 1e-6
+        s_blk = 16
+        rng = np.random.default_rng(7)
+        cols = []
+        for q in range(s_blk):
+            dvec = rng.standard_normal(3)
+            dvec /= np.linalg.norm(dvec)
+            pol = np.cross(dvec, rng.standard_normal(3))
+            pol /= np.linalg.norm(pol)
+            Eq, _ = vie.PlaneWave_Excitation(r, em['ko'] * dvec, em['omega_mu'], pol)
+            cols.append(vie.jvie_system(Eq, e_r, r, freq, form=2)[4])
+        Bd = torch.from_numpy(np.ascontiguousarray(np.stack(cols))).to(dev)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        Xseq = torch.zeros_like(Bd)
+        for q in range(s_blk):
+            op.gmres_device(Bd[q].data_ptr(), Xseq[q].data_ptr(), restart=50, tol=1e-6, maxit=200)
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t
+        blk = {'rhs': s_blk, 'sequential_fp64_s': t_seq}
+        for mode in (0, 1):
+            X = torch.zeros_like(Bd)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fl, rr, it, rv = op.block_gmres_device(Bd.data_ptr(), X.data_ptr(), s_blk, restart=40, tol=1e-6,
+                                                   maxit=100, mode=mode, inner_tol=1e-7, use_graph=True)
+            torch.cuda.synchronize()
+            tb = time.perf_counter() - t
+            diff = max((torch.linalg.norm(X[q] - Xseq[q]) / torch.linalg.norm(Xseq[q])).item() for q in range(s_blk))
+            key = 'block_fp64' if mode == 0 else 'block_fp32_products'
+            blk[key] = {'s': tb, 'block_steps_last_cycle_and_cycles': it, 'max_relres': float(rr.max()),
+                        'flags_max': int(fl.max()), 'max_rel_diff_vs_single_solves': diff}
+            log('block gmres mode %d: %.3f s for %d excitations (sequential FP64 %.3f s), cycles %d, max relres %.2e'
+                % (mode, tb, s_blk, t_seq, it[1], rr.max()))
+        out['batching_16_excitations'] = blk
+
+    if not args.no_solve and not args.no_adjoint:
+        from marie_amd import adjoint
+        body = np.abs(Mc) > 1e-12
+        prob = adjoint.JVIEProblem(op, body, Einc, freq, dx, tol=1e-8)
+        ev = torch.tensor(adjoint.to_voxels(eps_r), device=dev, requires_grad=True)
+        sv = torch.tensor(adjoint.to_voxels(sig), device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        X = adjoint.jvie_solve(ev, sv, prob)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t
+        J = torch.sum(X.abs() ** 2)
+        t = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t
+        out['adjoint_gradient'] = {'forward_solve_s': t_fwd, 'adjoint_solve_and_gradient_s': t_bwd,
+                                   'parameters': 2 * int(body.sum()), 'tol': 1e-8}
+        log('adjoint: forward %.3f s, adjoint solve + per-voxel gradient %.3f s (%d parameters)'
+            % (t_fwd, t_bwd, 2 * int(body.sum())))
 
     cpu = None
     if not args.no_cpu:
@@ -261,6 +339,7 @@ def main():
                    'grid': [L, M, N], 'circulant': [Px, Py, Pz], 'dx_m': dx, 'freq_Hz': freq,
                    'phantom_scale': scale, 'phantom': ph, 'non_air_dof': nidx, 'form': 2},
         'fp32_matvecs_per_s': mv32, 'fp32_ms_per_step': 1e3 * T32 / K,
+        'fp64_host_buffer_matvecs_per_s': mvh,
         'assembly_s': t_asm,
         'tiles_elements_per_thread': tiles,
         'kernels': kern,
