@@ -66,6 +66,40 @@ def mv_L(x, fN, L, M, N, Gram, idx, workers=None):
     return out.ravel(order='F')[idx]
 
 
+def mv_K(x, fK, L, M, N, idx, workers=None):
+    """JOut = K x (mv_K.m:55-95): per component the FFT of the input, the cross
+    product with the three fK components, inverse FFT and crop."""
+    J = _scatter(x, idx, L, M, N)
+    P = fK.shape[:3]
+    F = [sfft.fftn(J[..., c], s=P, workers=workers) for c in range(3)]
+    K = [fK[..., c] for c in range(3)]
+    out = [K[2] * F[1] - K[1] * F[2], -K[2] * F[0] + K[0] * F[2], K[1] * F[0] - K[0] * F[1]]
+    Y = np.stack([sfft.ifftn(o, workers=workers)[:L, :M, :N] for o in out], axis=-1)
+    return Y.ravel(order='F')[idx]
+
+
+def toeplitz_apply_dense_k(x, G, idx):
+    """Brute-force K x with the Toeplitz generator G (L x M x N x 3): the
+    kernel at offset d = m - n is G(|d|) with component c negated when d_c < 0
+    (the reflections of circulant_kop.m), applied as out = x cross-like
+    [K3 x2 - K2 x3, -K3 x1 + K1 x3, K2 x1 - K1 x2] (mv_K.m:66-80)."""
+    L, M, N, _ = G.shape
+    J = _scatter(x, idx, L, M, N)
+    Y = np.zeros_like(J)
+    for m in np.ndindex(L, M, N):
+        acc = np.zeros(3, complex)
+        for nn in np.ndindex(L, M, N):
+            d = np.subtract(m, nn)
+            K = G[abs(d[0]), abs(d[1]), abs(d[2])].copy()
+            for c in range(3):
+                if d[c] < 0:
+                    K[c] = -K[c]
+            xv = J[nn]
+            acc += np.array([K[2] * xv[1] - K[1] * xv[2], -K[2] * xv[0] + K[0] * xv[2], K[1] * xv[0] - K[0] * xv[1]])
+        Y[m] = acc
+    return Y.ravel(order='F')[idx]
+
+
 def toeplitz_apply_dense(x, G, Mr, Mc, Gram, idx):
     """Brute-force A_N x from the Toeplitz generator G (L x M x N x 6).
 

@@ -2,7 +2,9 @@
 
 Restates src_vie/src_operators/{getOPERATORS,assembly_nop,circulant_nop,
 gperiodic_coeff_nop,fft_operator,vector_projection,cube_faces,nearby_correct}.m,
-cubatures/{gauss_2d,cubature_nop_sym_near}.m and mexcubatures/cubature_nop.cpp.
+cubatures/{gauss_2d,cubature_nop_sym_near}.m and mexcubatures/cubature_nop.cpp;
+for the K operator assembly_kop.m, circulant_kop.m, gperiodic_coeff_kop.m,
+cubatures/{cubature_kop_near_sym,nearby_correct_kop}.m, mexcubatures/cubature_kop.cpp.
 Test infrastructure only (see oracle/__init__.py).
 """
 import numpy as np
@@ -273,15 +275,133 @@ def fft_operator(C, workers=None):
     return sfft.fftn(C, axes=(0, 1, 2), workers=workers)
 
 
+# ---------------------------------------------------------------- K operator
+# Face pairs of the far/medium K cubature and their signs (cubature_kop.cpp:29-36):
+# opposite faces of each axis, + for the same side, - for opposite sides.
+KOP_OBS = np.array([0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5])
+KOP_SRC = np.array([0, 1, 0, 1, 2, 3, 2, 3, 4, 5, 4, 5])
+KOP_SIGN = np.array([1, -1, -1, 1] * 3, float)
+# Sign of each K component (x, y, z) in the reflected circulant blocks
+# (gperiodic_coeff_kop.m:4-31): component c flips under reflection of axis c.
+GP_COEFF_K = {
+    'L': np.array([-1, 1, 1], float), 'M': np.array([1, -1, 1], float), 'N': np.array([1, 1, -1], float),
+    'LM': np.array([-1, -1, 1], float), 'LN': np.array([-1, 1, -1], float), 'MN': np.array([1, -1, -1], float),
+    'LMN': np.array([-1, -1, -1], float),
+}
+# entries of the 6x6 face-pair matrix that nearby_correct_kop.m:5-77 sets to
+# zero (the coincident-panel "ST" entries) for the eight near offsets (0-based)
+KOP_ST_ZERO = {(0, 0, 0): [(p, p) for p in range(6)], (1, 0, 0): [(0, 1)], (0, 0, 1): [(4, 5)],
+               (0, 1, 0): [(2, 3)], (1, 0, 1): [], (1, 1, 0): [], (0, 1, 1): [], (1, 1, 1): []}
+
+
+def kop_kernel(ko, R):
+    """(1 - exp(-jkR)(1 + jkR)) / R^3 in the form of cubature_kop.cpp:94."""
+    return (1.0 - np.exp(-1j * ko * R) * (1.0 + 1j * ko * R)) / R / R / R
+
+
+def _pair_points(order, i_obs, i_src, R_faces):
+    Np, w, u, v = gauss_2d(order)
+    p1 = R_faces[:, 0, i_obs]; ap = R_faces[:, 1, i_obs] - p1; bp = R_faces[:, 3, i_obs] - p1
+    q1 = R_faces[:, 0, i_src]; aq = R_faces[:, 1, i_src] - q1; bq = R_faces[:, 3, i_src] - q1
+    rp = p1[None, :] + np.outer(u, ap) + np.outer(v, bp)
+    rq = q1[None, :] + np.outer(u, aq) + np.outer(v, bq)
+    return rp, rq, np.outer(w, w)
+
+
+def _kop_pair_integrals(order, rm, i_obs, i_src, R_faces, ko):
+    """sum W G(R) R_c over one face pair for every offset row of rm (K x 3) -> K x 3."""
+    rp, rq, W = _pair_points(order, i_obs, i_src, R_faces)
+    d = rp[None, :, None, :] + rm[:, None, None, :] - rq[None, None, :, :]     # Rmn = rrp - rrq
+    R = np.sqrt(np.sum(d * d, axis=-1))
+    with np.errstate(divide='ignore', invalid='ignore'):
+        G = kop_kernel(ko, R)
+    return np.einsum('kpq,kpqc,pq->kc', G, d, W)
+
+
+def cubature_kop_batch(order, r_m, R_faces, ko, chunk=2048):
+    """IK (K x 3) of cubature_kop.cpp:8-125 for many offsets r_m (source at 0):
+    sum over the 12 face pairs of sign * sum W G(R) R_c (no dx^4 factor)."""
+    r_m = np.atleast_2d(np.asarray(r_m, float))
+    out = np.zeros((r_m.shape[0], 3), complex)
+    for s0 in range(0, r_m.shape[0], chunk):
+        rm = r_m[s0:s0 + chunk]
+        for io, iq, sg in zip(KOP_OBS, KOP_SRC, KOP_SIGN):
+            out[s0:s0 + chunk] += sg * _kop_pair_integrals(order, rm, io, iq, R_faces, ko)
+    return out
+
+
+def cubature_kop_near(order, m0, R_faces, ko, dx):
+    """IK (3,) of cubature_kop_near_sym.m:1-135 for a near offset m0 (0-based, in {0,1}^3):
+    the 21 unique face pairs x 3 components, completion, the coincident-panel
+    zeroing of nearby_correct_kop.m, then the row combination of :120-131."""
+    rm = (np.asarray(m0, float) * dx)[None, :]
+    I = np.zeros((6, 6, 3), complex)
+    for io, iq in zip(INDEX_OBS, INDEX_SRC):
+        if (io, iq) in KOP_ST_ZERO[tuple(m0)]:
+            continue                     # zeroed by the nearby correction (and singular)
+        I[io, iq] = _kop_pair_integrals(order, rm, io, iq, R_faces, ko)[0]
+    for (dr, dc), (sr, sc) in SYM_FILL:
+        I[dr, dc] = I[sr, sc]
+    for (p, q) in KOP_ST_ZERO[tuple(m0)]:
+        I[p, q] = 0.0
+    sgn = np.array([-1, 1, -1, 1, -1, 1], float)
+    comp = [0, 0, 1, 1, 2, 2]
+    J = np.stack([sgn[r] * I[r, :, comp[r]] for r in range(6)])     # 6 x 6
+    return np.array([np.sum(J[:, 1] - J[:, 0]), np.sum(J[:, 3] - J[:, 2]), np.sum(J[:, 5] - J[:, 4])])
+
+
+def assembly_kop(L, M, N, ko, dx):
+    """Toeplitz generator IK_mn (L x M x N x 3) of the K operator (assembly_kop.m:1-121)."""
+    if min(L, M, N) < N_MEDIUM:
+        raise ValueError("assembly_kop needs every grid side >= 5 (n_medium)")
+    R_faces = cube_faces(dx)
+    d = np.array([dx, dx, dx])
+    mx, my, mz = np.meshgrid(np.arange(L), np.arange(M), np.arange(N), indexing='ij')
+    offs = np.stack([mx.ravel(order='F'), my.ravel(order='F'), mz.ravel(order='F')], axis=1)
+    IK = cubature_kop_batch(NP_1D_FAR, offs * d, R_faces, ko).reshape(L, M, N, 3, order='F')
+    mm = np.arange(N_MEDIUM)
+    mx, my, mz = np.meshgrid(mm, mm, mm, indexing='ij')
+    offs = np.stack([mx.ravel(order='F'), my.ravel(order='F'), mz.ravel(order='F')], axis=1)
+    IK[:N_MEDIUM, :N_MEDIUM, :N_MEDIUM] = cubature_kop_batch(NP_1D_MEDIUM, offs * d, R_faces, ko).reshape(
+        N_MEDIUM, N_MEDIUM, N_MEDIUM, 3, order='F')
+    for m0 in [(a, b, c) for c in (0, 1) for b in (0, 1) for a in (0, 1)]:
+        IK[m0] = cubature_kop_near(NP_1D_NEAR, m0, R_faces, ko, dx)
+    return IK * (dx ** 4 / (1j * ko) ** 2 / (4.0 * np.pi))
+
+
+def circulant_kop(G, P=None):
+    """Embed the K Toeplitz octant in the circulant (circulant_kop.m:22-39);
+    P = (Px, Py, Pz) with Px >= 2L-1 embeds it in another circulant size."""
+    L, M, N, _ = G.shape
+    Px, Py, Pz = P or (2 * L, 2 * M, 2 * N)
+    C = np.zeros((Px, Py, Pz, 3), complex)
+    rL, rM, rN = slice(L - 1, 0, -1), slice(M - 1, 0, -1), slice(N - 1, 0, -1)
+    xL, xM, xN = slice(Px - L + 1, Px), slice(Py - M + 1, Py), slice(Pz - N + 1, Pz)
+    C[:L, :M, :N] = G
+    C[xL, :M, :N] = G[rL, :, :] * GP_COEFF_K['L']
+    C[:L, xM, :N] = G[:, rM, :] * GP_COEFF_K['M']
+    C[:L, :M, xN] = G[:, :, rN] * GP_COEFF_K['N']
+    C[xL, xM, :N] = G[rL, rM, :] * GP_COEFF_K['LM']
+    C[xL, :M, xN] = G[rL, :, rN] * GP_COEFF_K['LN']
+    C[:L, xM, xN] = G[:, rM, rN] * GP_COEFF_K['MN']
+    C[xL, xM, xN] = G[rL, rM, rN] * GP_COEFF_K['LMN']
+    return C
+
+
 def get_operators(L, M, N, dx, f, op='N', type_='FFTofC', method='own', ref=None):
-    """getOPERATORS.m:1-127 for op 'N' (ko from :48, dispatch :74-96)."""
-    if op != 'N':
-        raise NotImplementedError("oracle covers the N operator")
+    """getOPERATORS.m:1-127 (ko from :48, dispatch 'N' :74-96, 'K' :101-115)."""
     ko = 2.0 * np.pi / C0 * f
-    G = assembly_nop(L, M, N, ko, dx, method, ref)
+    if op == 'N':
+        G = assembly_nop(L, M, N, ko, dx, method, ref)
+        circ = circulant_nop
+    elif op == 'K':
+        G = assembly_kop(L, M, N, ko, dx)
+        circ = circulant_kop
+    else:
+        raise NotImplementedError("oracle covers the N and K operators")
     if type_ == 'T':
         return G
-    C = circulant_nop(G)
+    C = circ(G)
     if type_ == 'C':
         return C
     return fft_operator(C)

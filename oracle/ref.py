@@ -31,6 +31,8 @@ def _p(a):
 class Ref:
     def __init__(self):
         self.cub = _lib('cubature_nop')
+        self.kop = _lib('cubature_kop')
+        self.kop.ref_cubature_kop.argtypes = [ctypes.c_int] + [_D] * 6 + [ctypes.c_double, _D]
         self.fst = _lib('directfn_st')
         self.fea = _lib('directfn_ea')
         self.fva = _lib('directfn_va')
@@ -59,6 +61,12 @@ class Ref:
         return (out[0::2] + 1j * out[1::2]).reshape(6, 6)
 
     # --- triangle-pair primitives ------------------------------------------
+    def cubature_kop(self, Np, w, u, v, r_m, r_n, R_faces, ko):
+        out = np.zeros(6)
+        args = [_p(x) for x in (w, u, v, r_m, r_n, np.asarray(R_faces).ravel(order='F'))]
+        self.kop.ref_cubature_kop(int(Np), *[a[1] for a in args], float(ko), out.ctypes.data_as(_D))
+        return out[0::2] + 1j * out[1::2]
+
     def _call(self, fn, pts, tail):
         out = np.zeros(2)
         ps = [_p(np.asarray(p, float)) for p in pts]

@@ -4,6 +4,7 @@
 // the reference tree as it lies (see oracle/Makefile). Each gateway below
 // forwards its arguments exactly as the corresponding reference gateway does:
 //   cubature_nop_mex.cpp:47-59 (R_faces 3x4x6 column-major -> [3][4][6]),
+//   cubature_kop_mex.cpp:75 (same argument mapping),
 //   directfn_ws_st_const_mex.cpp, directfn_ws_ea_const_mex.cpp:50,
 //   directfn_ws_va_const_mex.cpp:51, direct_ws_st_const_mex.cpp:49,
 //   direct_ws_ea_const_mex.cpp:34, direct_ws_va_const_mex.cpp:57.
@@ -30,6 +31,25 @@ extern "C" void ref_cubature_nop(int Np, const double* w, const double* u, const
             out[2 * (6 * i + j)] = I[i][j].real();
             out[2 * (6 * i + j) + 1] = I[i][j].imag();
         }
+}
+#endif
+
+#if defined(SHIM_CUBATURE_KOP)
+void cubature_kop(const int Np, const double w[], const double u[], const double v[], const double r_m[3],
+                  const double r_n[3], const double R_faces[3][4][6], const double ko, complex<double> IK_mn[3]);
+// cubature_kop_mex.cpp:75 forwards the same arguments
+extern "C" void ref_cubature_kop(int Np, const double* w, const double* u, const double* v, const double* r_m,
+                                 const double* r_n, const double* R_faces_cm, double ko, double* out /* 3 complex */) {
+    double RR[3][4][6];
+    for (int k = 0; k < 6; k++)
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 3; i++) RR[i][j][k] = R_faces_cm[i + 3 * j + 12 * k];
+    complex<double> I[3];
+    cubature_kop(Np, w, u, v, r_m, r_n, RR, ko, I);
+    for (int c = 0; c < 3; c++) {
+        out[2 * c] = I[c].real();
+        out[2 * c + 1] = I[c].imag();
+    }
 }
 #endif
 

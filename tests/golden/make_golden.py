@@ -7,6 +7,9 @@ Run in a container that has the reference tree and oracle/_ref built
   * the five self-term integrals from DIRECTFN and DEMCEM at Np = 20
     (singular_st/ea/va.m, assembly_nop.m:16);
   * Gauss-Legendre nodes/weights quoted from gauss_1d.m for orders 4, 10, 20.
+and tests/golden/ref_cubature_kop.npz:
+  * cubature_kop 3-vectors (cubature_kop.cpp) at the orders 4/10 of
+    assembly_kop.m:32-66 for offsets without face contact.
 """
 import os
 import re
@@ -22,6 +25,9 @@ from oracle import ref  # noqa: E402
 
 CASES = [(6.25e-3, 298.2e6), (2e-3, 128e6), (2e-3, 298e6), (1.5e-3, 298e6)]
 OFFSETS = [(0, 0, 0), (1, 0, 0), (1, 1, 1), (2, 1, 0), (3, 1, 2), (4, 4, 4), (7, 0, 0), (10, 3, 5)]
+# the K cubature is evaluated on face pairs' Gauss points directly, so offsets
+# whose cubes share a face (coincident points) are left out
+OFFSETS_K = [(1, 1, 0), (1, 1, 1), (2, 1, 0), (3, 1, 2), (4, 4, 4), (7, 0, 0), (10, 3, 5)]
 C0 = 299792458.0
 
 
@@ -59,6 +65,19 @@ def main():
                         **{'gauss_x_%d' % o: gauss_table(o)[0] for o in (4, 10, 20)},
                         **{'gauss_w_%d' % o: gauss_table(o)[1] for o in (4, 10, 20)})
     print('wrote', len(blocks), 'cubature blocks')
+    kb, km = [], []
+    for dx, f in CASES:
+        ko = 2 * np.pi / C0 * f
+        Rf = oop.cube_faces(dx)
+        for order in (4, 10):
+            Np, w, u, v = oop.gauss_2d(order)
+            for m in OFFSETS_K:
+                rm = np.array(m, float) * dx
+                kb.append(R.cubature_kop(Np, w, u, v, rm, np.zeros(3), Rf, ko))
+                km.append([dx, f, order, m[0], m[1], m[2]])
+    np.savez_compressed(os.path.join(os.path.dirname(__file__), 'ref_cubature_kop.npz'),
+                        kop_blocks=np.array(kb), kop_meta=np.array(km))
+    print('wrote', len(kb), 'K cubature vectors')
 
 
 if __name__ == '__main__':
