@@ -42,6 +42,7 @@ extern "C" {
 #define MARIE_OP_AN 0 /* Gram*Mr*x - Mc*N x   (mv_AN.m)  */
 #define MARIE_OP_N 1  /* N x                  (mv_N.m)   */
 #define MARIE_OP_L 2  /* N x - Gram*x         (mv_L.m)   */
+#define MARIE_OP_K 3  /* K x                  (mv_K.m)   */
 
 typedef struct marie_vie_op marie_vie_op;
 
@@ -89,6 +90,23 @@ int marie_vie_assemble_N(marie_vie_op* h, double dx, double ko, const double* si
 
 /* The five self-term integrals the library uses (10 doubles out). */
 int marie_vie_singular_terms(double dx, double ko, double* out10);
+
+/* The K operator (curl of the vector potential: H = Hinc + K J / Gram,
+ * H_field_Kop.m:69-76): fK is 2L x 2M x 2N x 3 complex, the FFT of the
+ * circulant of getOPERATORS(r,f,'K') (getOPERATORS.m:101-115, assembly_kop.m,
+ * circulant_kop.m, fft_operator.m), interleaved complex, column-major. Stored
+ * beside fN; products with MARIE_OP_K (mv_K.m:55-95, no material, no
+ * transposed form). Same octant storage rule as fN (component c odd under
+ * reflection of axis c, gperiodic_coeff_kop.m). */
+int marie_vie_set_fK(marie_vie_op* h, const double* fK);
+int marie_vie_get_fK(marie_vie_op* h, double* fK);
+
+/* Assemble fK on the device for voxel size dx and wavenumber ko > 0
+ * (assembly_kop.m: the 12-pair cubature of cubature_kop.cpp with the 4x4 rule
+ * for every offset and the 10x10 rule below n_medium = 5; the 21-pair 20x20
+ * near rule of cubature_kop_near_sym.m with nearby_correct_kop.m for the 8
+ * nearest offsets; times dx^4/(j ko)^2/(4 pi); circulant_kop.m; FFT). */
+int marie_vie_assemble_K(marie_vie_op* h, double dx, double ko);
 
 /* Material: Mr, Mc (L x M x N complex, host) and the Gram scalar dx^3
  * (mv_AN.m:1 arguments Mr, Mc, Gram; JVIE_Solver.m:135-140, :242-244). */

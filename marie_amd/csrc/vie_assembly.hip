@@ -167,6 +167,167 @@ __global__ __launch_bounds__(AT) void k_pairs(c2<double>* __restrict__ Ju, const
     if (threadIdx.x == 0) Ju[blockIdx.x] = c2<double>{sh[0], sh[AT]};
 }
 
+// ------------------------------------------------------------ K operator
+// Face pairs of the far/medium K cubature and their signs (cubature_kop.cpp:29-36)
+__constant__ int c_kobs[12] = {0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5};
+__constant__ int c_ksrc[12] = {0, 1, 0, 1, 2, 3, 2, 3, 4, 5, 4, 5};
+__constant__ double c_ksign[12] = {1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1};
+
+// (1 - exp(-jkR)(1 + jkR)) / R^3, evaluated as cubature_kop.cpp:94 writes it
+__device__ __forceinline__ c2<double> kgreen(double ko, double R) {
+    double sn, cs;
+    sincos(ko * R, &sn, &cs);
+    const double kr = ko * R;
+    // exp(-jkR) (1 + jkR) = (cos + kR sin) + j (kR cos - sin)
+    const double re = 1.0 - (cs + kr * sn), im = -(kr * cs - sn);
+    return c2<double>{re / R / R / R, im / R / R / R};
+}
+
+// far formula for every offset, 4x4 rule (assembly_kop.m:27-41): one thread per
+// offset, IK_c = sum over 12 pairs of sign * sum W G(R) R_c, R = r_obs - r_src
+__global__ __launch_bounds__(AT) void k_kfar(c2<double>* __restrict__ G, int L, int M, int N, double dx, double ko,
+                                             double coef, Rule rule) {
+    const int64_t nv = (int64_t)L * M * N;
+    const int64_t t = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (t >= nv) return;
+    const int mx = (int)(t % L), my = (int)((t / L) % M), mz = (int)(t / ((int64_t)L * M));
+    const double rm[3] = {mx * dx, my * dx, mz * dx};
+    double ar[3] = {0.0, 0.0, 0.0}, ai[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < 12; ++i) {
+        double op[3], ap[3], bp[3], oq[3], aq[3], bq[3];
+        face_geom(c_kobs[i], dx, op, ap, bp);
+        face_geom(c_ksrc[i], dx, oq, aq, bq);
+        double pr[3] = {0.0, 0.0, 0.0}, pi[3] = {0.0, 0.0, 0.0};
+        for (int ip = 0; ip < rule.np; ++ip) {
+            double rp[3];
+            for (int q = 0; q < 3; ++q) rp[q] = op[q] + ap[q] * rule.u[ip] + bp[q] * rule.v[ip] + rm[q];
+            for (int iq = 0; iq < rule.np; ++iq) {
+                double d[3], d2 = 0.0;
+                for (int q = 0; q < 3; ++q) {
+                    d[q] = rp[q] - (oq[q] + aq[q] * rule.u[iq] + bq[q] * rule.v[iq]);
+                    d2 += d[q] * d[q];
+                }
+                // coincident points occur only for offsets the near rule replaces
+                if (d2 > 0.0) {
+                    const double Wt = rule.w[ip] * rule.w[iq];
+                    const c2<double> g = kgreen(ko, sqrt(d2));
+                    for (int q = 0; q < 3; ++q) {
+                        pr[q] += Wt * g.x * d[q];
+                        pi[q] += Wt * g.y * d[q];
+                    }
+                }
+            }
+        }
+        for (int q = 0; q < 3; ++q) {
+            ar[q] += c_ksign[i] * pr[q];
+            ai[q] += c_ksign[i] * pi[q];
+        }
+    }
+    for (int q = 0; q < 3; ++q) G[t + q * nv] = c2<double>{ar[q] * coef, ai[q] * coef};
+}
+
+// one workgroup per (task, pair): sum W G(R) R_c (3 complex) of one face pair;
+// near = 0 takes the 12 far pairs, near = 1 the 21 unique pairs
+__global__ __launch_bounds__(AT) void k_kpairs(c2<double>* __restrict__ Jk, const int* __restrict__ offs, double dx,
+                                               double ko, Rule rule, int near) {
+    const int npair = near ? 21 : 12;
+    const int task = blockIdx.x / npair, i = blockIdx.x % npair;
+    const double rm[3] = {offs[3 * task] * dx, offs[3 * task + 1] * dx, offs[3 * task + 2] * dx};
+    double op[3], ap[3], bp[3], oq[3], aq[3], bq[3];
+    face_geom(near ? c_obs[i] : c_kobs[i], dx, op, ap, bp);
+    face_geom(near ? c_src[i] : c_ksrc[i], dx, oq, aq, bq);
+    const int np = rule.np;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int e = threadIdx.x; e < np * np; e += AT) {
+        const int ip = e / np, iq = e % np;
+        double d[3], d2 = 0.0;
+        for (int q = 0; q < 3; ++q) {
+            const double rp = op[q] + ap[q] * rule.u[ip] + bp[q] * rule.v[ip] + rm[q];
+            const double rq = oq[q] + aq[q] * rule.u[iq] + bq[q] * rule.v[iq];
+            d[q] = rp - rq;
+            d2 += d[q] * d[q];
+        }
+        // coincident points only on the coincident-panel pairs, which
+        // nearby_correct_kop.m sets to zero
+        if (d2 > 0.0) {
+            const double Wt = rule.w[ip] * rule.w[iq];
+            const c2<double> g = kgreen(ko, sqrt(d2));
+            for (int q = 0; q < 3; ++q) {
+                acc[2 * q] += Wt * g.x * d[q];
+                acc[2 * q + 1] += Wt * g.y * d[q];
+            }
+        }
+    }
+    __shared__ double sh[6][AT];
+    for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int o = AT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int q = 0; q < 6; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) Jk[(int64_t)blockIdx.x * 3 + q] = c2<double>{sh[2 * q][0], sh[2 * q + 1][0]};
+}
+
+// medium offsets (tasks 0..nmed-1, far formula) and the 8 near offsets
+// (cubature_kop_near_sym.m: completion :67-81, nearby_correct_kop.m
+// coincident-panel zeroing, rows J(r,:) = s_r I(r,:,c_r), :120-131); one
+// thread per task, then scaled into G.
+__global__ void k_kfix(c2<double>* __restrict__ G, const c2<double>* __restrict__ Jmed,
+                       const c2<double>* __restrict__ Jnear, const int* __restrict__ offs, int nmed, int L, int M,
+                       int N, double coef) {
+    const int task = blockIdx.x * blockDim.x + threadIdx.x;
+    if (task >= nmed + 8) return;
+    const int mx = offs[3 * task], my = offs[3 * task + 1], mz = offs[3 * task + 2];
+    const int64_t nv = (int64_t)L * M * N;
+    const int64_t vox = mx + (int64_t)L * (my + (int64_t)M * mz);
+    double ir[3] = {0.0, 0.0, 0.0}, ii[3] = {0.0, 0.0, 0.0};
+    if (task < nmed) {
+        if (mx < 2 && my < 2 && mz < 2) return;      // replaced by the near rule
+        for (int i = 0; i < 12; ++i)
+            for (int q = 0; q < 3; ++q) {
+                const c2<double> v = Jmed[((int64_t)task * 12 + i) * 3 + q];
+                ir[q] += c_ksign[i] * v.x;
+                ii[q] += c_ksign[i] * v.y;
+            }
+    } else {
+        const int t8 = task - nmed;
+        c2<double> I[6][6][3];
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b < 6; ++b)
+                for (int q = 0; q < 3; ++q) I[a][b][q] = c2<double>{0.0, 0.0};
+        for (int i = 0; i < 21; ++i)
+            for (int q = 0; q < 3; ++q) I[c_obs[i]][c_src[i]][q] = Jnear[((int64_t)t8 * 21 + i) * 3 + q];
+        // completion by translation (cubature_kop_near_sym.m:67-81, as cubature_nop.cpp:96-114)
+        const int fill[15][4] = {{1, 1, 0, 0}, {2, 0, 1, 3}, {2, 1, 0, 3}, {3, 0, 1, 2}, {3, 1, 0, 2},
+                                 {3, 3, 2, 2}, {4, 0, 1, 5}, {4, 1, 0, 5}, {4, 2, 3, 5}, {4, 3, 2, 5},
+                                 {5, 0, 1, 4}, {5, 1, 0, 4}, {5, 2, 3, 4}, {5, 3, 2, 4}, {5, 5, 4, 4}};
+        for (int f = 0; f < 15; ++f)
+            for (int q = 0; q < 3; ++q) I[fill[f][0]][fill[f][1]][q] = I[fill[f][2]][fill[f][3]][q];
+        // coincident-panel entries set to zero (nearby_correct_kop.m:5-77)
+        if (mx == 0 && my == 0 && mz == 0) {
+            for (int a = 0; a < 6; ++a)
+                for (int q = 0; q < 3; ++q) I[a][a][q] = c2<double>{0.0, 0.0};
+        } else if (mx == 1 && my == 0 && mz == 0) {
+            for (int q = 0; q < 3; ++q) I[0][1][q] = c2<double>{0.0, 0.0};
+        } else if (mx == 0 && my == 1 && mz == 0) {
+            for (int q = 0; q < 3; ++q) I[2][3][q] = c2<double>{0.0, 0.0};
+        } else if (mx == 0 && my == 0 && mz == 1) {
+            for (int q = 0; q < 3; ++q) I[4][5][q] = c2<double>{0.0, 0.0};
+        }
+        // IK_c = sum_r s_r (I(r, 2c+1, c_r) - I(r, 2c, c_r)), s = -,+ per axis, c_r = r/2
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 6; ++r) {
+                const double sg = (r & 1) ? 1.0 : -1.0;
+                const int cr = r >> 1;
+                ir[c] += sg * (I[r][2 * c + 1][cr].x - I[r][2 * c][cr].x);
+                ii[c] += sg * (I[r][2 * c + 1][cr].y - I[r][2 * c][cr].y);
+            }
+    }
+    for (int q = 0; q < 3; ++q) G[vox + q * nv] = c2<double>{ir[q] * coef, ii[q] * coef};
+}
+
 // ----------------------------------------------------------- self-terms
 // Five integrals I = int_Sp int_Sq exp(-jkR)/R over squares of side a, by the
 // convolution reduction to the difference vector (see header of this file):
@@ -357,9 +518,14 @@ __global__ void k_fixup(c2<double>* __restrict__ G, const c2<double>* __restrict
 }
 
 // circulant embedding (circulant_nop.m:23-40) into fN buffer [6][Pz][Py][Px]
+// kind 0: the 6 tensor components of N (signs of gperiodic_coeff_nop.m: a
+// component flips once per reflected axis among its two indices); kind 1: the
+// 3 components of K (gperiodic_coeff_kop.m: component c flips when axis c is
+// reflected)
 __global__ void k_circulant(const c2<double>* __restrict__ G, c2<double>* __restrict__ C, int L, int M, int N,
-                            int Px, int Py, int Pz) {
-    const int64_t tot = (int64_t)Px * Py * Pz * 6;
+                            int Px, int Py, int Pz, int kind) {
+    const int nc = kind ? 3 : 6;
+    const int64_t tot = (int64_t)Px * Py * Pz * nc;
     const int A[6] = {0, 0, 0, 1, 1, 2}, Bc[6] = {0, 1, 2, 1, 2, 2};
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(e % Px);
@@ -375,9 +541,13 @@ __global__ void k_circulant(const c2<double>* __restrict__ G, c2<double>* __rest
         if (!zero) {
             v = G[si + (int64_t)L * (sj + (int64_t)M * (sk + (int64_t)N * c))];
             int flips = 0;
-            if (ri) flips += (A[c] == 0) + (Bc[c] == 0);
-            if (rj) flips += (A[c] == 1) + (Bc[c] == 1);
-            if (rk) flips += (A[c] == 2) + (Bc[c] == 2);
+            if (kind) {
+                flips = (ri && c == 0) + (rj && c == 1) + (rk && c == 2);
+            } else {
+                if (ri) flips += (A[c] == 0) + (Bc[c] == 0);
+                if (rj) flips += (A[c] == 1) + (Bc[c] == 1);
+                if (rk) flips += (A[c] == 2) + (Bc[c] == 2);
+            }
             if (flips & 1) v = c2<double>{-v.x, -v.y};
         }
         C[e] = v;
@@ -613,7 +783,8 @@ int assemble_N(Op* h, double dx, double ko, const double* singular) {
     hipLaunchKernelGGL(k_pairs, dim3(nmed * 21), dim3(AT), 0, s, Ju, offs, dx, ko, r10.r);
     hipLaunchKernelGGL(k_pairs, dim3(8 * 21), dim3(AT), 0, s, Ju + nmed * 21, offs + 3 * nmed, dx, ko, r20.r);
     hipLaunchKernelGGL(k_fixup, dim3(1), dim3(192), 0, s, G, Ju, offs, nmed, sing, L, M, N, dx);
-    hipLaunchKernelGGL(k_circulant, dim3(4096), dim3(256), 0, s, G, (c2<double>*)full, L, M, N, h->Px, h->Py, h->Pz);
+    hipLaunchKernelGGL(k_circulant, dim3(4096), dim3(256), 0, s, G, (c2<double>*)full, L, M, N, h->Px, h->Py, h->Pz,
+                       0);
     TRYH(hipGetLastError());
     TRY(fft3_forward_inplace(h, (c2<double>*)full, 6, s));
     TRYH(hipStreamSynchronize(s));
@@ -621,6 +792,81 @@ int assemble_N(Op* h, double dx, double ko, const double* singular) {
         void* f = full;
         full = nullptr;          // ownership passes to the handle
         TRY(store_fN(h, f));
+    }
+#undef TRY
+#undef TRYH
+    cleanup();
+    return OK;
+}
+
+int assemble_K(Op* h, double dx, double ko) {
+    const int L = h->L, M = h->M, N = h->N;
+    if (h->Px < 2 * L - 1 || h->Py < 2 * M - 1 || h->Pz < 2 * N - 1) {
+        set_error("assembly needs a circulant of at least (2L-1) x (2M-1) x (2N-1)");
+        return ERR_ARG;
+    }
+    if (L < 5 || M < 5 || N < 5) {
+        set_error("assembly needs every grid side >= 5 (n_medium of assembly_kop.m)");
+        return ERR_ARG;
+    }
+    if (!(dx > 0) || !std::isfinite(ko) || !(ko > 0)) { set_error("invalid dx or ko (ko > 0)"); return ERR_ARG; }
+    hipStream_t s = h->stream;
+    const int64_t nv = (int64_t)L * M * N;
+    const size_t nf = (size_t)3 * h->Px * h->Py * h->Pz;
+    void* full = nullptr;
+    if (hipMalloc(&full, nf * sizeof(c2<double>)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("out of device memory for fK");
+        return ERR_ALLOC;
+    }
+    c2<double>* G = nullptr;
+    c2<double>* Jm = nullptr;
+    c2<double>* Jn = nullptr;
+    int* offs = nullptr;
+    DevRule r4, r10, r20;
+    int st = OK;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s);
+        if (G) (void)hipFree(G);
+        if (Jm) (void)hipFree(Jm);
+        if (Jn) (void)hipFree(Jn);
+        if (offs) (void)hipFree(offs);
+        if (full) (void)hipFree(full);
+        if (r4.buf) (void)hipFree(r4.buf);
+        if (r10.buf) (void)hipFree(r10.buf);
+        if (r20.buf) (void)hipFree(r20.buf);
+    };
+#define TRY(x) do { st = (x); if (st) { cleanup(); return st; } } while (0)
+#define TRYH(x) do { if ((x) != hipSuccess) { (void)hipGetLastError(); set_error(#x " failed"); cleanup(); return ERR_HIP; } } while (0)
+    const int nmed = 125;
+    TRYH(hipMalloc(&G, nv * 3 * sizeof(c2<double>)));
+    TRYH(hipMalloc(&Jm, (size_t)nmed * 12 * 3 * sizeof(c2<double>)));
+    TRYH(hipMalloc(&Jn, (size_t)8 * 21 * 3 * sizeof(c2<double>)));
+    TRYH(hipMalloc(&offs, (size_t)(nmed + 8) * 3 * sizeof(int)));
+    std::vector<int> ho;
+    for (int k = 0; k < 5; ++k)
+        for (int j = 0; j < 5; ++j)
+            for (int i = 0; i < 5; ++i) { ho.push_back(i); ho.push_back(j); ho.push_back(k); }
+    for (int code = 0; code < 8; ++code) { ho.push_back(code >> 2); ho.push_back((code >> 1) & 1); ho.push_back(code & 1); }
+    TRYH(hipMemcpyAsync(offs, ho.data(), ho.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    TRY(make_rule(4, r4));
+    TRY(make_rule(10, r10));
+    TRY(make_rule(20, r20));
+    // dx^4 / (j ko)^2 / (4 pi) (assembly_kop.m:117) is real
+    const double coef = -(dx * dx * dx * dx) / (ko * ko) / (4.0 * M_PI);
+    hipLaunchKernelGGL(k_kfar, dim3((unsigned)((nv + AT - 1) / AT)), dim3(AT), 0, s, G, L, M, N, dx, ko, coef, r4.r);
+    hipLaunchKernelGGL(k_kpairs, dim3(nmed * 12), dim3(AT), 0, s, Jm, offs, dx, ko, r10.r, 0);
+    hipLaunchKernelGGL(k_kpairs, dim3(8 * 21), dim3(AT), 0, s, Jn, offs + 3 * nmed, dx, ko, r20.r, 1);
+    hipLaunchKernelGGL(k_kfix, dim3(1), dim3(192), 0, s, G, Jm, Jn, offs, nmed, L, M, N, coef);
+    hipLaunchKernelGGL(k_circulant, dim3(4096), dim3(256), 0, s, G, (c2<double>*)full, L, M, N, h->Px, h->Py, h->Pz,
+                       1);
+    TRYH(hipGetLastError());
+    TRY(fft3_forward_inplace(h, (c2<double>*)full, 3, s));
+    TRYH(hipStreamSynchronize(s));
+    {
+        void* f = full;
+        full = nullptr;          // ownership passes to the handle
+        TRY(store_circ(h, 1, f));
     }
 #undef TRY
 #undef TRYH

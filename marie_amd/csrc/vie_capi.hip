@@ -118,7 +118,7 @@ int marie_vie_destroy(marie_vie_op* h) {
     if (!h) return MARIE_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* bufs[] = {h->fN64, h->fN32, h->tw64, h->tw32, h->Mr64, h->Mc64, h->Mr32, h->Mc32,
+    void* bufs[] = {h->fN64, h->fN32, h->fK64, h->fK32, h->tw64, h->tw32, h->Mr64, h->Mc64, h->Mr32, h->Mc32,
                     h->pos, h->idx, h->A, h->B};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -197,6 +197,47 @@ int marie_vie_get_fN(marie_vie_op* h, double* fN) {
     return st;
 }
 
+int marie_vie_set_fK(marie_vie_op* h, const double* fK) {
+    CHECK_H(h);
+    if (!fK) { set_error("null fK"); return MARIE_ERR_ARG; }
+    const size_t n = (size_t)3 * h->Px * h->Py * h->Pz;
+    for (size_t i = 0; i < 2 * n; ++i)
+        if (!std::isfinite(fK[i])) { set_error("fK has non-finite entries"); return MARIE_ERR_NONFINITE; }
+    void* full = nullptr;
+    if (hipMalloc(&full, n * sizeof(c2<double>)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("out of device memory for fK");
+        return MARIE_ERR_ALLOC;
+    }
+    if (hipMemcpy(full, fK, n * sizeof(c2<double>), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(full);
+        set_error("fK upload failed");
+        return MARIE_ERR_HIP;
+    }
+    return store_circ(h, 1, full);
+}
+
+int marie_vie_get_fK(marie_vie_op* h, double* fK) {
+    CHECK_H(h);
+    if (!h->have_fK || !h->fK64) { set_error("no FP64 fK on the handle"); return MARIE_ERR_STATE; }
+    const size_t n = (size_t)3 * h->Px * h->Py * h->Pz;
+    void* full = nullptr;
+    MARIE_HIP(hipMalloc(&full, n * sizeof(c2<double>)));
+    int st = load_circ(h, 1, full);
+    if (st == OK && hipMemcpyAsync(fK, full, n * sizeof(c2<double>), hipMemcpyDeviceToHost, h->stream) != hipSuccess) {
+        set_error("fK download failed");
+        st = MARIE_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(full);
+    return st;
+}
+
+int marie_vie_assemble_K(marie_vie_op* h, double dx, double ko) {
+    CHECK_H(h);
+    return assemble_K(h, dx, ko);
+}
+
 int marie_vie_set_material(marie_vie_op* h, const double* Mr, const double* Mc, double Gram) {
     CHECK_H(h);
     if (!Mr || !Mc) { set_error("null material"); return MARIE_ERR_ARG; }
@@ -245,12 +286,12 @@ int marie_vie_set_index(marie_vie_op* h, const int64_t* idx, int64_t nidx, int o
 
 int marie_vie_apply_device(marie_vie_op* h, int op, int transp, int prec, const void* x, void* y, int nrhs) {
     CHECK_H(h);
-    if (op < MARIE_OP_AN || op > MARIE_OP_L || (prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || nrhs < 1 ||
+    if (op < MARIE_OP_AN || op > MARIE_OP_K || (prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || nrhs < 1 ||
         (h->nidx > 0 && (!x || !y))) {
         set_error("invalid apply arguments");
         return MARIE_ERR_ARG;
     }
-    if (!h->have_fN || !h->have_idx || (op == MARIE_OP_AN && !h->have_mat)) {
+    if (!(op == MARIE_OP_K ? h->have_fK : h->have_fN) || !h->have_idx || (op == MARIE_OP_AN && !h->have_mat)) {
         set_error("operator, index or material not set");
         return MARIE_ERR_STATE;
     }
@@ -260,8 +301,8 @@ int marie_vie_apply_device(marie_vie_op* h, int op, int transp, int prec, const 
 
 int marie_vie_apply(marie_vie_op* h, int op, int transp, const double* x, double* y, int nrhs) {
     CHECK_H(h);
-    if (nrhs < 1 || op < MARIE_OP_AN || op > MARIE_OP_L) { set_error("invalid apply arguments"); return MARIE_ERR_ARG; }
-    if (!h->have_fN || !h->have_idx || (op == MARIE_OP_AN && !h->have_mat)) {
+    if (nrhs < 1 || op < MARIE_OP_AN || op > MARIE_OP_K) { set_error("invalid apply arguments"); return MARIE_ERR_ARG; }
+    if (!(op == MARIE_OP_K ? h->have_fK : h->have_fN) || !h->have_idx || (op == MARIE_OP_AN && !h->have_mat)) {
         set_error("operator, index or material not set");
         return MARIE_ERR_STATE;
     }

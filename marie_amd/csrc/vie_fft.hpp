@@ -246,6 +246,43 @@ __device__ void tile_fft(c2<T>* buf, const c2<T>* tw, const AxisPlan& pl, int lg
         }                                                                  \
     } while (0)
 
+// The product of the z pass at one frequency, in place on the three transformed
+// components x0, x1, x2. f points at the operator entry of component 0 (the
+// others follow at stride fs); OCT: the entry was read from the octant of a
+// circulant with per-axis parity and fx, fy, fz say which axes were mirrored.
+//   PROD 0, the N operator: symmetric 3x3 tensor of the 6 components
+//     (xx, xy, xz, yy, yz, zz; mv_AN.m:108-123), xy odd under x and y
+//     reflection, xz under x and z, yz under y and z;
+//   PROD 1, the K operator: y = [Kz x1 - Ky x2, -Kz x0 + Kx x2, Ky x0 - Kx x1]
+//     (mv_K.m:66-80), component c odd under reflection of axis c.
+template <typename T, int PROD, bool OCT>
+__device__ __forceinline__ void freq_product(const c2<T>* __restrict__ f, int64_t fs, bool fx, bool fy, bool fz,
+                                             c2<T>& x0, c2<T>& x1, c2<T>& x2) {
+    if constexpr (PROD == 0) {
+        c2<T> f0 = f[0], f1 = f[fs], f2 = f[2 * fs], f3 = f[3 * fs], f4 = f[4 * fs], f5 = f[5 * fs];
+        if (OCT) {
+            if (fx != fy) f1 = c2<T>{-f1.x, -f1.y};
+            if (fx != fz) f2 = c2<T>{-f2.x, -f2.y};
+            if (fy != fz) f4 = c2<T>{-f4.x, -f4.y};
+        }
+        const c2<T> a = x0, b = x1, c = x2;
+        x0 = cadd(cadd(cmul(f0, a), cmul(f1, b)), cmul(f2, c));
+        x1 = cadd(cadd(cmul(f1, a), cmul(f3, b)), cmul(f4, c));
+        x2 = cadd(cadd(cmul(f2, a), cmul(f4, b)), cmul(f5, c));
+    } else {
+        c2<T> k0 = f[0], k1 = f[fs], k2 = f[2 * fs];
+        if (OCT) {
+            if (fx) k0 = c2<T>{-k0.x, -k0.y};
+            if (fy) k1 = c2<T>{-k1.x, -k1.y};
+            if (fz) k2 = c2<T>{-k2.x, -k2.y};
+        }
+        const c2<T> a = x0, b = x1, c = x2;
+        x0 = csub(cmul(k2, b), cmul(k1, c));
+        x1 = csub(cmul(k0, c), cmul(k2, a));
+        x2 = csub(cmul(k1, a), cmul(k0, b));
+    }
+}
+
 // Dispatch order pairing mirror images: 0, 1, n-1, 2, n-2, ..., n/2 (n even).
 __device__ __forceinline__ int mirror_order2(int y, int n) {
     if (y == 0 || (n & 1)) return y;

@@ -19,7 +19,7 @@ enum Status {
 };
 
 enum Precision { PREC_F64 = 1, PREC_F32 = 2 };
-enum OpKind { OP_AN = 0, OP_N = 1, OP_L = 2 };
+enum OpKind { OP_AN = 0, OP_N = 1, OP_L = 2, OP_K = 3 };
 
 void set_error(const std::string& msg);
 const char* get_error();
@@ -48,6 +48,11 @@ struct Op {
     void* fN64 = nullptr;
     void* fN32 = nullptr;
     bool fN_oct = false;
+    // fK: the K operator, c2<T>[3][Pz][Py][Px] or its octant (fK_oct)
+    void* fK64 = nullptr;
+    void* fK32 = nullptr;
+    bool fK_oct = false;
+    bool have_fK = false;
     void* tw64 = nullptr;           // 3 axis tables, offsets 0, Px, Px+Py
     void* tw32 = nullptr;
     void* Mr64 = nullptr;           // c2<double>[N][M][L]
@@ -85,6 +90,9 @@ int build_plan(int P, AxisPlan* pl);
 int upload_twiddles(Op* h);
 int store_fN(Op* h, void* full64);   // takes ownership of a device complex128 fN
 int load_fN(Op* h, void* out64);     // full complex128 fN into a device buffer
+int store_circ(Op* h, int kind, void* full64);   // kind 0: fN, 1: fK
+int load_circ(Op* h, int kind, void* out64);
+int assemble_K(Op* h, double dx, double ko);
 int autotune(Op* h, int prec, int iters, double* best_ms5);
 
 }  // namespace marie

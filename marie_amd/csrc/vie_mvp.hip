@@ -111,7 +111,7 @@ __global__ __launch_bounds__(NT) void k_strided(const c2<T>* __restrict__ src, c
 // (gperiodic_coeff_nop.m), so its DFT has the same parity:
 // F(P-k) = s F(k) with s = -1 iff exactly one of the component's two indices is
 // the reflected axis (xy: x,y; xz: x,z; yz: y,z).
-template <typename T, int EMAX, int RSET, bool OCT>
+template <typename T, int EMAX, int RSET, bool OCT, int PROD>
 __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __restrict__ fN,
                                           const c2<T>* __restrict__ tw, AxisPlan pl, int Px, int Py,
                                           int N, int lg, int nrhs) {
@@ -167,25 +167,11 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
         if (e < tot) {
             const int b = e & msk, k = e >> lg;
             if (b < nx) {
-                c2<T> f0, f1, f2, f3, f4, f5;
-                if (OCT) {
-                    const int i = i0 + b;
-                    const bool fx = i > hx, fz = k > hz;
-                    const int64_t o = (int64_t)(fz ? P - k : k) * fplane + frow + (fx ? Px - i : i);
-                    f0 = fN[o]; f1 = fN[o + fstride]; f2 = fN[o + 2 * fstride];
-                    f3 = fN[o + 3 * fstride]; f4 = fN[o + 4 * fstride]; f5 = fN[o + 5 * fstride];
-                    if (fx != fy) f1 = c2<T>{-f1.x, -f1.y};
-                    if (fx != fz) f2 = c2<T>{-f2.x, -f2.y};
-                    if (fy != fz) f4 = c2<T>{-f4.x, -f4.y};
-                } else {
-                    const int64_t o = k * fplane + frow + b;
-                    f0 = fN[o]; f1 = fN[o + fstride]; f2 = fN[o + 2 * fstride];
-                    f3 = fN[o + 3 * fstride]; f4 = fN[o + 4 * fstride]; f5 = fN[o + 5 * fstride];
-                }
-                const c2<T> x0 = X[0][q], x1 = X[1][q], x2 = X[2][q];
-                X[0][q] = cadd(cadd(cmul(f0, x0), cmul(f1, x1)), cmul(f2, x2));
-                X[1][q] = cadd(cadd(cmul(f1, x0), cmul(f3, x1)), cmul(f4, x2));
-                X[2][q] = cadd(cadd(cmul(f2, x0), cmul(f4, x1)), cmul(f5, x2));
+                const int i = i0 + b;
+                const bool fx = OCT && i > hx, fz = OCT && k > hz;
+                const int64_t o = OCT ? (int64_t)(fz ? P - k : k) * fplane + frow + (fx ? Px - i : i)
+                                      : k * fplane + frow + b;
+                freq_product<T, PROD, OCT>(fN + o, fstride, fx, fy, fz, X[0][q], X[1][q], X[2][q]);
             }
         }
     }
@@ -206,10 +192,27 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
     }
 }
 
-// ------------------------------------------------------ fN storage (octant)
+// ------------------------------------------------------ operator storage (octant)
+// kind 0: the N operator's 6 tensor components (xx, xy, xz, yy, yz, zz), odd
+// under reflection of an axis iff exactly one of the component's indices is
+// that axis (gperiodic_coeff_nop.m); kind 1: the K operator's 3 vector
+// components, component c odd under reflection of axis c (gperiodic_coeff_kop.m).
+__device__ __forceinline__ void parity(int kind, int m, bool& ox, bool& oy, bool& oz) {
+    if (kind == 0) {
+        ox = (m == 1 || m == 2);
+        oy = (m == 1 || m == 4);
+        oz = (m == 2 || m == 4);
+    } else {
+        ox = m == 0;
+        oy = m == 1;
+        oz = m == 2;
+    }
+}
+
 // max |F(k) - s F(P-k)| over all entries (per-block partial maxima), and max |F|
-__global__ void k_sym_check(const c2<double>* __restrict__ F, int Px, int Py, int Pz, double* __restrict__ part) {
-    const int64_t tot = (int64_t)Px * Py * Pz * 6;
+__global__ void k_sym_check(const c2<double>* __restrict__ F, int Px, int Py, int Pz, int kind, int nc,
+                            double* __restrict__ part) {
+    const int64_t tot = (int64_t)Px * Py * Pz * nc;
     double md = 0.0, ma = 0.0;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(e % Px);
@@ -218,13 +221,13 @@ __global__ void k_sym_check(const c2<double>* __restrict__ F, int Px, int Py, in
         const int m = (int)(e / ((int64_t)Px * Py * Pz));
         const int mi = i ? Px - i : 0, mj = j ? Py - j : 0, mk = k ? Pz - k : 0;
         const c2<double> a = F[e];
-        // reflect each axis separately (parity per axis)
-        const int ax[6] = {0, 1, 1, 0, 0, 0}, ay[6] = {0, 1, 0, 0, 1, 0}, az[6] = {0, 0, 1, 0, 1, 0};
+        bool ox, oy, oz;
+        parity(kind, m, ox, oy, oz);
         const int64_t base = (int64_t)m * Px * Py * Pz;
         const c2<double> bx = F[base + ((int64_t)k * Py + j) * Px + mi];
         const c2<double> by = F[base + ((int64_t)k * Py + mj) * Px + i];
         const c2<double> bz = F[base + ((int64_t)mk * Py + j) * Px + i];
-        const double sx = ax[m] ? -1.0 : 1.0, sy = ay[m] ? -1.0 : 1.0, sz = az[m] ? -1.0 : 1.0;
+        const double sx = ox ? -1.0 : 1.0, sy = oy ? -1.0 : 1.0, sz = oz ? -1.0 : 1.0;
         md = fmax(md, fabs(a.x - sx * bx.x) + fabs(a.y - sx * bx.y));
         md = fmax(md, fabs(a.x - sy * by.x) + fabs(a.y - sy * by.y));
         md = fmax(md, fabs(a.x - sz * bz.x) + fabs(a.y - sz * bz.y));
@@ -248,9 +251,10 @@ __global__ void k_sym_check(const c2<double>* __restrict__ F, int Px, int Py, in
 }
 
 template <typename T>
-__global__ void k_oct_compress(const c2<double>* __restrict__ F, c2<T>* __restrict__ O, int Px, int Py, int Pz) {
+__global__ void k_oct_compress(const c2<double>* __restrict__ F, c2<T>* __restrict__ O, int Px, int Py, int Pz,
+                               int nc) {
     const int hx = Px / 2 + 1, hy = Py / 2 + 1, hz = Pz / 2 + 1;
-    const int64_t tot = (int64_t)hx * hy * hz * 6;
+    const int64_t tot = (int64_t)hx * hy * hz * nc;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(e % hx);
         const int j = (int)((e / hx) % hy);
@@ -261,9 +265,10 @@ __global__ void k_oct_compress(const c2<double>* __restrict__ F, c2<T>* __restri
     }
 }
 
-__global__ void k_oct_expand(const c2<double>* __restrict__ O, c2<double>* __restrict__ F, int Px, int Py, int Pz) {
+__global__ void k_oct_expand(const c2<double>* __restrict__ O, c2<double>* __restrict__ F, int Px, int Py, int Pz,
+                             int kind, int nc) {
     const int hx = Px / 2, hy = Py / 2, hz = Pz / 2;
-    const int64_t tot = (int64_t)Px * Py * Pz * 6;
+    const int64_t tot = (int64_t)Px * Py * Pz * nc;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(e % Px);
         const int j = (int)((e / Px) % Py);
@@ -272,28 +277,43 @@ __global__ void k_oct_expand(const c2<double>* __restrict__ O, c2<double>* __res
         const bool fx = i > hx, fy = j > hy, fz = k > hz;
         const int ie = fx ? Px - i : i, je = fy ? Py - j : j, ke = fz ? Pz - k : k;
         c2<double> v = O[(((int64_t)m * (hz + 1) + ke) * (hy + 1) + je) * (hx + 1) + ie];
-        bool neg = false;
-        if (m == 1) neg = fx != fy;
-        if (m == 2) neg = fx != fz;
-        if (m == 4) neg = fy != fz;
-        if (neg) v = c2<double>{-v.x, -v.y};
+        bool ox, oy, oz;
+        parity(kind, m, ox, oy, oz);
+        if ((ox && fx) != ((oy && fy) != (oz && fz))) v = c2<double>{-v.x, -v.y};
         F[e] = v;
     }
 }
 
-// Takes ownership of a device buffer holding the full complex128 fN and stores
-// it on the handle: octant form when every component has the circulant's
-// parity to 1e-14 of max|fN| (even sides only), full form otherwise.
-int store_fN(Op* h, void* full64) {
+namespace {
+struct CircSlot {
+    void** d64;
+    void** d32;
+    bool* oct;
+    bool* have;
+    int nc;
+};
+CircSlot slot(Op* h, int kind) {
+    if (kind == 0) return {&h->fN64, &h->fN32, &h->fN_oct, &h->have_fN, 6};
+    return {&h->fK64, &h->fK32, &h->fK_oct, &h->have_fK, 3};
+}
+}  // namespace
+
+// Takes ownership of a device buffer holding the full complex128 operator
+// (kind 0: fN, 6 components; kind 1: fK, 3 components) and stores it on the
+// handle: octant form when every component has the circulant's parity to
+// 1e-14 of its largest entry (even sides only), full form otherwise.
+int store_circ(Op* h, int kind, void* full64) {
+    const CircSlot c = slot(h, kind);
     const int Px = h->Px, Py = h->Py, Pz = h->Pz;
-    const size_t nf = (size_t)6 * Px * Py * Pz;
+    const size_t nf = (size_t)c.nc * Px * Py * Pz;
     hipStream_t s = h->stream;
     bool oct = false;
     if (!(Px & 1) && !(Py & 1) && !(Pz & 1)) {
         const int G = 1024;
         double* part = nullptr;
         MARIE_HIP(hipMalloc(&part, 2 * G * sizeof(double)));
-        hipLaunchKernelGGL(k_sym_check, dim3(G), dim3(256), 0, s, (const c2<double>*)full64, Px, Py, Pz, part);
+        hipLaunchKernelGGL(k_sym_check, dim3(G), dim3(256), 0, s, (const c2<double>*)full64, Px, Py, Pz, kind, c.nc,
+                           part);
         std::vector<double> hp(2 * G);
         MARIE_HIP(hipMemcpyAsync(hp.data(), part, 2 * G * sizeof(double), hipMemcpyDeviceToHost, s));
         MARIE_HIP(hipStreamSynchronize(s));
@@ -302,56 +322,60 @@ int store_fN(Op* h, void* full64) {
         for (int b = 0; b < G; ++b) { md = std::max(md, hp[2 * b]); ma = std::max(ma, hp[2 * b + 1]); }
         oct = ma > 0.0 && md <= 1e-14 * ma;
     }
-    if (h->fN64) { MARIE_HIP(hipFree(h->fN64)); h->fN64 = nullptr; }
-    if (h->fN32) { MARIE_HIP(hipFree(h->fN32)); h->fN32 = nullptr; }
-    h->fN_oct = oct;
+    if (*c.d64) { MARIE_HIP(hipFree(*c.d64)); *c.d64 = nullptr; }
+    if (*c.d32) { MARIE_HIP(hipFree(*c.d32)); *c.d32 = nullptr; }
+    *c.oct = oct;
     if (oct) {
-        const size_t no = (size_t)6 * (Px / 2 + 1) * (Py / 2 + 1) * (Pz / 2 + 1);
+        const size_t no = (size_t)c.nc * (Px / 2 + 1) * (Py / 2 + 1) * (Pz / 2 + 1);
         if (h->precisions & PREC_F64) {
-            MARIE_HIP(hipMalloc(&h->fN64, no * sizeof(c2<double>)));
+            MARIE_HIP(hipMalloc(c.d64, no * sizeof(c2<double>)));
             hipLaunchKernelGGL(k_oct_compress<double>, dim3(4096), dim3(256), 0, s, (const c2<double>*)full64,
-                               (c2<double>*)h->fN64, Px, Py, Pz);
+                               (c2<double>*)*c.d64, Px, Py, Pz, c.nc);
         }
         if (h->precisions & PREC_F32) {
-            MARIE_HIP(hipMalloc(&h->fN32, no * sizeof(c2<float>)));
+            MARIE_HIP(hipMalloc(c.d32, no * sizeof(c2<float>)));
             hipLaunchKernelGGL(k_oct_compress<float>, dim3(4096), dim3(256), 0, s, (const c2<double>*)full64,
-                               (c2<float>*)h->fN32, Px, Py, Pz);
+                               (c2<float>*)*c.d32, Px, Py, Pz, c.nc);
         }
         MARIE_HIP(hipGetLastError());
         MARIE_HIP(hipStreamSynchronize(s));
         MARIE_HIP(hipFree(full64));
     } else {
         if (h->precisions & PREC_F32) {
-            MARIE_HIP(hipMalloc(&h->fN32, nf * sizeof(c2<float>)));
+            MARIE_HIP(hipMalloc(c.d32, nf * sizeof(c2<float>)));
             hipLaunchKernelGGL(k_to_f32_full, dim3(4096), dim3(256), 0, s, (const c2<double>*)full64,
-                               (c2<float>*)h->fN32, (int64_t)nf);
+                               (c2<float>*)*c.d32, (int64_t)nf);
             MARIE_HIP(hipGetLastError());
         }
         if (h->precisions & PREC_F64) {
-            h->fN64 = full64;
+            *c.d64 = full64;
         } else {
             MARIE_HIP(hipStreamSynchronize(s));
             MARIE_HIP(hipFree(full64));
         }
         MARIE_HIP(hipStreamSynchronize(s));
     }
-    h->have_fN = true;
+    *c.have = true;
     return OK;
 }
 
-// full complex128 fN into a caller device buffer (expands the octant form)
-int load_fN(Op* h, void* out64) {
-    const size_t nf = (size_t)6 * h->Px * h->Py * h->Pz;
-    if (!h->fN64) { set_error("no FP64 fN on the handle"); return ERR_STATE; }
-    if (h->fN_oct) {
-        hipLaunchKernelGGL(k_oct_expand, dim3(4096), dim3(256), 0, h->stream, (const c2<double>*)h->fN64,
-                           (c2<double>*)out64, h->Px, h->Py, h->Pz);
+// full complex128 operator into a caller device buffer (expands the octant form)
+int load_circ(Op* h, int kind, void* out64) {
+    const CircSlot c = slot(h, kind);
+    const size_t nf = (size_t)c.nc * h->Px * h->Py * h->Pz;
+    if (!*c.d64) { set_error("no FP64 operator of this kind on the handle"); return ERR_STATE; }
+    if (*c.oct) {
+        hipLaunchKernelGGL(k_oct_expand, dim3(4096), dim3(256), 0, h->stream, (const c2<double>*)*c.d64,
+                           (c2<double>*)out64, h->Px, h->Py, h->Pz, kind, c.nc);
         MARIE_HIP(hipGetLastError());
     } else {
-        MARIE_HIP(hipMemcpyAsync(out64, h->fN64, nf * sizeof(c2<double>), hipMemcpyDeviceToDevice, h->stream));
+        MARIE_HIP(hipMemcpyAsync(out64, *c.d64, nf * sizeof(c2<double>), hipMemcpyDeviceToDevice, h->stream));
     }
     return OK;
 }
+
+int store_fN(Op* h, void* full64) { return store_circ(h, 0, full64); }
+int load_fN(Op* h, void* out64) { return load_circ(h, 0, out64); }
 
 // Inverse x pass with the crop, the 1/(Px Py Pz) scale and the material epilogue.
 template <typename T, int EMAX, int RSET>
@@ -527,7 +551,10 @@ static int effective_emax(int emax, const AxisPlan& pl, int dflt) {
 template <typename T>
 static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs, hipStream_t s) {
     const bool f64 = sizeof(T) == 8;
-    const c2<T>* fN = reinterpret_cast<const c2<T>*>(f64 ? h->fN64 : h->fN32);
+    const bool kop = (op == OP_K);
+    const c2<T>* fN = reinterpret_cast<const c2<T>*>(kop ? (f64 ? h->fK64 : h->fK32) : (f64 ? h->fN64 : h->fN32));
+    const bool oct = kop ? h->fK_oct : h->fN_oct;
+    const int prod = kop ? 1 : 0;
     const c2<T>* Mr = reinterpret_cast<const c2<T>*>(f64 ? h->Mr64 : h->Mr32);
     const c2<T>* Mc = reinterpret_cast<const c2<T>*>(f64 ? h->Mc64 : h->Mc32);
     const c2<T>* tw = reinterpret_cast<const c2<T>*>(f64 ? h->tw64 : h->tw32);
@@ -589,17 +616,23 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     if (tm) MARIE_HIP(hipEventRecord(h->ev[2], s));
     // z: fused forward, tensor product, inverse
     if (reg[2]) {
-        reg_z<T>(h->fN_oct, Pz, -em[2], s, B, fN, twz, Px, Py, N, nrhs);
+        reg_z<T>(oct, prod, Pz, -em[2], s, B, fN, twz, Px, Py, N, nrhs);
     } else {
         const int lg = pick_lg(Pz, e[2] * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg) * Py * nrhs);
         const size_t sm = ((size_t)Pz << lg) * sizeof(c2<T>);
-        if (h->fN_oct)
-            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, true>), g, dim3(NT),
-                       sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
+        if (oct && !kop)
+            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, true, 0>), g,
+                       dim3(NT), sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
+        else if (!kop)
+            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, false, 0>), g,
+                       dim3(NT), sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
+        else if (oct)
+            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, true, 1>), g,
+                       dim3(NT), sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
         else
-            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, false>), g, dim3(NT),
-                       sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
+            MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, false, 1>), g,
+                       dim3(NT), sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
     }
     MARIE_HIP(hipGetLastError());
     if (tm) MARIE_HIP(hipEventRecord(h->ev[3], s));
@@ -647,9 +680,14 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
 }
 
 int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s) {
-    if (!h->have_fN || !h->have_idx || (op == OP_AN && !h->have_mat)) {
-        set_error("operator, index or material not set");
+    const bool have_op = (op == OP_K) ? h->have_fK : h->have_fN;
+    if (!have_op || !h->have_idx || (op == OP_AN && !h->have_mat)) {
+        set_error(op == OP_K ? "K operator or index not set" : "operator, index or material not set");
         return ERR_STATE;
+    }
+    if (op == OP_K && transp) {
+        set_error("the K operator has no transposed product");
+        return ERR_ARG;
     }
     if (!(h->precisions & prec)) {
         set_error("requested precision was not enabled at create");

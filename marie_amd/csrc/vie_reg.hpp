@@ -14,8 +14,8 @@ void reg_strided(bool inv, int P, int E, hipStream_t s, const c2<T>* src, c2<T>*
                  int64_t s_ls, int64_t s_plane, int64_t d_ls, int64_t d_plane, int nin, int nout, int64_t s_comp,
                  int64_t d_comp, int nplanes, int ncomp);
 template <typename T>
-void reg_z(bool oct, int P, int E, hipStream_t s, c2<T>* B, const c2<T>* fN, const c2<T>* tw, int Px, int Py, int N,
-           int nrhs);
+void reg_z(bool oct, int prod, int P, int E, hipStream_t s, c2<T>* B, const c2<T>* fN, const c2<T>* tw, int Px,
+           int Py, int N, int nrhs);
 template <typename T>
 void reg_inv_x(int P, int E, hipStream_t s, const c2<T>* A, c2<T>* y, const c2<T>* x, const int* pos, const c2<T>* Mr,
                const c2<T>* Mc, const c2<T>* tw, int L, int M, int N, int64_t nidx, int op, int transp, T Gram,

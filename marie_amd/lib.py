@@ -16,6 +16,7 @@ PREC_F32 = 2
 OP_AN = 0
 OP_N = 1
 OP_L = 2
+OP_K = 3
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -35,6 +36,9 @@ SIGNATURES = [
     ('marie_vie_get_fN', _I, [_P, _P]),
     ('marie_vie_assemble_N', _I, [_P, _D, _D, _P]),
     ('marie_vie_singular_terms', _I, [_D, _D, _P]),
+    ('marie_vie_set_fK', _I, [_P, _P]),
+    ('marie_vie_get_fK', _I, [_P, _P]),
+    ('marie_vie_assemble_K', _I, [_P, _D, _D]),
     ('marie_vie_set_material', _I, [_P, _P, _P, _D]),
     ('marie_vie_set_index', _I, [_P, _P, _I64, _I]),
     ('marie_vie_apply', _I, [_P, _I, _I, _P, _P, _I]),
@@ -176,6 +180,23 @@ class Operator:
             check(load().marie_vie_assemble_N(self._h, float(dx), float(ko), _ptr(s)))
         else:
             check(load().marie_vie_assemble_N(self._h, float(dx), float(ko), None))
+
+    def set_fK(self, fK):
+        Px, Py, Pz = self.P
+        fK = np.asarray(fK)
+        if fK.shape != (Px, Py, Pz, 3):
+            raise ValueError('fK must be %s x 3' % (self.P,))
+        buf = _cplx(fK)
+        check(load().marie_vie_set_fK(self._h, _ptr(buf)))
+
+    def get_fK(self):
+        Px, Py, Pz = self.P
+        out = np.empty(Px * Py * Pz * 3, np.complex128)
+        check(load().marie_vie_get_fK(self._h, _ptr(out)))
+        return out.reshape(Px, Py, Pz, 3, order='F')
+
+    def assemble_K(self, dx, ko):
+        check(load().marie_vie_assemble_K(self._h, float(dx), float(ko)))
 
     # -- material / index -----------------------------------------------------
     def set_material(self, Mr, Mc, Gram):

@@ -11,6 +11,9 @@ Same names, argument meaning and return values as the reference functions
   JVIE_Solver(Einc, e_r, r, freq, fN, tol, form, solver, ...)   src_vie/src_solver/JVIE_Solver.m:1
   EMconstants(freq)                                             src_vie/src_solver/EMconstants.m
   E_field_Nop(J, fN, Gram, freq, Einc)                          src_vie/src_fields/src_op/E_field_Nop.m:1
+  mv_K(JIn0, fK, r, idx, GPU_flag)                              src_vie/src_mvp/mv_K.m:1
+  H_field_Kop(J, fK, Gram, Hinc)                                src_vie/src_fields/src_op/H_field_Kop.m:1
+  Scat_Solver(Einc, Hinc, freq, r, epsilon_r, sigma_e, rho, ...) src_vie/src_solver/Scat_Solver.m:1
   local_SAR / absorbed_power                                    src_vie/src_solver/Scat_Solver.m:179-228
 
 Every product runs in libmarie_vie.so on the GPU. `fN` may be the host array
@@ -91,6 +94,56 @@ def mv_L(JIn0, fN, r, Gram, idx, GPU_flag=1):
     return op.apply(np.asarray(JIn0).ravel(), lib.OP_L)
 
 
+_kcache = {}
+
+
+def _k_operator(fK, shape):
+    if isinstance(fK, lib.Operator):
+        return fK
+    key = (id(fK), shape)
+    ent = _kcache.get(key)
+    if ent is not None and ent[0] is fK:
+        return ent[1]
+    L, M, N = shape
+    op = lib.Operator(L, M, N, fK.shape[:3])
+    op.set_fK(fK)
+    _kcache.clear()
+    _kcache[key] = (fK, op)
+    return op
+
+
+def mv_K(JIn0, fK, r, idx, GPU_flag=1):
+    """JOut = K JIn0 on the DOFs idx (1-based) (mv_K.m:1)."""
+    _require_gpu(GPU_flag)
+    shape = np.shape(r)[:3]
+    op = _k_operator(fK, shape)
+    op.set_index(idx, one_based=True)
+    return op.apply(np.asarray(JIn0).ravel(), lib.OP_K)
+
+
+def H_field_Kop(J, fK, Gram, Hinc=None, GPU_flag=1):
+    """Total field Hinc + K J / Gram over every voxel (H_field_Kop.m:49-76)."""
+    _require_gpu(GPU_flag)
+    J = np.asarray(J, complex)
+    if Hinc is None:
+        Hinc = np.zeros_like(J)
+    if not np.any(J):
+        return np.asarray(Hinc, complex)
+    L, M, N, _ = J.shape
+    op = _k_operator(fK, (L, M, N))
+    op.set_index(np.arange(1, 3 * L * M * N + 1), one_based=True)
+    H = op.apply(J.ravel(order='F'), lib.OP_K).reshape(L, M, N, 3, order='F')
+    return Hinc + H / Gram
+
+
+def B1_plus(B):
+    """B1+ = (Bx + j By) / 2, the co-rotating field component in MARIE's
+    exp(+j omega t) convention (the reference defines no B1+; this is the
+    usual MR definition, stated here)."""
+    B = np.asarray(B)
+    return (B[..., 0] + 1j * B[..., 1]) / 2
+
+
 def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operator=False,
                  precisions=lib.PREC_F64, circulant_shape=None):
     """fN (2L x 2M x 2N x 6, complex128) assembled on the device.
@@ -106,7 +159,7 @@ def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operat
     return_operator=True the device-resident lib.Operator is returned instead
     of the host array.
     """
-    if op != 'N':
+    if op not in ('N', 'K'):
         raise NotImplementedError("operator '%s' is not on this package's path" % op)
     if type != 'FFTofC':
         raise NotImplementedError("type '%s': only 'FFTofC' is produced on the device" % type)
@@ -114,8 +167,11 @@ def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operat
     L, M, N = r.shape[:3]
     dx = r[1, 0, 0, 0] - r[0, 0, 0, 0]
     ko = 2 * np.pi / C0 * f                    # getOPERATORS.m:48
-    sv = None if isinstance(singular, str) else np.asarray(singular, np.complex128)
     o = lib.Operator(L, M, N, circulant_shape, precisions=precisions)
+    if op == 'K':                              # getOPERATORS.m:101-115
+        o.assemble_K(dx, ko)
+        return o if return_operator else o.get_fK()
+    sv = None if isinstance(singular, str) else np.asarray(singular, np.complex128)
     o.assemble_N(dx, ko, sv)
     if return_operator:
         return o
@@ -156,6 +212,37 @@ def local_SAR(J, sigma_e, rho):
 def absorbed_power(E, J, Gram):
     """Pabs = 0.5 Re sum(E . conj J) Gram (Scat_Solver.m:226-228)."""
     return 0.5 * np.real(np.sum(E * np.conj(J))) * Gram
+
+
+def Scat_Solver(Einc, Hinc, freq, r, epsilon_r, sigma_e, rho, tol=1e-3, fN=None, fK=None, form=2, solver='G',
+                precond=0, max_it=1000, inner_it=50, outer_it=200, ritz=5, gpu_flag=1, logfile=None,
+                precision='fp64'):
+    """[Jout, Sout, Eout, Bout, Gsar, Pabs, fN, fK] of Scat_Solver.m:1 on the device.
+
+    e_r = eps_r - j sigma/(eo omega) (:144); operators assembled when not given
+    (:157-162); currents by JVIE_Solver (:169); local SAR |J|^2/(2 sigma rho)
+    (:179-186); E = E_field_Nop (:204); B = mu0 H_field_Kop (:215-216);
+    Pabs = 0.5 Re sum(E . conj J) Gram and Gsar = sum(Sout) Gram (:226-229).
+    """
+    _require_gpu(gpu_flag)
+    em = EMconstants(freq)
+    e_r = np.asarray(epsilon_r) - 1j * np.asarray(sigma_e) / (em['eo'] * em['omega'])
+    r = np.asarray(r)
+    dx = r[1, 0, 0, 0] - r[0, 0, 0, 0]
+    Gram = dx ** 3
+    if fN is None:
+        fN = getOPERATORS(r, freq, 'N', return_operator=True,
+                          precisions=lib.PREC_F64 | (lib.PREC_F32 if precision == 'fp32-refined' else 0))
+    if fK is None:
+        fK = getOPERATORS(r, freq, 'K', return_operator=True)
+    Jout, *_ = JVIE_Solver(Einc, e_r, r, freq, fN, tol, form, solver, precond, max_it, inner_it, outer_it, ritz,
+                           gpu_flag, logfile, precision)
+    Sout = local_SAR(Jout, sigma_e, rho)
+    Eout = E_field_Nop(Jout, fN, Gram, freq, Einc)
+    Bout = em['mu'] * H_field_Kop(Jout, fK, Gram, Hinc)
+    Pabs = absorbed_power(Eout, Jout, Gram)
+    Gsar = float(np.sum(Sout) * Gram)
+    return Jout, Sout, Eout, Bout, Gsar, Pabs, fN, fK
 
 
 def PlaneWave_Excitation(r, k, omega_mu, Eo):
