@@ -207,6 +207,24 @@ int marie_vie_block_gmres(marie_vie_op* h, const double* B, double* X, int nrhs,
                           int maxit, int mode, double inner_tol, int options, int* flag, double* relres,
                           int* iter, double* resvec, int resvec_cap, int* nres);
 
+/* Incident E and H of current elements at observation points (eval_DGF.m:37-106
+ * as called by CCL_Excitation.m:41-70 and Dipole_Excitation.m): segment
+ * midpoints seg_center (nseg x 3, row per segment), current elements seg_J
+ * (nseg x 3 complex interleaved, current times segment vector), grouped into
+ * ports by port_start[0..nport] (segments port_start[p] .. port_start[p+1]-1
+ * drive port p); points column-major npoints x 3 (MATLAB's
+ * reshape(r, L*M*N, 3)); ko = omega/c0, omega_mu = omega*mu0. Outputs E, H:
+ * nport x 3 x npoints complex (per port the column-major npoints x 3 array of
+ * reshape(E, L, M, N, 3)). No counterpart in the C of the reference (eval_DGF
+ * is MATLAB); this is its device form. */
+int marie_vie_dgf_fields(const double* seg_center, const double* seg_J, const int* port_start, int nport,
+                         const double* points, int64_t npoints, double ko, double omega_mu, double* E,
+                         double* H, int device);
+/* Same on device buffers, on `stream` (synchronised before returning). */
+int marie_vie_dgf_fields_device(const double* seg_center_dev, const void* seg_J_dev, const int* port_start_dev,
+                                int nport, const double* points_dev, int64_t npoints, double ko, double omega_mu,
+                                void* E_dev, void* H_dev, void* stream);
+
 /* Tile autotuner: times every compiled variant of the five matvec kernels
  * with HIP events on the handle's stream, `iters` products each, and keeps
  * the fastest per kernel for precision `prec`. A variant is cfg6[0..4], one

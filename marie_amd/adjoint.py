@@ -20,6 +20,12 @@ import torch
 from . import lib, vie
 
 
+def _dense(t):
+    """Contiguous memory holding the tensor's values: torch.conj (and views of
+    imaginary parts) only set a flag that raw device pointers do not see."""
+    return t.resolve_conj().resolve_neg().contiguous()
+
+
 def to_voxels(a):
     """(L, M, N) array -> flat voxel vector (column-major order)."""
     return np.asarray(a).ravel(order='F')
@@ -37,7 +43,7 @@ class JVIEProblem:
     freq in Hz, dx the voxel size.
     """
 
-    def __init__(self, op, body, Einc, freq, dx, tol=1e-10, restart=50, maxit=200, device='cuda:0'):
+    def __init__(self, op, body, Einc, freq, dx, tol=1e-10, restart=50, maxit=200, device='cuda:0', Hinc=None):
         self.op = op
         self.shape = (op.L, op.M, op.N)
         body = np.asarray(body, bool)
@@ -58,6 +64,12 @@ class JVIEProblem:
         self.vox = torch.from_numpy(self.idx % n).to(device)
         self.tol, self.restart, self.maxit = tol, restart, maxit
         self.device = device
+        self.hinc = None
+        if Hinc is not None:
+            Hs = Hinc if isinstance(Hinc, (list, tuple)) else [Hinc]
+            self.hinc = torch.stack([torch.from_numpy(np.ascontiguousarray(np.asarray(h, np.complex128)
+                                                                            .ravel(order='F')[self.idx]))
+                                     for h in Hs]).to(device)
 
     @property
     def nrhs(self):
@@ -83,7 +95,7 @@ class _Solve(torch.autograd.Function):
     def forward(ctx, eps_r, sigma, prob):
         Mc = prob.material(eps_r, sigma)
         prob.load_material(Mc)
-        B = prob.rhs(Mc).contiguous()
+        B = _dense(prob.rhs(Mc))
         X = torch.zeros_like(B)
         torch.cuda.synchronize()
         for q in range(prob.nrhs):
@@ -105,7 +117,7 @@ class _Solve(torch.autograd.Function):
         gs = torch.zeros_like(ge)
         te = torch.empty_like(ge)
         ts = torch.empty_like(ge)
-        G = (gX / 2).contiguous()
+        G = _dense(gX / 2)
         for q in range(prob.nrhs):
             lam = torch.zeros_like(G[q])
             torch.cuda.synchronize()
@@ -113,12 +125,60 @@ class _Solve(torch.autograd.Function):
                                                                 prob.tol, prob.maxit)
             if flag != 0:
                 raise lib.MarieError('adjoint solve did not converge (relres %.3g)' % relres)
-            einc = prob.einc[q].contiguous()
-            prob.op.material_gradient_device(lam.data_ptr(), X[q].contiguous().data_ptr(), einc.data_ptr(),
+            einc = _dense(prob.einc[q])
+            prob.op.material_gradient_device(lam.data_ptr(), _dense(X[q]).data_ptr(), einc.data_ptr(),
                                              prob.omega_eo, te.data_ptr(), ts.data_ptr())
             ge += te
             gs += ts
         return ge, gs, None
+
+
+class _KApply(torch.autograd.Function):
+    """Y = K X on the DOFs (mv_K.m); K is symmetric as a matrix (each
+    component of its kernel is odd in its own axis and enters through an
+    antisymmetric cross product), so the backward product is
+    K^H g = conj(K conj(g))."""
+
+    @staticmethod
+    def forward(ctx, X, prob):
+        ctx.prob = prob
+        return _k_product(prob, X)
+
+    @staticmethod
+    def backward(ctx, g):
+        return torch.conj(_k_product(ctx.prob, torch.conj(g))), None
+
+
+def _k_product(prob, X):
+    # the product runs on the handle's stream: torch's work on X must be done
+    # before it starts, and Y must be complete before torch reads it
+    Xc = _dense(X)
+    Y = torch.empty_like(Xc)
+    torch.cuda.synchronize()
+    prob.op.apply_device(Xc.data_ptr(), Y.data_ptr(), lib.OP_K, False, lib.PREC_F64, Xc.shape[0])
+    torch.cuda.synchronize()
+    return Y
+
+
+def k_apply(X, prob):
+    """K X (nrhs x nidx) on the problem's DOFs; the handle must hold fK
+    (assemble_K or set_fK)."""
+    return _KApply.apply(X, prob)
+
+
+def total_B(X, prob):
+    """B = mu0 (Hinc + K J / Gram) on the DOFs (H_field_Kop.m:69-76,
+    Scat_Solver.m:215-216), one row per excitation; needs Hinc in the problem."""
+    if prob.hinc is None:
+        raise ValueError('JVIEProblem was built without Hinc')
+    return 4e-7 * np.pi * (prob.hinc + k_apply(X, prob) / prob.Gram)
+
+
+def b1_plus(B, prob):
+    """(Bx + j By)/2 per body voxel (the usual MR definition; the reference defines
+    none), from B on the DOFs: nrhs x n_body."""
+    nb = prob.idx.size // 3
+    return (B[:, :nb] + 1j * B[:, nb:2 * nb]) / 2
 
 
 def jvie_solve(eps_r, sigma, prob):

@@ -54,6 +54,8 @@ SIGNATURES = [
                                           ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_block_gmres', _I, [_P, _P, _P, _I, _I, _D, _I, _I, _D, _I, ctypes.POINTER(_I), _DP,
                                    ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_dgf_fields', _I, [_P, _P, _P, _I, _P, _I64, _D, _D, _P, _P, _I]),
+    ('marie_vie_dgf_fields_device', _I, [_P, _P, _P, _I, _P, _I64, _D, _D, _P, _P, _P]),
     ('marie_vie_autotune', _I, [_P, _I, _I, ctypes.POINTER(_I), _DP]),
     ('marie_vie_set_tiles', _I, [_P, _I, ctypes.POINTER(_I)]),
     ('marie_vie_enable_timing', _I, [_P, _I]),
@@ -106,6 +108,27 @@ def _cplx(a):
     """C-contiguous complex128 view in MATLAB column-major element order."""
     a = np.asarray(a, dtype=np.complex128)
     return np.ascontiguousarray(a.ravel(order='F'))
+
+
+def dgf_fields(seg_center, seg_J, port_start, points, ko, omega_mu, device=0):
+    """E, H (nport x npoints x 3 complex) of current elements (eval_DGF.m) on the GPU.
+
+    seg_center: nseg x 3 midpoints; seg_J: nseg x 3 complex current elements;
+    port_start: nport+1 segment offsets; points: npoints x 3.
+    """
+    lib = load()
+    sc = np.ascontiguousarray(np.asarray(seg_center, np.float64).reshape(-1, 3))
+    sj = np.ascontiguousarray(np.asarray(seg_J, np.complex128).reshape(-1, 3))
+    ps = np.ascontiguousarray(np.asarray(port_start, np.int32))
+    pts = np.asarray(points, np.float64).reshape(-1, 3)
+    npts = pts.shape[0]
+    cm = np.ascontiguousarray(pts.T)                 # column-major npoints x 3
+    nport = ps.size - 1
+    E = np.empty(nport * 3 * npts, np.complex128)
+    H = np.empty_like(E)
+    check(lib.marie_vie_dgf_fields(_ptr(sc), _ptr(sj), _ptr(ps), nport, _ptr(cm), npts, float(ko), float(omega_mu),
+                                   _ptr(E), _ptr(H), device))
+    return (E.reshape(nport, 3, npts).transpose(0, 2, 1), H.reshape(nport, 3, npts).transpose(0, 2, 1))
 
 
 BLOCK_GRAPH = 1
@@ -222,6 +245,8 @@ class Operator:
         return y.T if x.ndim == 2 else y
 
     def apply_device(self, x_ptr, y_ptr, op=OP_AN, transp=False, prec=PREC_F64, nrhs=1):
+        """Device product, enqueued on the handle's stream and not waited for:
+        synchronise (or share the stream, set_stream) before reading y."""
         check(load().marie_vie_apply_device(self._h, op, 1 if transp else 0, prec, ctypes.c_void_p(x_ptr),
                                             ctypes.c_void_p(y_ptr), nrhs))
 

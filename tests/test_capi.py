@@ -7,6 +7,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -53,3 +54,18 @@ def test_vie_mirror_rejects_cpu_flag():
     from marie_amd import vie
     with pytest.raises(ValueError):
         vie.mv_AN([1.0], None, [[[1.0]]], [[[0.0]]], 1.0, 'notransp', [1], 0)
+
+
+def test_device_pointers_carry_resolved_values():
+    """Raw pointers handed to the library hold the tensor's values: a lazily
+    conjugated view (torch.conj) shares the unconjugated memory even after
+    .contiguous(), so the autograd harness resolves it first."""
+    import torch
+    from marie_amd import adjoint
+    g = torch.tensor([1 + 2j, -3 + 0.5j], dtype=torch.complex128)
+    c = adjoint._dense(torch.conj(g))
+    assert not c.is_conj() and c.data_ptr() != g.data_ptr()
+    raw = torch.view_as_real(c).numpy()
+    assert np.array_equal(raw[:, 1], -g.imag.numpy())
+    n = adjoint._dense(torch.conj(g).imag)
+    assert np.array_equal(n.numpy(), -g.imag.numpy())
