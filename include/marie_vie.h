@@ -43,6 +43,9 @@ extern "C" {
 #define MARIE_OP_N 1  /* N x                  (mv_N.m)   */
 #define MARIE_OP_L 2  /* N x - Gram*x         (mv_L.m)   */
 #define MARIE_OP_K 3  /* K x                  (mv_K.m)   */
+/* the JVIE preconditioner: mv_AN with the material (1./Mr, 1./Mr - 1)
+ * (JVIE_Solver.m:213, :232, :278, :297); with Mr = 1 (form II) it is Gram*x */
+#define MARIE_OP_P 4
 
 typedef struct marie_vie_op marie_vie_op;
 
@@ -149,6 +152,43 @@ int marie_vie_gmres_device(marie_vie_op* h, const void* b_dev, void* x_dev, int 
 int marie_vie_gmres(marie_vie_op* h, const double* b, double* x, int restart, double tol, int maxit,
                     int mode, double inner_tol, int* flag, double* relres, int* iter, double* resvec,
                     int resvec_cap, int* nres);
+
+/* Preconditioned GMRES(restart), FP64, as JVIE_Solver.m:327, :347 call
+ * pgmres with the preconditioner handle fPGPU (mv_AN with 1./Mr, 1./Mr - 1;
+ * MARIE_OP_P): side MARIE_PRECOND_RIGHT solves A P z = b and returns
+ * x = P z (pgmres R1, pgmres_iter.m:64-76, :131-145); MARIE_PRECOND_LEFT
+ * solves P A x = P b with the stopping test and resvec against ||P b||
+ * (pgmres L1, pgmres.m:122-142, pgmres_iter.m:80-96). flag and relres from
+ * the true residual ||b - A x||/||b|| (pgmres.m:175-180). Other arguments as
+ * marie_vie_gmres_device with mode 0. */
+#define MARIE_PRECOND_NONE 0
+#define MARIE_PRECOND_RIGHT 1
+#define MARIE_PRECOND_LEFT 2
+int marie_vie_gmres_precond_device(marie_vie_op* h, const void* b_dev, void* x_dev, int restart, double tol,
+                                   int maxit, int side, int* flag, double* relres, int* iter, double* resvec,
+                                   int resvec_cap, int* nres);
+
+/* Same with host buffers b, x (complex128, nidx each). */
+int marie_vie_gmres_precond(marie_vie_op* h, const double* b, double* x, int restart, double tol, int maxit,
+                            int side, int* flag, double* relres, int* iter, double* resvec, int resvec_cap,
+                            int* nres);
+
+/* BiCGStab, FP64, the solver JVIE_Solver.m:463-490 calls (MATLAB's
+ * bicgstab(afun, Vrhs, tol, max_it[, pfun]) with pfun = MARIE_OP_P when
+ * `precond` = 1, applied to p and s as a function-handle M). Van der Vorst's
+ * recurrences with shadow residual r0; resvec: absolute residual norm of
+ * every half step, starting with ||b - A x0||; stops when a half- or
+ * full-step norm <= tol*||b||; *iter = half steps / 2 (k - 0.5 when it stops
+ * halfway); flag 0 converged, 1 maxit reached, 4 breakdown (a zero <r~,r>,
+ * <r~,v>, <t,t> or omega), by the true residual. x_dev holds x0 on entry.
+ * resvec needs up to 2*maxit + 1 entries (count in *nres). */
+int marie_vie_bicgstab_device(marie_vie_op* h, const void* b_dev, void* x_dev, double tol, int maxit,
+                              int precond, int* flag, double* relres, double* iter, double* resvec,
+                              int resvec_cap, int* nres);
+
+/* Same with host buffers b, x (complex128, nidx each). */
+int marie_vie_bicgstab(marie_vie_op* h, const double* b, double* x, double tol, int maxit, int precond,
+                       int* flag, double* relres, double* iter, double* resvec, int resvec_cap, int* nres);
 
 /* GMRES on the adjoint system A^H y = g (A^H applied through mv_AN's
  * 'transp' branch, mv_AN.m:65-98, the product MATLAB's solvers request as

@@ -119,7 +119,7 @@ int marie_vie_destroy(marie_vie_op* h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->fN64, h->fN32, h->fK64, h->fK32, h->tw64, h->tw32, h->Mr64, h->Mc64, h->Mr32, h->Mc32,
-                    h->pos, h->idx, h->A, h->B};
+                    h->Pr64, h->Pc64, h->Pr32, h->Pc32, h->pos, h->idx, h->A, h->B};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < 8; ++k)
@@ -254,6 +254,9 @@ int marie_vie_set_material(marie_vie_op* h, const double* Mr, const double* Mc, 
     if (st) return st;
     h->Gram = Gram;
     h->have_mat = true;
+    h->have_P = false;
+    h->mr_unit = true;
+    for (size_t i = 0; i < n && h->mr_unit; ++i) h->mr_unit = Mr[2 * i] == 1.0 && Mr[2 * i + 1] == 0.0;
     return MARIE_OK;
 }
 
@@ -286,12 +289,13 @@ int marie_vie_set_index(marie_vie_op* h, const int64_t* idx, int64_t nidx, int o
 
 int marie_vie_apply_device(marie_vie_op* h, int op, int transp, int prec, const void* x, void* y, int nrhs) {
     CHECK_H(h);
-    if (op < MARIE_OP_AN || op > MARIE_OP_K || (prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || nrhs < 1 ||
+    if (op < MARIE_OP_AN || op > MARIE_OP_P || (prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || nrhs < 1 ||
         (h->nidx > 0 && (!x || !y))) {
         set_error("invalid apply arguments");
         return MARIE_ERR_ARG;
     }
-    if (!(op == MARIE_OP_K ? h->have_fK : h->have_fN) || !h->have_idx || (op == MARIE_OP_AN && !h->have_mat)) {
+    if (!(op == MARIE_OP_K ? h->have_fK : h->have_fN) || !h->have_idx ||
+        ((op == MARIE_OP_AN || op == MARIE_OP_P) && !h->have_mat)) {
         set_error("operator, index or material not set");
         return MARIE_ERR_STATE;
     }
@@ -301,8 +305,9 @@ int marie_vie_apply_device(marie_vie_op* h, int op, int transp, int prec, const 
 
 int marie_vie_apply(marie_vie_op* h, int op, int transp, const double* x, double* y, int nrhs) {
     CHECK_H(h);
-    if (nrhs < 1 || op < MARIE_OP_AN || op > MARIE_OP_K) { set_error("invalid apply arguments"); return MARIE_ERR_ARG; }
-    if (!(op == MARIE_OP_K ? h->have_fK : h->have_fN) || !h->have_idx || (op == MARIE_OP_AN && !h->have_mat)) {
+    if (nrhs < 1 || op < MARIE_OP_AN || op > MARIE_OP_P) { set_error("invalid apply arguments"); return MARIE_ERR_ARG; }
+    if (!(op == MARIE_OP_K ? h->have_fK : h->have_fN) || !h->have_idx ||
+        ((op == MARIE_OP_AN || op == MARIE_OP_P) && !h->have_mat)) {
         set_error("operator, index or material not set");
         return MARIE_ERR_STATE;
     }

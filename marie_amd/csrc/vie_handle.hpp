@@ -19,7 +19,8 @@ enum Status {
 };
 
 enum Precision { PREC_F64 = 1, PREC_F32 = 2 };
-enum OpKind { OP_AN = 0, OP_N = 1, OP_L = 2, OP_K = 3 };
+// OP_P: the JVIE preconditioner, mv_AN with the material (1/Mr, 1/Mr - 1)
+enum OpKind { OP_AN = 0, OP_N = 1, OP_L = 2, OP_K = 3, OP_P = 4 };
 
 void set_error(const std::string& msg);
 const char* get_error();
@@ -61,6 +62,13 @@ struct Op {
     void* Mc32 = nullptr;
     double Gram = 0.0;
     bool have_fN = false, have_mat = false, have_idx = false;
+    // preconditioner material 1/Mr, 1/Mr - 1 (derived from Mr on first use);
+    // mr_unit: every Mr is exactly 1 (form II), so the preconditioner is Gram*x
+    void* Pr64 = nullptr;
+    void* Pc64 = nullptr;
+    void* Pr32 = nullptr;
+    void* Pc32 = nullptr;
+    bool have_P = false, mr_unit = false;
 
     int* pos = nullptr;             // [3 L M N] -> compact position or -1
     int64_t* idx = nullptr;         // [nidx] 0-based grid index of each compact entry

@@ -21,6 +21,7 @@
 // the cycle's update never reads).
 #include "../../include/marie_vie.h"
 #include "vie_handle.hpp"
+#include "vie_reduce.hpp"
 
 #include <cmath>
 #include <complex>
@@ -33,32 +34,6 @@ constexpr int KT = 256;      // threads per block
 constexpr int KG = 1024;     // max blocks of a reduction
 constexpr int KV = 8;        // basis vectors per pass of the multi-dot
 
-
-template <int W>
-__device__ __forceinline__ void block_reduce(double (&v)[W], double* sh) {
-    // sh: KT/64 * W doubles; wave64 shuffle then across waves
-#pragma unroll
-    for (int q = 0; q < W; ++q) {
-        double s = v[q];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        v[q] = s;
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
-#pragma unroll
-        for (int q = 0; q < W; ++q) sh[wid * W + q] = v[q];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < W; ++q) {
-            double s = 0.0;
-            for (int w = 0; w < KT / 64; ++w) s += sh[w * W + q];
-            v[q] = s;
-        }
-    }
-    __syncthreads();
-}
 
 // Vectors of the Krylov space are stored as c2<TV> (TV = double, or float for
 // the complex64-basis mode); every product and sum accumulates in double.
@@ -221,6 +196,7 @@ struct Ws {
     c2<double>* t = nullptr;     // n, FP64 scratch / correction
     c2<float>* xs = nullptr;     // n (complex64 staging for FP64-basis mixed mode)
     c2<float>* ys = nullptr;
+    c2<double>* pt = nullptr;    // n, FP64 preconditioner intermediate
     c2<double>* part = nullptr;  // (m+1) x KG
     c2<double>* h = nullptr;     // (m+1) x (m+1): Hessenberg columns, then update coefficients
     double* npart = nullptr;     // KG
@@ -228,7 +204,7 @@ struct Ws {
     c2<double>* hp = nullptr;    // pinned host mirror of h
     hipEvent_t ev[2] = {nullptr, nullptr};
     void free_all() {
-        void* p[] = {V, w, rT, r, t, xs, ys, part, h, npart, nout};
+        void* p[] = {V, w, rT, r, t, xs, ys, pt, part, h, npart, nout};
         for (void* q : p)
             if (q) (void)hipFree(q);
         if (hp) (void)hipHostFree(hp);
@@ -243,6 +219,7 @@ struct Ctx {
     int64_t n;
     int G;
     int transp = 0;              // 1: solve with A^H (mv_AN's 'transp' branch)
+    int precond = 0;             // 0 none, 1 right (A P), 2 left (P A): pgmres R1 / L1
     Ws ws;
 };
 
@@ -281,6 +258,22 @@ template <typename TV> static int matvec(Ctx& c, const c2<TV>* x, c2<TV>* y, boo
     }
 }
 
+// The cycle's operator: A, or with the preconditioner P = mv_AN(., 1./Mr,
+// 1./Mr - 1) applied as pgmres_iter.m:64-96 does, A P (right) or P A (left)
+template <typename TV> static int cycle_matvec(Ctx& c, const c2<TV>* x, c2<TV>* y, bool f32op) {
+    if constexpr (sizeof(TV) == 8) {
+        if (c.precond == 1) {
+            const int st = apply(c.h, OP_P, 0, PREC_F64, x, c.ws.pt, 1, c.s);
+            return st ? st : apply(c.h, OP_AN, 0, PREC_F64, c.ws.pt, y, 1, c.s);
+        }
+        if (c.precond == 2) {
+            const int st = apply(c.h, OP_AN, 0, PREC_F64, x, c.ws.pt, 1, c.s);
+            return st ? st : apply(c.h, OP_P, 0, PREC_F64, c.ws.pt, y, 1, c.s);
+        }
+    }
+    return matvec<TV>(c, x, y, f32op);
+}
+
 // One restart cycle of pgmres_iter on A z = r (r: current residual, stored as TV).
 // Accumulates the update into the FP64 vector x and leaves the implicit
 // residual V (rhs - H y) in r. Appends |LS residual| values to resv.
@@ -311,7 +304,7 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<TV>* r, int m, double abstol, b
     // Arnoldi step k (1-based): w = A V(:,k), Hessenberg column k-1 on the
     // device, V(:,k+1) = w / H(k+1,k), column copied to pinned memory + event
     auto enqueue = [&](int k) -> int {
-        int st2 = matvec<TV>(c, V + (int64_t)(k - 1) * n, w, f32op);
+        int st2 = cycle_matvec<TV>(c, V + (int64_t)(k - 1) * n, w, f32op);
         if (st2) return st2;
         c2<double>* hc = c.ws.h + (size_t)(k - 1) * ld;
         // h = V(:,1:k)^H w  (classical Gram-Schmidt, pgmres_iter.m:106)
@@ -406,8 +399,18 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<TV>* r, int m, double abstol, b
     c2<double>* dr = c.ws.h + (size_t)m * ld + ld;
     MARIE_HIP(hipMemcpyAsync(dy, cy, k * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
     MARIE_HIP(hipMemcpyAsync(dr, cr, (k + 1) * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
-    hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, V, n, k, dy, 1.0, n,
-                       (double*)nullptr, c.G);
+    if (c.precond == 1) {
+        // right preconditioning: x += P (V y)  (pgmres_iter.m:131-145)
+        hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, c.ws.t, 0.0, V, n, k, dy, 1.0, n,
+                           (double*)nullptr, c.G);
+        int st2 = apply(c.h, OP_P, 0, PREC_F64, c.ws.t, c.ws.pt, 1, c.s);
+        if (st2) return st2;
+        hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, x, 1.0, c.ws.pt,
+                           n);
+    } else {
+        hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, V, n, k, dy, 1.0, n,
+                           (double*)nullptr, c.G);
+    }
     hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, r, 0.0, V, n, k + 1, dr, 1.0, n,
                        (double*)nullptr, c.G);
     MARIE_HIP(hipGetLastError());
@@ -435,6 +438,7 @@ static int alloc_ws(Ctx& c, int m, int mode) {
     if (!st) st = A((void**)&c.ws.t, n * sizeof(c2<double>));
     if (!st && mode == 1) st = A((void**)&c.ws.xs, n * sizeof(c2<float>));
     if (!st && mode == 1) st = A((void**)&c.ws.ys, n * sizeof(c2<float>));
+    if (!st && c.precond) st = A((void**)&c.ws.pt, n * sizeof(c2<double>));
     if (!st) st = A((void**)&c.ws.part, (size_t)(m + 1) * KG * sizeof(c2<double>));
     // Hessenberg columns (m x (m+1)) + the two coefficient vectors of the update
     const size_t nh = (size_t)(m + 2) * (m + 1);
@@ -500,10 +504,11 @@ static int refine(Ctx& c, const c2<double>* b, c2<double>* x, int m, double tol,
 }
 
 int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxit, int mode, double inner_tol,
-          int* flag, double* relres, int* iter, std::vector<double>& resvec, int transp) {
+          int* flag, double* relres, int* iter, std::vector<double>& resvec, int transp, int precond) {
     Ctx c;
     c.h = h;
     c.transp = transp;
+    c.precond = precond;
     c.s = h->stream;
     c.n = h->nidx;
     c.G = grid_for(c.n);
@@ -532,19 +537,35 @@ int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxi
     } else {
         MARIE_HIP(hipMemcpyAsync(c.ws.r, b, c.n * sizeof(c2<double>), hipMemcpyDeviceToDevice, c.s));
     }
+    // left preconditioning works on P r and measures against ||P b||
+    // (pgmres.m:122-142); flag and relres stay on the true residual
+    double bnorm_sys = bnorm;
+    if (c.precond == 2) {
+        st = apply(c.h, OP_P, 0, PREC_F64, c.ws.r, c.ws.pt, 1, c.s);
+        if (st) return done(st);
+        MARIE_HIP(hipMemcpyAsync(c.ws.r, c.ws.pt, c.n * sizeof(c2<double>), hipMemcpyDeviceToDevice, c.s));
+        st = apply(c.h, OP_P, 0, PREC_F64, b, c.ws.pt, 1, c.s);
+        if (st) return done(st);
+        st = norm<double>(c, c.ws.pt, &bnorm_sys);
+        if (st) return done(st);
+        if (bnorm_sys == 0.0) {
+            set_error("the preconditioner maps the right-hand side to zero");
+            return done(ERR_ARG);
+        }
+    }
     double rn = 0.0;
     st = norm<double>(c, c.ws.r, &rn);
     if (st) return done(st);
-    resvec.assign(1, rn / bnorm);
+    resvec.assign(1, rn / bnorm_sys);
     int outit = 0, p = 0;
     if (mode == 0) {
         // pgmres.m:153-167
         while (resvec.back() > tol && outit < maxit) {
             ++outit;
             std::vector<double> rv;
-            st = gmres_cycle<double>(c, x, c.ws.r, m, tol * bnorm, false, &p, rv);
+            st = gmres_cycle<double>(c, x, c.ws.r, m, tol * bnorm_sys, false, &p, rv);
             if (st) return done(st);
-            for (double v : rv) resvec.push_back(v / bnorm);
+            for (double v : rv) resvec.push_back(v / bnorm_sys);
             if (rv.empty()) break;
         }
     } else if (mode == 1) {
@@ -575,7 +596,7 @@ using namespace marie;
 
 static int gmres_entry(marie_vie_op* hh, const void* b, void* x, int restart, double tol, int maxit, int mode,
                        double inner_tol, int* flag, double* relres, int* iter, double* resvec, int resvec_cap, int* nres,
-                       int transp) {
+                       int transp, int precond = 0) {
     Op* h = reinterpret_cast<Op*>(hh);
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
     MARIE_HIP(hipSetDevice(h->device));
@@ -597,9 +618,13 @@ static int gmres_entry(marie_vie_op* hh, const void* b, void* x, int restart, do
         if (nres) *nres = 0;
         return MARIE_OK;
     }
+    if (precond != 0 && (mode != 0 || transp)) {
+        set_error("preconditioned GMRES runs in mode 0 on the forward system");
+        return MARIE_ERR_ARG;
+    }
     std::vector<double> rv;
     int st = gmres(h, (const c2<double>*)b, (c2<double>*)x, restart, tol, maxit, mode, inner_tol, flag, relres,
-                   iter, rv, transp);
+                   iter, rv, transp, precond);
     if (st) return st;
     if (resvec && resvec_cap > 0) {
         const int nc = (int)rv.size() < resvec_cap ? (int)rv.size() : resvec_cap;
@@ -623,9 +648,20 @@ extern "C" int marie_vie_gmres_adjoint_device(marie_vie_op* hh, const void* g, v
                        1);
 }
 
-extern "C" int marie_vie_gmres(marie_vie_op* hh, const double* b, double* x, int restart, double tol, int maxit,
-                               int mode, double inner_tol, int* flag, double* relres, int* iter, double* resvec,
-                               int resvec_cap, int* nres) {
+extern "C" int marie_vie_gmres_precond_device(marie_vie_op* hh, const void* b, void* x, int restart, double tol,
+                                              int maxit, int side, int* flag, double* relres, int* iter,
+                                              double* resvec, int resvec_cap, int* nres) {
+    if (side != MARIE_PRECOND_NONE && side != MARIE_PRECOND_RIGHT && side != MARIE_PRECOND_LEFT) {
+        set_error("side must be MARIE_PRECOND_NONE, _RIGHT or _LEFT");
+        return MARIE_ERR_ARG;
+    }
+    return gmres_entry(hh, b, x, restart, tol, maxit, 0, 0.0, flag, relres, iter, resvec, resvec_cap, nres, 0,
+                       side);
+}
+
+// Host-buffer form of a device solve: upload b and x0, run, download x.
+template <typename F>
+static int host_solve(marie_vie_op* hh, const double* b, double* x, F&& solve) {
     Op* h = reinterpret_cast<Op*>(hh);
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
     MARIE_HIP(hipSetDevice(h->device));
@@ -647,8 +683,7 @@ extern "C" int marie_vie_gmres(marie_vie_op* hh, const double* b, double* x, int
         MARIE_HIP(hipMemcpy(db, b, n * sizeof(c2<double>), hipMemcpyHostToDevice));
         MARIE_HIP(hipMemcpy(dxp, x, n * sizeof(c2<double>), hipMemcpyHostToDevice));
     }
-    int st = marie_vie_gmres_device(hh, db, dxp, restart, tol, maxit, mode, inner_tol, flag, relres, iter, resvec,
-                                    resvec_cap, nres);
+    int st = solve(db, dxp);
     if (st == MARIE_OK && n) {
         if (hipMemcpy(x, dxp, n * sizeof(c2<double>), hipMemcpyDeviceToHost) != hipSuccess) {
             set_error("D2H copy failed");
@@ -658,4 +693,22 @@ extern "C" int marie_vie_gmres(marie_vie_op* hh, const double* b, double* x, int
     if (db) (void)hipFree(db);
     if (dxp) (void)hipFree(dxp);
     return st;
+}
+
+extern "C" int marie_vie_gmres(marie_vie_op* hh, const double* b, double* x, int restart, double tol, int maxit,
+                               int mode, double inner_tol, int* flag, double* relres, int* iter, double* resvec,
+                               int resvec_cap, int* nres) {
+    return host_solve(hh, b, x, [&](void* db, void* dx) {
+        return marie_vie_gmres_device(hh, db, dx, restart, tol, maxit, mode, inner_tol, flag, relres, iter, resvec,
+                                      resvec_cap, nres);
+    });
+}
+
+extern "C" int marie_vie_gmres_precond(marie_vie_op* hh, const double* b, double* x, int restart, double tol,
+                                       int maxit, int side, int* flag, double* relres, int* iter, double* resvec,
+                                       int resvec_cap, int* nres) {
+    return host_solve(hh, b, x, [&](void* db, void* dx) {
+        return marie_vie_gmres_precond_device(hh, db, dx, restart, tol, maxit, side, flag, relres, iter, resvec,
+                                              resvec_cap, nres);
+    });
 }

@@ -548,6 +548,62 @@ static int effective_emax(int emax, const AxisPlan& pl, int dflt) {
     return plan_rset(pl) == RSET_ALL ? dflt : emax;
 }
 
+// Preconditioner material of JVIE_Solver.m:278, :297 (mv_AN called with
+// 1./Mr and 1./Mr - 1), derived from Mr in FP64, rounded once for complex64.
+__global__ void k_precond_material(const c2<double>* __restrict__ Mr, c2<double>* __restrict__ Pr,
+                                   c2<double>* __restrict__ Pc, c2<float>* __restrict__ Pr32,
+                                   c2<float>* __restrict__ Pc32, int64_t n) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        const c2<double> m = Mr[v];
+        const double d = m.x * m.x + m.y * m.y;
+        const c2<double> r = {m.x / d, -m.y / d};
+        const c2<double> c = {r.x - 1.0, r.y};
+        Pr[v] = r;
+        Pc[v] = c;
+        if (Pr32) {
+            Pr32[v] = c2<float>{(float)r.x, (float)r.y};
+            Pc32[v] = c2<float>{(float)c.x, (float)c.y};
+        }
+    }
+}
+
+// y = Gram * x: the preconditioner when every Mr is 1 (then 1./Mr - 1 = 0 and
+// mv_AN reduces to Gram*1*x - 0*(N x), the same value without the FFTs)
+template <typename T>
+__global__ void k_gram_scale(const c2<T>* __restrict__ x, c2<T>* __restrict__ y, T gram, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        y[e] = cscale(x[e], gram);
+}
+
+static int ensure_precond(Op* h, hipStream_t s) {
+    if (h->have_P) return OK;
+    const size_t n = (size_t)h->nvox();
+    auto A = [&](void** p, size_t bytes) -> int {
+        if (*p) return OK;
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            set_error("out of device memory for the preconditioner material");
+            return ERR_ALLOC;
+        }
+        return OK;
+    };
+    int st = A(&h->Pr64, n * sizeof(c2<double>));
+    if (!st) st = A(&h->Pc64, n * sizeof(c2<double>));
+    const bool f32 = h->precisions & PREC_F32;
+    if (!st && f32) st = A(&h->Pr32, n * sizeof(c2<float>));
+    if (!st && f32) st = A(&h->Pc32, n * sizeof(c2<float>));
+    if (st) return st;
+    int64_t g = ((int64_t)n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_precond_material, dim3((unsigned)g), dim3(256), 0, s, (const c2<double>*)h->Mr64,
+                       (c2<double>*)h->Pr64, (c2<double>*)h->Pc64, f32 ? (c2<float>*)h->Pr32 : nullptr,
+                       f32 ? (c2<float>*)h->Pc32 : nullptr, (int64_t)n);
+    MARIE_HIP(hipGetLastError());
+    h->have_P = true;
+    return OK;
+}
+
 template <typename T>
 static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs, hipStream_t s) {
     const bool f64 = sizeof(T) == 8;
@@ -555,8 +611,10 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     const c2<T>* fN = reinterpret_cast<const c2<T>*>(kop ? (f64 ? h->fK64 : h->fK32) : (f64 ? h->fN64 : h->fN32));
     const bool oct = kop ? h->fK_oct : h->fN_oct;
     const int prod = kop ? 1 : 0;
-    const c2<T>* Mr = reinterpret_cast<const c2<T>*>(f64 ? h->Mr64 : h->Mr32);
-    const c2<T>* Mc = reinterpret_cast<const c2<T>*>(f64 ? h->Mc64 : h->Mc32);
+    const bool pop = (op == OP_P);
+    const c2<T>* Mr = reinterpret_cast<const c2<T>*>(pop ? (f64 ? h->Pr64 : h->Pr32) : (f64 ? h->Mr64 : h->Mr32));
+    const c2<T>* Mc = reinterpret_cast<const c2<T>*>(pop ? (f64 ? h->Pc64 : h->Pc32) : (f64 ? h->Mc64 : h->Mc32));
+    if (pop) op = OP_AN;       // the same product with the preconditioner's material
     const c2<T>* tw = reinterpret_cast<const c2<T>*>(f64 ? h->tw64 : h->tw32);
     const c2<T>* twx = tw;
     const c2<T>* twy = tw + h->Px;
@@ -681,7 +739,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
 
 int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s) {
     const bool have_op = (op == OP_K) ? h->have_fK : h->have_fN;
-    if (!have_op || !h->have_idx || (op == OP_AN && !h->have_mat)) {
+    if (!have_op || !h->have_idx || ((op == OP_AN || op == OP_P) && !h->have_mat)) {
         set_error(op == OP_K ? "K operator or index not set" : "operator, index or material not set");
         return ERR_STATE;
     }
@@ -698,6 +756,24 @@ int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs,
     if ((int64_t)h->Px * h->Py * nrhs * NT >= (int64_t)1 << 32) {
         set_error("too many right-hand sides for one product on this grid");
         return ERR_ARG;
+    }
+    if (op == OP_P) {
+        if (h->mr_unit) {
+            const int64_t n = h->nidx * nrhs;
+            int64_t g = (n + 255) / 256;
+            if (g > 4096) g = 4096;
+            if (g < 1) g = 1;
+            if (prec == PREC_F64)
+                hipLaunchKernelGGL(k_gram_scale<double>, dim3((unsigned)g), dim3(256), 0, s, (const c2<double>*)x,
+                                   (c2<double>*)y, h->Gram, n);
+            else
+                hipLaunchKernelGGL(k_gram_scale<float>, dim3((unsigned)g), dim3(256), 0, s, (const c2<float>*)x,
+                                   (c2<float>*)y, (float)h->Gram, n);
+            MARIE_HIP(hipGetLastError());
+            return OK;
+        }
+        const int stp = ensure_precond(h, s);
+        if (stp) return stp;
     }
     int st = ensure_work(h, nrhs);
     if (st) return st;

@@ -17,6 +17,10 @@ OP_AN = 0
 OP_N = 1
 OP_L = 2
 OP_K = 3
+OP_P = 4            # the JVIE preconditioner: mv_AN with (1./Mr, 1./Mr - 1)
+PRECOND_NONE = 0
+PRECOND_RIGHT = 1
+PRECOND_LEFT = 2
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -49,6 +53,14 @@ SIGNATURES = [
                              ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_gmres_adjoint_device', _I, [_P, _P, _P, _I, _D, _I, _I, _D, ctypes.POINTER(_I), _DP,
                                             ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_gmres_precond_device', _I, [_P, _P, _P, _I, _D, _I, _I, ctypes.POINTER(_I), _DP,
+                                            ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_gmres_precond', _I, [_P, _P, _P, _I, _D, _I, _I, ctypes.POINTER(_I), _DP,
+                                     ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_bicgstab_device', _I, [_P, _P, _P, _D, _I, _I, ctypes.POINTER(_I), _DP, _DP, _DP, _I,
+                                       ctypes.POINTER(_I)]),
+    ('marie_vie_bicgstab', _I, [_P, _P, _P, _D, _I, _I, ctypes.POINTER(_I), _DP, _DP, _DP, _I,
+                                ctypes.POINTER(_I)]),
     ('marie_vie_material_gradient_device', _I, [_P, _P, _P, _P, _D, _P, _P]),
     ('marie_vie_block_gmres_device', _I, [_P, _P, _P, _I, _I, _D, _I, _I, _D, _I, ctypes.POINTER(_I), _DP,
                                           ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
@@ -334,6 +346,64 @@ class Operator:
                                      ctypes.byref(flag), ctypes.byref(relres), it, resvec.ctypes.data_as(_DP),
                                      resvec_cap, ctypes.byref(nres)))
         return x, flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    def gmres_precond(self, b, x0=None, restart=50, tol=1e-6, maxit=200, side=PRECOND_RIGHT, resvec_cap=20000):
+        """Host-buffer preconditioned GMRES (pgmres with R1 or L1 = the JVIE
+        preconditioner); returns (x, flag, relres, [inner, outer], resvec)."""
+        b = np.ascontiguousarray(np.asarray(b, np.complex128))
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(np.asarray(x0, np.complex128)).copy()
+        flag = ctypes.c_int()
+        relres = ctypes.c_double()
+        it = (ctypes.c_int * 2)()
+        resvec = np.zeros(resvec_cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_gmres_precond(self._h, _ptr(b), _ptr(x), restart, tol, maxit, side,
+                                             ctypes.byref(flag), ctypes.byref(relres), it,
+                                             resvec.ctypes.data_as(_DP), resvec_cap, ctypes.byref(nres)))
+        return x, flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    def gmres_precond_device(self, b_ptr, x_ptr, restart=50, tol=1e-6, maxit=200, side=PRECOND_RIGHT,
+                             resvec_cap=20000):
+        flag = ctypes.c_int()
+        relres = ctypes.c_double()
+        it = (ctypes.c_int * 2)()
+        resvec = np.zeros(resvec_cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_gmres_precond_device(self._h, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr),
+                                                    restart, tol, maxit, side, ctypes.byref(flag),
+                                                    ctypes.byref(relres), it, resvec.ctypes.data_as(_DP),
+                                                    resvec_cap, ctypes.byref(nres)))
+        return flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    def bicgstab(self, b, x0=None, tol=1e-6, maxit=1000, precond=False):
+        """Host-buffer BiCGStab (MATLAB bicgstab semantics); returns
+        (x, flag, relres, iter, resvec) with iter in half steps / 2 and resvec
+        the absolute residual norms of every half step."""
+        b = np.ascontiguousarray(np.asarray(b, np.complex128))
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(np.asarray(x0, np.complex128)).copy()
+        flag = ctypes.c_int()
+        relres = ctypes.c_double()
+        it = ctypes.c_double()
+        cap = 2 * maxit + 1
+        resvec = np.zeros(cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_bicgstab(self._h, _ptr(b), _ptr(x), tol, maxit, 1 if precond else 0,
+                                        ctypes.byref(flag), ctypes.byref(relres), ctypes.byref(it),
+                                        resvec.ctypes.data_as(_DP), cap, ctypes.byref(nres)))
+        return x, flag.value, relres.value, it.value, resvec[:min(nres.value, cap)].copy()
+
+    def bicgstab_device(self, b_ptr, x_ptr, tol=1e-6, maxit=1000, precond=False):
+        flag = ctypes.c_int()
+        relres = ctypes.c_double()
+        it = ctypes.c_double()
+        cap = 2 * maxit + 1
+        resvec = np.zeros(cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_bicgstab_device(self._h, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), tol, maxit,
+                                               1 if precond else 0, ctypes.byref(flag), ctypes.byref(relres),
+                                               ctypes.byref(it), resvec.ctypes.data_as(_DP), cap,
+                                               ctypes.byref(nres)))
+        return flag.value, relres.value, it.value, resvec[:min(nres.value, cap)].copy()
 
     # -- tiles ------------------------------------------------------------------
     def autotune(self, prec=PREC_F64, iters=5):

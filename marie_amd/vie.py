@@ -214,12 +214,13 @@ def absorbed_power(E, J, Gram):
     return 0.5 * np.real(np.sum(E * np.conj(J))) * Gram
 
 
-def Scat_Solver(Einc, Hinc, freq, r, epsilon_r, sigma_e, rho, tol=1e-3, fN=None, fK=None, form=2, solver='G',
+def Scat_Solver(Einc, Hinc, freq, r, epsilon_r, sigma_e, rho, tol=1e-3, fN=None, fK=None, form=2, solver='M',
                 precond=0, max_it=1000, inner_it=50, outer_it=200, ritz=5, gpu_flag=1, logfile=None,
                 precision='fp64'):
     """[Jout, Sout, Eout, Bout, Gsar, Pabs, fN, fK] of Scat_Solver.m:1 on the device.
 
-    e_r = eps_r - j sigma/(eo omega) (:144); operators assembled when not given
+    e_r = eps_r - j sigma/(eo omega) (:144); solver 'M' (the default, :94-99)
+    reaches JVIE_Solver as BiCGStab; operators assembled when not given
     (:157-162); currents by JVIE_Solver (:169); local SAR |J|^2/(2 sigma rho)
     (:179-186); E = E_field_Nop (:204); B = mu0 H_field_Kop (:215-216);
     Pabs = 0.5 Re sum(E . conj J) Gram and Gsar = sum(Sout) Gram (:226-229).
@@ -288,22 +289,29 @@ def jvie_system(Einc, e_r, r, freq, form=2):
     return Mr, Mc, Gram, idxS3 + 1, Vrhs
 
 
-def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='G', precond=0, max_it=1000,
+def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0, max_it=1000,
                 inner_it=50, outer_it=200, ritz=5, gpu_flag=1, logfile=None, precision='fp64',
                 inner_tol=1e-4):
     """[Je, flag, relres, iter, resvec, solvert] of JVIE_Solver.m:1 on the device.
 
-    Implemented: solver 'G' (restarted GMRES, pgmres semantics) without
-    preconditioner, forms I and II. precision='fp32-refined' runs complex64
-    matvecs inside FP64 GMRES with FP64 iterative refinement.
+    solver 'G': restarted GMRES (pgmres semantics), precond 'R' / 'L' with the
+    preconditioner mv_AN(., 1./Mr, 1./Mr - 1) (JVIE_Solver.m:316-372);
+    precision='fp32-refined' runs complex64 matvecs inside FP64 GMRES with
+    FP64 iterative refinement (no preconditioner). Any other solver letter
+    except 'D' runs BiCGStab (JVIE_Solver.m:451-492, the default), preconditioned
+    when `precond` is 'L' or 'R'; iter is then in half steps / 2 and resvec absolute.
+    Forms I and II (JVIE_Solver.m:176-300).
     """
     _require_gpu(gpu_flag)
     if tol < 1e-15 or tol >= 1:          # JVIE_Solver.m:78-80
         tol = 1e-3
-    if solver != 'G':
-        raise NotImplementedError("solver '%s': the device path provides GMRES ('G')" % solver)
-    if precond in ('L', 'R'):
-        raise NotImplementedError('preconditioned GMRES is not on the device path')
+    if solver not in ('G', 'D'):         # JVIE_Solver.m:85-91: anything else is BiCGStab
+        solver = 'B'
+    if solver == 'D':
+        raise NotImplementedError("solver 'D' (GMRES-DR) is not on the device path")
+    use_pc = precond in ('L', 'R')       # JVIE_Solver.m:92-98
+    if use_pc and precision != 'fp64':
+        raise ValueError('the preconditioned solvers run in FP64')
     Mr, Mc, Gram, idx, Vrhs = jvie_system(Einc, e_r, r, freq, form)
     L, M, N = e_r.shape
     prec = lib.PREC_F64 | (lib.PREC_F32 if precision == 'fp32-refined' else 0)
@@ -315,8 +323,18 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='G', precond=0,
     op.set_material(Mr, Mc, Gram)
     op.set_index(idx, one_based=True)
     t0 = time.perf_counter()
-    x, flag, relres, it, resvec = op.gmres(Vrhs, None, inner_it, tol, outer_it,
-                                           mode=1 if precision == 'fp32-refined' else 0, inner_tol=inner_tol)
+    if solver == 'G':
+        if use_pc:
+            side = lib.PRECOND_RIGHT if precond == 'R' else lib.PRECOND_LEFT
+            x, flag, relres, it, resvec = op.gmres_precond(Vrhs, None, inner_it, tol, outer_it, side=side)
+        else:
+            x, flag, relres, it, resvec = op.gmres(Vrhs, None, inner_it, tol, outer_it,
+                                                   mode=1 if precision == 'fp32-refined' else 0,
+                                                   inner_tol=inner_tol)
+    else:
+        if precision != 'fp64':
+            raise ValueError('BiCGStab runs in FP64')
+        x, flag, relres, it, resvec = op.bicgstab(Vrhs, None, tol, max_it, precond=use_pc)
     solvert = time.perf_counter() - t0
     Je = np.zeros(3 * L * M * N, complex)
     Je[idx - 1] = x

@@ -85,51 +85,76 @@ def pgmres(A, b, restart=50, tol=1e-3, maxit=10, L1=None, L2=None, R1=None, R2=N
 
 
 def bicgstab(A, b, tol=1e-6, maxit=None, M=None, x0=None):
-    """Textbook preconditioned BiCGStab (van der Vorst 1992) with MATLAB's
-    return convention (x, flag, relres, iter, resvec); resvec holds the
-    residual norm at every half step. MATLAB's built-in bicgstab (used by
-    JVIE_Solver.m:463-490) is not in the reference tree, so its exact iterate
-    sequence is not reproduced (parity unpinned beyond the solution itself).
+    """Preconditioned BiCGStab (van der Vorst 1992) with MATLAB's return
+    convention (x, flag, relres, iter, resvec), the solver of
+    JVIE_Solver.m:463-490 (bicgstab(afun, Vrhs, tol, max_it[, pfun]); the
+    preconditioner handle returns the preconditioned vector, iterapp.m:42-58).
+
+    MATLAB's built-in bicgstab is not in the reference tree, so this follows
+    its documented contract: the shadow residual is r0; resvec holds the
+    absolute residual norm at every half step starting with ||r0||; the
+    iteration stops when a half- or full-step residual is <= tol*||b|| and
+    iter counts half steps (k - 0.5 when it stops halfway); a zero <r~, r>,
+    <r~, v>, <t, t> or omega ends it as a breakdown (flag 4 unless the true
+    residual meets tol); flag and relres come from the true residual.
+    Its iterates are pinned by scipy.sparse.linalg.bicgstab (the same
+    recurrences) at fixed iteration counts in the tests.
     """
     n = b.shape[0]
     maxit = maxit or min(n, 20)
-    x = np.zeros(n, complex) if x0 is None else x0.astype(complex)
     Mi = (lambda v: v) if M is None else M
+    x = np.zeros(n, complex) if x0 is None else np.array(x0, complex)
     r = b - A(x) if np.count_nonzero(x) else b.astype(complex)
     nb = np.linalg.norm(b)
     rt = r.copy()
     resvec = [np.linalg.norm(r)]
-    rho_old = alpha = omega = 1.0 + 0j
-    v = np.zeros(n, complex)
-    p = np.zeros(n, complex)
-    it = 0
-    for it in range(1, maxit + 1):
+    half = 0
+    breakdown = False
+    if resvec[0] > tol * nb:
         rho = np.vdot(rt, r)
-        if rho == 0:
-            break
-        if it == 1:
-            p = r.copy()
-        else:
-            beta = (rho / rho_old) * (alpha / omega)
-            p = r + beta * (p - omega * v)
-        ph = Mi(p)
-        v = A(ph)
-        alpha = rho / np.vdot(rt, v)
-        s = r - alpha * v
-        x_half = x + alpha * ph
-        resvec.append(np.linalg.norm(s))
-        if resvec[-1] <= tol * nb:
-            x = x_half
-            break
-        sh = Mi(s)
-        t = A(sh)
-        omega = np.vdot(t, s) / np.vdot(t, t)
-        x = x_half + omega * sh
-        r = s - omega * t
-        resvec.append(np.linalg.norm(r))
-        if resvec[-1] <= tol * nb:
-            break
-        rho_old = rho
-    relres = np.linalg.norm(b - A(x)) / nb
-    flag = 0 if relres <= tol else 1
-    return x, flag, relres, it, np.array(resvec)
+        rho_old = alpha = omega = 1.0 + 0j
+        v = np.zeros(n, complex)
+        p = np.zeros(n, complex)
+        for it in range(1, maxit + 1):
+            if rho == 0:
+                breakdown = True
+                break
+            if it == 1:
+                p = r.copy()
+            else:
+                beta = (rho / rho_old) * (alpha / omega)
+                p = r + beta * (p - omega * v)
+            ph = Mi(p)
+            v = A(ph)
+            rtv = np.vdot(rt, v)
+            if rtv == 0:
+                breakdown = True
+                break
+            alpha = rho / rtv
+            s = r - alpha * v
+            x = x + alpha * ph
+            resvec.append(np.linalg.norm(s))
+            half += 1
+            if resvec[-1] <= tol * nb:
+                break
+            sh = Mi(s)
+            t = A(sh)
+            tt = np.vdot(t, t)
+            if tt == 0:
+                breakdown = True
+                break
+            omega = np.vdot(t, s) / tt
+            if omega == 0:
+                breakdown = True
+                break
+            x = x + omega * sh
+            r = s - omega * t
+            resvec.append(np.linalg.norm(r))
+            half += 1
+            if resvec[-1] <= tol * nb:
+                break
+            rho_old = rho
+            rho = np.vdot(rt, r)
+    relres = np.linalg.norm(b - A(x)) / nb if nb > 0 else 0.0
+    flag = 0 if relres <= tol else (4 if breakdown else 1)
+    return x, flag, relres, half / 2, np.array(resvec)

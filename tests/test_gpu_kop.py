@@ -106,7 +106,7 @@ def test_sphere_H_field_against_mie():
         Einc, Hinc = vie.PlaneWave_Excitation(r, [0, 0, em['ko']], em['omega_mu'], [1, 0, 0])
         fN = vie.getOPERATORS(r, f, 'N', return_operator=True)
         fK = vie.getOPERATORS(r, f, 'K', return_operator=True)
-        J, flag, relres, *_ = vie.JVIE_Solver(Einc, e_r, r, f, fN, tol=1e-8)
+        J, flag, relres, *_ = vie.JVIE_Solver(Einc, e_r, r, f, fN, tol=1e-8, solver='G')
         assert flag == 0
         H = vie.H_field_Kop(J, fK, dx ** 3, Hinc)
         inner = R < 0.6 * a

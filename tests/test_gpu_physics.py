@@ -45,7 +45,7 @@ def _solve_sphere(n, dx, f, eps, sig):
     print("case sphere", n, dx, f, eps, sig, flush=True)
     r, e_r, Einc, inside, em = _sphere(n, dx, f, eps, sig)
     op = vie.getOPERATORS(r, f, 'N', return_operator=True)
-    Je, flag, relres, it, resvec, _ = vie.JVIE_Solver(Einc, e_r, r, f, op, tol=1e-6)
+    Je, flag, relres, it, resvec, _ = vie.JVIE_Solver(Einc, e_r, r, f, op, tol=1e-6, solver='G')
     assert flag == 0 and relres <= 1e-6
     tau = 1j * em['omega'] * em['eo'] * (e_r - 1)
     E = Je[inside] / tau[inside][:, None]

@@ -1,0 +1,151 @@
+"""GPU parity of the remaining JVIE_Solver branches: the preconditioner
+operator (mv_AN with 1./Mr, 1./Mr - 1), left/right-preconditioned GMRES
+(pgmres L1 / R1) and BiCGStab (plain and preconditioned), each against the
+oracle on the same seeded problems; forms I and II."""
+import numpy as np
+import pytest
+
+from oracle import fields, krylov, mvp
+from oracle import operators as oop
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _problem(n=12, dx=6.25e-3, f=298.2e6, form=2, eps=50.0, sig=0.5):
+    x = (np.arange(n) - (n - 1) / 2) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    r = np.stack([X, Y, Z], axis=-1)
+    inside = X ** 2 + Y ** 2 + Z ** 2 < (0.45 * n * dx) ** 2
+    em = fields.em_constants(f)
+    e_r = np.where(inside, eps - 1j * sig / (em['eo'] * em['omega']), 1.0 + 0j)
+    Einc, _ = fields.plane_wave_excitation(r, [0, 0, em['ko']], em['omega_mu'], [1, 0, 0])
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(Einc, e_r, dx, f, form=form)
+    return fN, Mr, Mc, Gram, idx, b
+
+
+def _op(fN, Mr, Mc, Gram, idx):
+    from marie_amd import lib
+    n = Mr.shape[0]
+    op = lib.Operator(n, n, n, precisions=lib.PREC_F64 | lib.PREC_F32)
+    op.set_fN(fN)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx)
+    return op
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_preconditioner_operator(form):
+    """OP_P = mv_AN(., 1./Mr, 1./Mr - 1): the full product in form I, and
+    exactly Gram*x in form II (Mr = 1), both directions."""
+    from marie_amd import lib
+    print("case OP_P form", form, flush=True)
+    fN, Mr, Mc, Gram, idx, b = _problem(form=form)
+    op = _op(fN, Mr, Mc, Gram, idx)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((idx.size, 2)) + 1j * rng.standard_normal((idx.size, 2))
+    for transp in ('notransp', 'transp'):
+        y = op.apply(x, lib.OP_P, transp=transp == 'transp')
+        for q in range(2):
+            yr = mvp.mv_AN(x[:, q], fN, 1.0 / Mr, 1.0 / Mr - 1.0, Gram, transp, idx)
+            assert _rel(y[:, q], yr) <= 1e-12
+            if form == 2:
+                assert np.array_equal(y[:, q], Gram * x[:, q])
+    # the operator follows a material change (derived data recomputed)
+    op.set_material(2.0 * Mr, Mc, Gram)
+    y = op.apply(x[:, 0], lib.OP_P)
+    yr = mvp.mv_AN(x[:, 0], fN, 0.5 / Mr, 0.5 / Mr - 1.0, Gram, 'notransp', idx)
+    assert _rel(y, yr) <= 1e-12
+
+
+@pytest.mark.parametrize("side,form", [('R', 1), ('L', 1), ('R', 2), ('L', 2)])
+def test_preconditioned_gmres_parity(side, form):
+    from marie_amd import lib
+    print("case pgmres", side, "form", form, flush=True)
+    fN, Mr, Mc, Gram, idx, b = _problem(form=form)
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx)
+    P = lambda v: mvp.mv_AN(v, fN, 1.0 / Mr, 1.0 / Mr - 1.0, Gram, 'notransp', idx)
+    if side == 'R':
+        xr, flr, rr, itr, rvr = krylov.pgmres(A, b, 20, 1e-8, 20, None, None, P)
+    else:
+        xr, flr, rr, itr, rvr = krylov.pgmres(A, b, 20, 1e-8, 20, P)
+    op = _op(fN, Mr, Mc, Gram, idx)
+    s = lib.PRECOND_RIGHT if side == 'R' else lib.PRECOND_LEFT
+    x, fl, rel, it, rv = op.gmres_precond(b, restart=20, tol=1e-8, maxit=20, side=s)
+    assert fl == flr == 0
+    assert it == itr and rv.size == rvr.size
+    assert np.max(np.abs(rv - rvr)) <= 1e-9
+    assert _rel(x, xr) <= 1e-8
+    assert abs(rel - rr) <= 1e-9
+
+
+@pytest.mark.parametrize("precond,form", [(False, 2), (True, 1), (True, 2), (False, 1)])
+def test_bicgstab_parity(precond, form):
+    """Same recurrences as the oracle: the same number of half steps, resvec
+    to rounding until the iterates separate, the solution to the tolerance."""
+    print("case bicgstab precond", precond, "form", form, flush=True)
+    fN, Mr, Mc, Gram, idx, b = _problem(form=form)
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx)
+    M = (lambda v: mvp.mv_AN(v, fN, 1.0 / Mr, 1.0 / Mr - 1.0, Gram, 'notransp', idx)) if precond else None
+    tol = 1e-9
+    xr, flr, rr, itr, rvr = krylov.bicgstab(A, b, tol, 400, M=M)
+    op = _op(fN, Mr, Mc, Gram, idx)
+    x, fl, rel, it, rv = op.bicgstab(b, tol=tol, maxit=400, precond=precond)
+    assert flr == 0 and fl == 0 and rel <= tol
+    # a short-recurrence method: rounding differences grow (a perturbation of
+    # b by 2^-50 moves the oracle's own count by up to 2.5 steps here), so the
+    # counts may differ by a few steps; the first steps agree to rounding
+    assert abs(it - itr) <= 4.0
+    k = min(rv.size, rvr.size, 12)
+    assert np.max(np.abs(rv[:k] - rvr[:k]) / rvr[0]) <= 1e-10
+    assert _rel(x, xr) <= 100 * tol
+
+
+def test_bicgstab_fixed_iterations_and_edges():
+    """maxit reached: flag 1, exactly 2 maxit + 1 resvec entries and the oracle's
+    iterate after those steps; x0 given; zero right-hand side; a converged
+    initial guess takes no step."""
+    print("case bicgstab maxit / x0 / zero", flush=True)
+    fN, Mr, Mc, Gram, idx, b = _problem(n=10)
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx)
+    op = _op(fN, Mr, Mc, Gram, idx)
+    rng = np.random.default_rng(2)
+    x0 = 1e-3 * (rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)) * np.abs(b).max()
+    for k in (1, 3, 6):
+        xr, flr, rr, itr, rvr = krylov.bicgstab(A, b, 1e-14, k, x0=x0)
+        x, fl, rel, it, rv = op.bicgstab(b, x0=x0, tol=1e-14, maxit=k)
+        assert fl == flr == 1 and it == itr == k
+        assert rv.size == rvr.size == 2 * k + 1
+        assert np.max(np.abs(rv - rvr) / rvr[0]) <= 1e-11
+        assert _rel(x, xr) <= 1e-11
+        assert abs(rel - rr) <= 1e-11
+    x, fl, rel, it, rv = op.bicgstab(np.zeros_like(b))
+    assert fl == 0 and it == 0 and np.all(x == 0)
+    xs, fl, rel, it, rv = op.bicgstab(b, tol=1e-10, maxit=300)
+    x, fl2, rel2, it2, rv2 = op.bicgstab(b, x0=xs, tol=1e-6, maxit=300)
+    assert fl2 == 0 and it2 == 0 and rv2.size == 1 and np.array_equal(x, xs)
+
+
+def test_jvie_solver_dispatch():
+    """JVIE_Solver's default (BiCGStab), 'G' and the preconditioned branches
+    reach the same currents on a small sphere in form I."""
+    from marie_amd import vie
+    print("case JVIE_Solver dispatch", flush=True)
+    n, dx, f = 12, 6.25e-3, 298.2e6
+    x = (np.arange(n) - (n - 1) / 2) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    r = np.stack([X, Y, Z], axis=-1)
+    em = vie.EMconstants(f)
+    e_r = np.where(X ** 2 + Y ** 2 + Z ** 2 < (0.45 * n * dx) ** 2, 50 - 1j * 0.5 / (em['eo'] * em['omega']), 1.0)
+    Einc, _ = vie.PlaneWave_Excitation(r, [0, 0, em['ko']], em['omega_mu'], [1, 0, 0])
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    ref, flag, *_ = vie.JVIE_Solver(Einc, e_r, r, f, fN, 1e-10, 1, 'G')
+    assert flag == 0
+    for solver, precond in (('B', 0), ('M', 'L'), ('G', 'R'), ('G', 'L'), ('B', 'R')):
+        J, flag, relres, it, resvec, t = vie.JVIE_Solver(Einc, e_r, r, f, fN, 1e-8, 1, solver, precond)
+        assert flag == 0 and relres <= 1e-8, (solver, precond)
+        assert _rel(J, ref) <= 1e-6, (solver, precond)

@@ -155,3 +155,36 @@ def test_bicgstab_converges():
     b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     x, flag, relres, it, resvec = krylov.bicgstab(lambda v: A @ v, b, 1e-10, 200)
     assert flag == 0 and np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 6])
+def test_bicgstab_iterates_match_scipy(k):
+    """After k full iterations (tolerance never met) the oracle's iterate is
+    scipy.sparse.linalg.bicgstab's, whose recurrences are van der Vorst's with
+    the shadow residual r0 -- with and without a preconditioner."""
+    import scipy.sparse.linalg as sla
+    rng = np.random.default_rng(3)
+    n = 40
+    A = np.eye(n) * 2 + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n) * 0.8
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    P = np.diag(1 / np.diag(A)) + 0.01 * rng.standard_normal((n, n))
+    xs, _ = sla.bicgstab(A, b, rtol=1e-30, atol=0, maxiter=k)
+    x, flag, relres, it, resvec = krylov.bicgstab(lambda v: A @ v, b, 1e-30, k)
+    assert np.linalg.norm(x - xs) <= 1e-13 * np.linalg.norm(xs)
+    assert it == k and len(resvec) == 2 * k + 1 and flag == 1
+    assert abs(resvec[-1] - np.linalg.norm(b - A @ x)) <= 1e-10 * resvec[0]
+    xs, _ = sla.bicgstab(A, b, rtol=1e-30, atol=0, maxiter=k, M=sla.aslinearoperator(P))
+    x, *_ = krylov.bicgstab(lambda v: A @ v, b, 1e-30, k, M=lambda v: P @ v)
+    assert np.linalg.norm(x - xs) <= 1e-13 * np.linalg.norm(xs)
+
+
+def test_bicgstab_half_step_and_breakdown():
+    # A = c I: the first half step is exact (iter 0.5, x = b / c)
+    b = np.array([1 + 1j, 2.0, -0.5j])
+    x, flag, relres, it, resvec = krylov.bicgstab(lambda v: (2 - 1j) * v, b, 1e-12, 10)
+    assert flag == 0 and it == 0.5 and len(resvec) == 2
+    assert np.allclose(x, b / (2 - 1j), rtol=1e-15, atol=0)
+    # real skew-symmetric A: <r0, A r0> = 0 breaks down before any update
+    A = np.array([[0.0, 1.0], [-1.0, 0.0]])
+    x, flag, relres, it, resvec = krylov.bicgstab(lambda v: A @ v, np.array([1.0, 2.0]), 1e-10, 10)
+    assert flag == 4 and it == 0 and relres == 1.0 and np.all(x == 0)
