@@ -173,6 +173,25 @@ int marie_vie_gmres_precond(marie_vie_op* h, const double* b, double* x, int res
                             int side, int* flag, double* relres, int* iter, double* resvec, int resvec_cap,
                             int* nres);
 
+/* GMRES with deflated restarts, FP64: JVIE_Solver.m:378-450 (solver 'D')
+ * calling pgmresDR(afun, Vrhs, inner_it, tol, outer_it, ritz[, L1 | R1])
+ * (src_numeric/src_iterative/pgmresDR.m:73-282, pgmresDR_iter.m:23-142,
+ * getHarmVecs1.m, getHarmVecs2.m): a GCRO-DR cycle that keeps the `ritz`
+ * harmonic Ritz vectors of smallest magnitude (U, C = A U) between restarts.
+ * ritz < 1 selects min(10, round(restart/2)); larger values are limited to
+ * restart - 1 and to 32. side as marie_vie_gmres_precond_device. iter =
+ * {steps of the last cycle, cycles after the first}; resvec relative to
+ * ||b|| (||P b|| for left preconditioning); flag/relres by the true
+ * residual. restart 2..64. */
+int marie_vie_gmres_dr_device(marie_vie_op* h, const void* b_dev, void* x_dev, int restart, double tol,
+                              int maxit, int ritz, int side, int* flag, double* relres, int* iter,
+                              double* resvec, int resvec_cap, int* nres);
+
+/* Same with host buffers b, x (complex128, nidx each). */
+int marie_vie_gmres_dr(marie_vie_op* h, const double* b, double* x, int restart, double tol, int maxit,
+                       int ritz, int side, int* flag, double* relres, int* iter, double* resvec,
+                       int resvec_cap, int* nres);
+
 /* BiCGStab, FP64, the solver JVIE_Solver.m:463-490 calls (MATLAB's
  * bicgstab(afun, Vrhs, tol, max_it[, pfun]) with pfun = MARIE_OP_P when
  * `precond` = 1, applied to p and s as a function-handle M). Van der Vorst's

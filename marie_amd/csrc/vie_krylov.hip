@@ -22,6 +22,7 @@
 #include "../../include/marie_vie.h"
 #include "vie_handle.hpp"
 #include "vie_reduce.hpp"
+#include "vie_dense.hpp"
 
 #include <cmath>
 #include <complex>
@@ -106,7 +107,7 @@ template <typename TY, typename TV>
 __global__ __launch_bounds__(KT) void k_vcomb(c2<TY>* __restrict__ y, double beta, const c2<TV>* __restrict__ V,
                                               int64_t ldv, int k, const c2<double>* __restrict__ coef, double csign,
                                               int64_t n, double* __restrict__ npart, int G) {
-    __shared__ c2<double> cs[80];
+    __shared__ c2<double> cs[128];
     __shared__ double sh[KT / 64];
     for (int i = threadIdx.x; i < k; i += KT) cs[i] = coef[i];
     __syncthreads();
@@ -590,6 +591,463 @@ int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxi
     return done(OK);
 }
 
+// ------------------------------------------------------------ GMRES with deflated restarts
+// pgmresDR.m:73-282 / pgmresDR_iter.m:23-142 (GCRO-DR): between cycles a pair
+// U, C = A U (C orthonormal, k columns) spanned by the harmonic Ritz vectors
+// of smallest magnitude is kept; each cycle's Arnoldi vectors are made
+// orthogonal to C as well; the update minimises the residual over
+// span([U V_p]) through H2 = [D B; 0 H].
+//
+// Device layout: one array W of kmax + m + 1 columns holds C in its last k of
+// the first kmax slots followed by V, so [C V] is one contiguous block for the
+// multi-dot and the combinations. Since V is built orthogonal to C, the
+// reference's two projections (B = C^H w, then H = V^H (w - C B)) are taken
+// in one classical Gram-Schmidt pass over [C V] (equal in exact arithmetic).
+// The small dense algebra (least squares, thin QR, the harmonic Ritz
+// eigenproblems) runs on the host in FP64 (vie_dense.hpp).
+
+// Out[:, j0 + q] (+)= sum_i In[:, i] coef[(j0 + q) * nin + i], q < KO, nin <= 128
+template <int KO>
+__global__ __launch_bounds__(KT) void k_mcomb(const c2<double>* __restrict__ In, int nin,
+                                              const c2<double>* __restrict__ coef, int nout, int j0,
+                                              c2<double>* __restrict__ Out, int accumulate, int64_t n) {
+    __shared__ c2<double> cs[KO * 128];
+    const int nq = (nout - j0) < KO ? (nout - j0) : KO;
+    for (int t = threadIdx.x; t < nq * nin; t += KT) cs[t] = coef[(int64_t)j0 * nin + t];
+    __syncthreads();
+    for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)gridDim.x * KT) {
+        double ar[KO], ai[KO];
+#pragma unroll
+        for (int q = 0; q < KO; ++q) ar[q] = ai[q] = 0.0;
+        for (int i = 0; i < nin; ++i) {
+            const c2<double> v = In[(int64_t)i * n + e];
+#pragma unroll
+            for (int q = 0; q < KO; ++q) {
+                if (q < nq) {
+                    const c2<double> c = cs[q * nin + i];
+                    ar[q] += v.x * c.x - v.y * c.y;
+                    ai[q] += v.x * c.y + v.y * c.x;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < KO; ++q) {
+            if (q < nq) {
+                c2<double>* o = Out + (int64_t)(j0 + q) * n + e;
+                if (accumulate) {
+                    const c2<double> old = *o;
+                    *o = c2<double>{old.x + ar[q], old.y + ai[q]};
+                } else {
+                    *o = c2<double>{ar[q], ai[q]};
+                }
+            }
+        }
+    }
+}
+
+namespace {
+
+constexpr int DR_MAXK = 32;
+
+struct DrWs {
+    c2<double>*U = nullptr, *T1 = nullptr, *T2 = nullptr, *xacc = nullptr, *coef = nullptr;
+    void free_all() {
+        void* p[] = {U, T1, T2, xacc, coef};
+        for (void* q : p)
+            if (q) (void)hipFree(q);
+    }
+};
+
+using dense::cd;
+using dense::Mat;
+
+// coefficients (host, column-major nin x nout) -> device, then Out = In coef
+static int mcomb(Ctx& c, DrWs& d, const c2<double>* In, int nin, const Mat& coef, int nout, c2<double>* Out,
+                 bool accumulate) {
+    if (nout == 0 || nin == 0) return OK;
+    MARIE_HIP(hipStreamSynchronize(c.s));      // the pinned staging area is reused
+    c2<double>* hc = c.ws.hp;
+    for (size_t i = 0; i < (size_t)nin * nout; ++i) hc[i] = c2<double>{coef[i].real(), coef[i].imag()};
+    MARIE_HIP(hipMemcpyAsync(d.coef, hc, (size_t)nin * nout * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
+    for (int j0 = 0; j0 < nout; j0 += 8)
+        hipLaunchKernelGGL(k_mcomb<8>, dim3(c.G), dim3(KT), 0, c.s, In, nin, d.coef, nout, j0, Out,
+                           accumulate ? 1 : 0, c.n);
+    MARIE_HIP(hipGetLastError());
+    return OK;
+}
+
+// g = [cols]^H v (ncols entries) to the host
+static int multidot(Ctx& c, const c2<double>* cols, int ncols, const c2<double>* v, std::vector<cd>& g) {
+    hipLaunchKernelGGL(k_mdot<double>, dim3(c.G), dim3(KT), 0, c.s, cols, c.n, ncols, v, c.n, c.ws.part, c.G);
+    hipLaunchKernelGGL(k_reduce, dim3(ncols), dim3(KT), 0, c.s, c.ws.part, c.G, c.ws.h);
+    MARIE_HIP(hipGetLastError());
+    MARIE_HIP(hipMemcpyAsync(c.ws.hp, c.ws.h, ncols * sizeof(c2<double>), hipMemcpyDeviceToHost, c.s));
+    MARIE_HIP(hipStreamSynchronize(c.s));
+    g.resize(ncols);
+    for (int i = 0; i < ncols; ++i) g[i] = cd(c.ws.hp[i].x, c.ws.hp[i].y);
+    return OK;
+}
+
+// y += sign * cols coef (coef on the host)
+static int vcomb_host(Ctx& c, c2<double>* y, const c2<double>* cols, int ncols, const std::vector<cd>& coef,
+                      double sign) {
+    if (ncols == 0) return OK;
+    MARIE_HIP(hipStreamSynchronize(c.s));
+    for (int i = 0; i < ncols; ++i) c.ws.hp[i] = c2<double>{coef[i].real(), coef[i].imag()};
+    MARIE_HIP(hipMemcpyAsync(c.ws.h, c.ws.hp, ncols * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
+    hipLaunchKernelGGL((k_vcomb<double, double>), dim3(c.G), dim3(KT), 0, c.s, y, 1.0, cols, c.n, ncols, c.ws.h,
+                       sign, c.n, (double*)nullptr, c.G);
+    MARIE_HIP(hipGetLastError());
+    return OK;
+}
+
+// One pgmresDR_iter cycle on r with the recycled block Cb (k columns right
+// before V in memory). Returns p, H ((p+1) x p), B (k x p); appends the LS
+// residuals to resv.
+static int dr_cycle(Ctx& c, c2<double>* Cb, int k, c2<double>* V, const c2<double>* r, int m, double abstol,
+                    int* pout, Mat& H, Mat& B, std::vector<double>& resv) {
+    const int64_t n = c.n;
+    c2<double>* w = reinterpret_cast<c2<double>*>(c.ws.w);
+    double beta = 0.0;
+    int st = norm<double>(c, r, &beta);
+    if (st) return st;
+    hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, V, 1.0 / beta, r, 0.0, r, n);
+    const int ld = m + 1;
+    Mat Hf((size_t)ld * m, 0.0), Bf((size_t)(k > 0 ? k : 1) * m, 0.0);
+    std::vector<double> cs(m);
+    std::vector<cd> sn(m), g(m + 1, 0.0), col(k + m + 1);
+    g[0] = beta;
+    int p = m;
+    for (int j = 1; j <= m; ++j) {
+        st = cycle_matvec<double>(c, V + (int64_t)(j - 1) * n, w, false);
+        if (st) return st;
+        const int nc = k + j;                 // [C V_1..j]
+        hipLaunchKernelGGL(k_mdot<double>, dim3(c.G), dim3(KT), 0, c.s, Cb, n, nc, w, n, c.ws.part, c.G);
+        hipLaunchKernelGGL(k_reduce, dim3(nc), dim3(KT), 0, c.s, c.ws.part, c.G, c.ws.h);
+        hipLaunchKernelGGL((k_vcomb<double, double>), dim3(c.G), dim3(KT), 0, c.s, w, 1.0, Cb, n, nc, c.ws.h, -1.0,
+                           n, c.ws.npart, c.G);
+        hipLaunchKernelGGL(k_norm_entry, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, c.ws.h + nc);
+        hipLaunchKernelGGL(k_scale_by_entry<double>, dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)j * n, w,
+                           c.ws.h + nc, n);
+        MARIE_HIP(hipGetLastError());
+        MARIE_HIP(hipMemcpyAsync(c.ws.hp, c.ws.h, (nc + 1) * sizeof(c2<double>), hipMemcpyDeviceToHost, c.s));
+        MARIE_HIP(hipStreamSynchronize(c.s));
+        for (int i = 0; i < k; ++i) Bf[(size_t)(j - 1) * k + i] = cd(c.ws.hp[i].x, c.ws.hp[i].y);
+        for (int i = 0; i <= j; ++i) {
+            col[i] = cd(c.ws.hp[k + i].x, c.ws.hp[k + i].y);
+            Hf[(size_t)(j - 1) * ld + i] = col[i];
+        }
+        // Givens update of the least-squares problem H y = beta e1
+        for (int i = 0; i < j - 1; ++i) {
+            const cd a = col[i], b = col[i + 1];
+            col[i] = cs[i] * a + sn[i] * b;
+            col[i + 1] = -std::conj(sn[i]) * a + cs[i] * b;
+        }
+        double cc;
+        cd ss;
+        dense::givens(col[j - 1], col[j], cc, ss);
+        cs[j - 1] = cc;
+        sn[j - 1] = ss;
+        const cd gk = g[j - 1];
+        g[j - 1] = cc * gk;
+        g[j] = -std::conj(ss) * gk;
+        const double res = std::abs(g[j]);
+        resv.push_back(res);
+        if (res < abstol) {
+            p = j;
+            break;
+        }
+    }
+    H.assign((size_t)(p + 1) * p, 0.0);
+    for (int j = 0; j < p; ++j)
+        for (int i = 0; i <= p; ++i) H[(size_t)j * (p + 1) + i] = Hf[(size_t)j * ld + i];
+    B.assign((size_t)k * p, 0.0);
+    for (size_t i = 0; i < (size_t)k * p; ++i) B[i] = Bf[i];
+    *pout = p;
+    return OK;
+}
+
+// indices of the eigenvalues sorted by magnitude (ascending, stable)
+static std::vector<int> by_magnitude(const std::vector<cd>& v) {
+    std::vector<int> o(v.size());
+    for (size_t i = 0; i < v.size(); ++i) o[i] = (int)i;
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return std::abs(v[a]) < std::abs(v[b]); });
+    return o;
+}
+
+}  // namespace
+
+int gmres_dr(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxit, int ritz, int precond,
+             int* flag, double* relres, int* iter, std::vector<double>& resvec) {
+    Ctx c;
+    c.h = h;
+    c.precond = precond;
+    c.s = h->stream;
+    c.n = h->nidx;
+    c.G = grid_for(c.n);
+    const int64_t n = c.n;
+    // pgmresDR.m:112-116
+    ritz = ritz < 1 ? std::min(10, (m + 1) / 2) : std::min(ritz, m - 1);
+    const int kmax = ritz;
+    DrWs d;
+    auto A = [&](void** p, size_t bytes) -> int {
+        if (bytes == 0) return OK;
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            set_error("out of device memory for the GMRES-DR workspace");
+            return ERR_ALLOC;
+        }
+        return OK;
+    };
+    const size_t vb = (size_t)n * sizeof(c2<double>);
+    const int ncol = kmax + m + 1;
+    int st = A(&c.ws.V, (size_t)ncol * vb);
+    if (!st) st = A(&c.ws.w, vb);
+    if (!st) st = A((void**)&c.ws.r, vb);
+    if (!st) st = A((void**)&c.ws.t, vb);
+    if (!st && precond) st = A((void**)&c.ws.pt, vb);
+    if (!st) st = A((void**)&c.ws.part, (size_t)ncol * KG * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.h, (size_t)(ncol + 1) * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.ws.npart, KG * sizeof(double));
+    if (!st) st = A((void**)&c.ws.nout, sizeof(double));
+    if (!st) st = A((void**)&d.U, (size_t)kmax * vb);
+    if (!st) st = A((void**)&d.T1, (size_t)kmax * vb);
+    if (!st) st = A((void**)&d.T2, (size_t)kmax * vb);
+    if (!st) st = A((void**)&d.xacc, vb);
+    const size_t ncoef = (size_t)ncol * (kmax > 0 ? kmax : 1) + ncol;
+    if (!st) st = A((void**)&d.coef, ncoef * sizeof(c2<double>));
+    if (!st && hipHostMalloc((void**)&c.ws.hp, ncoef * sizeof(c2<double>), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c.ws.hp = nullptr;
+        set_error("pinned host allocation failed");
+        st = ERR_ALLOC;
+    }
+    auto done = [&](int code) {
+        (void)hipStreamSynchronize(c.s);
+        c.ws.free_all();
+        d.free_all();
+        return code;
+    };
+    if (st) return done(st);
+    c2<double>* W = reinterpret_cast<c2<double>*>(c.ws.V);
+    c2<double>* V = W + (int64_t)kmax * n;
+    c2<double>* r = c.ws.r;
+    auto fail = [&](const char* msg) {
+        set_error(msg);
+        return done(ERR_STATE);
+    };
+
+    double bnorm_true = 0.0;
+    st = norm<double>(c, b, &bnorm_true);
+    if (st) return done(st);
+    if (bnorm_true == 0.0) {
+        MARIE_HIP(hipMemsetAsync(x, 0, vb, c.s));
+        *flag = 0; *relres = 0.0; iter[0] = 0; iter[1] = 0;
+        resvec.assign(1, 0.0);
+        return done(OK);
+    }
+    // r = b - A x0 (pgmresDR.m:150); left preconditioning works on P r, P b
+    double xn = 0.0;
+    st = norm<double>(c, x, &xn);
+    if (st) return done(st);
+    if (xn > 0.0) {
+        st = matvec<double>(c, x, c.ws.t, false);
+        if (st) return done(st);
+        hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, r, 1.0, b, -1.0, c.ws.t,
+                           n);
+    } else {
+        MARIE_HIP(hipMemcpyAsync(r, b, vb, hipMemcpyDeviceToDevice, c.s));
+    }
+    double bnorm = bnorm_true;
+    if (precond == 2) {
+        st = apply(h, OP_P, 0, PREC_F64, r, c.ws.pt, 1, c.s);
+        if (st) return done(st);
+        MARIE_HIP(hipMemcpyAsync(r, c.ws.pt, vb, hipMemcpyDeviceToDevice, c.s));
+        st = apply(h, OP_P, 0, PREC_F64, b, c.ws.pt, 1, c.s);
+        if (st) return done(st);
+        st = norm<double>(c, c.ws.pt, &bnorm);
+        if (st) return done(st);
+        if (bnorm == 0.0) return fail("the preconditioner maps the right-hand side to zero");
+    }
+    MARIE_HIP(hipMemsetAsync(d.xacc, 0, vb, c.s));
+    double rn = 0.0;
+    st = norm<double>(c, r, &rn);
+    if (st) return done(st);
+    resvec.assign(1, rn / bnorm);
+    const double abstol = tol * bnorm;
+
+    // first cycle: GMRES, y = H \ (V^H r), harmonic Ritz vectors of H (pgmresDR.m:165-201)
+    int p = 0, k = 0, outit = 0;
+    Mat H, B;
+    std::vector<double> rv;
+    st = dr_cycle(c, W + (int64_t)kmax * n, 0, V, r, m, abstol, &p, H, B, rv);
+    if (st) return done(st);
+    for (double v : rv) resvec.push_back(v / bnorm);
+    {
+        std::vector<cd> g, y;
+        st = multidot(c, V, p + 1, r, g);
+        if (st) return done(st);
+        if (!dense::lstsq(H, p + 1, p, g, y)) return fail("GMRES-DR: singular least-squares problem");
+        st = vcomb_host(c, d.xacc, V, p, y, 1.0);
+        if (st) return done(st);
+        std::vector<cd> z(p + 1, 0.0);
+        for (int j = 0; j < p; ++j)
+            for (int i = 0; i <= p; ++i) z[i] += H[(size_t)j * (p + 1) + i] * y[j];
+        st = vcomb_host(c, r, V, p + 1, z, -1.0);
+        if (st) return done(st);
+    }
+    if (resvec.back() > tol && p > 1) {
+        k = ritz > p - 1 ? p - 1 : ritz;
+        // getHarmVecs1.m:16-28: eig(H_p + h^2 H_p^{-H} e_p e_p^T), k smallest
+        Mat Hp((size_t)p * p), Z((size_t)p, 0.0);
+        for (int j = 0; j < p; ++j)
+            for (int i = 0; i < p; ++i) Hp[(size_t)j * p + i] = H[(size_t)j * (p + 1) + i];
+        Mat HpH((size_t)p * p);
+        for (int j = 0; j < p; ++j)
+            for (int i = 0; i < p; ++i) HpH[(size_t)j * p + i] = std::conj(Hp[(size_t)i * p + j]);
+        // z = H_p^{-H} e_p
+        std::vector<cd> ep(p, 0.0), zz;
+        ep[p - 1] = 1.0;
+        if (!dense::lstsq(HpH, p, p, ep, zz)) return fail("GMRES-DR: singular Hessenberg block");
+        const cd hh = H[(size_t)(p - 1) * (p + 1) + p];
+        Mat Mt = Hp;
+        for (int i = 0; i < p; ++i) Mt[(size_t)(p - 1) * p + i] += hh * hh * zz[i];
+        std::vector<cd> vals;
+        Mat vecs;
+        if (!dense::eig(Mt, p, vals, vecs)) return fail("GMRES-DR: eigenvalue iteration did not converge");
+        const std::vector<int> o = by_magnitude(vals);
+        Mat P((size_t)p * k);
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < p; ++i) P[(size_t)j * p + i] = vecs[(size_t)o[j] * p + i];
+        // [Q, R] = qr(H P, 0); C = V Q; U = V_p P R^{-1}
+        Mat HP((size_t)(p + 1) * k, 0.0), Q, R;
+        for (int j = 0; j < k; ++j)
+            for (int l = 0; l < p; ++l)
+                for (int i = 0; i <= p; ++i) HP[(size_t)j * (p + 1) + i] += H[(size_t)l * (p + 1) + i] * P[(size_t)j * p + l];
+        dense::qr_thin(HP, p + 1, k, Q, R);
+        if (!dense::right_solve_upper(P, p, k, R)) return fail("GMRES-DR: rank-deficient recycled block");
+        st = mcomb(c, d, V, p, P, k, d.U, false);
+        if (!st) st = mcomb(c, d, V, p + 1, Q, k, W + (int64_t)(kmax - k) * n, false);
+        if (st) return done(st);
+    }
+    // later cycles (pgmresDR.m:205-251)
+    while (resvec.back() > tol && outit < maxit) {
+        ++outit;
+        c2<double>* Cb = W + (int64_t)(kmax - k) * n;
+        rv.clear();
+        st = dr_cycle(c, Cb, k, V, r, m, abstol, &p, H, B, rv);
+        if (st) return done(st);
+        for (double v : rv) resvec.push_back(v / bnorm);
+        // D = diag(1 ./ ||U_i||)
+        std::vector<double> dinv(k);
+        for (int i = 0; i < k; ++i) {
+            double ni = 0.0;
+            st = norm<double>(c, d.U + (int64_t)i * n, &ni);
+            if (st) return done(st);
+            if (!(ni > 0.0)) return fail("GMRES-DR: zero recycled vector");
+            dinv[i] = 1.0 / ni;
+        }
+        // H2 = [D B; 0 H], (k + p + 1) x (k + p)
+        const int r2 = k + p + 1, c2n = k + p;
+        Mat H2((size_t)r2 * c2n, 0.0);
+        for (int i = 0; i < k; ++i) H2[(size_t)i * r2 + i] = dinv[i];
+        for (int j = 0; j < p; ++j) {
+            for (int i = 0; i < k; ++i) H2[(size_t)(k + j) * r2 + i] = B[(size_t)j * k + i];
+            for (int i = 0; i <= p; ++i) H2[(size_t)(k + j) * r2 + k + i] = H[(size_t)j * (p + 1) + i];
+        }
+        std::vector<cd> g, y;
+        st = multidot(c, Cb, r2, r, g);
+        if (st) return done(st);
+        if (!dense::lstsq(H2, r2, c2n, g, y)) return fail("GMRES-DR: singular least-squares problem");
+        // x += [U D, V_p] y ; r -= [C V] H2 y
+        std::vector<cd> yu(k), yv(p), z(r2, 0.0);
+        for (int i = 0; i < k; ++i) yu[i] = y[i] * dinv[i];
+        for (int i = 0; i < p; ++i) yv[i] = y[k + i];
+        st = vcomb_host(c, d.xacc, d.U, k, yu, 1.0);
+        if (!st) st = vcomb_host(c, d.xacc, V, p, yv, 1.0);
+        for (int j = 0; j < c2n; ++j)
+            for (int i = 0; i < r2; ++i) z[i] += H2[(size_t)j * r2 + i] * y[j];
+        if (!st) st = vcomb_host(c, r, Cb, r2, z, -1.0);
+        if (st) return done(st);
+        if (!(resvec.back() > tol) || k == 0) continue;
+        // getHarmVecs2.m:18-35 with the normalised U = U_raw D
+        const int m2 = k + p;
+        Mat Mcv((size_t)r2 * k);                        // [C V]^H U (normalised)
+        for (int i = 0; i < k; ++i) {
+            st = multidot(c, Cb, r2, d.U + (int64_t)i * n, g);
+            if (st) return done(st);
+            for (int l = 0; l < r2; ++l) Mcv[(size_t)i * r2 + l] = g[l] * dinv[i];
+        }
+        Mat Am((size_t)(m2 + 1) * m2, 0.0);
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < r2; ++i) Am[(size_t)j * (m2 + 1) + i] = Mcv[(size_t)j * r2 + i];
+        for (int j = k; j < m2; ++j) Am[(size_t)j * (m2 + 1) + j] = 1.0;
+        Mat AH((size_t)m2 * m2, 0.0), BH((size_t)m2 * m2, 0.0);
+        for (int j = 0; j < m2; ++j)
+            for (int i = 0; i < m2; ++i) {
+                cd sa = 0.0, sb = 0.0;
+                for (int l = 0; l < r2; ++l) {
+                    const cd hc = std::conj(H2[(size_t)i * r2 + l]);
+                    sa += hc * Am[(size_t)j * (m2 + 1) + l];
+                    sb += hc * H2[(size_t)j * r2 + l];
+                }
+                AH[(size_t)j * m2 + i] = sa;
+                BH[(size_t)j * m2 + i] = sb;
+            }
+        // eig(A, B) with B = H2^H H2 Hermitian positive definite: eig(B^{-1} A)
+        if (!dense::chol_solve(BH, m2, AH, m2)) return fail("GMRES-DR: H2 lost full column rank");
+        std::vector<cd> vals;
+        Mat vecs;
+        if (!dense::eig(AH, m2, vals, vecs)) return fail("GMRES-DR: eigenvalue iteration did not converge");
+        const std::vector<int> o = by_magnitude(vals);
+        Mat P((size_t)m2 * k);
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < m2; ++i) P[(size_t)j * m2 + i] = vecs[(size_t)o[m2 - k + j] * m2 + i];
+        // [Q, R] = qr(H2 P, 0); C = [C V] Q; U = [U D, V_p] P R^{-1}
+        Mat HP((size_t)r2 * k, 0.0), Q, R;
+        for (int j = 0; j < k; ++j)
+            for (int l = 0; l < m2; ++l)
+                for (int i = 0; i < r2; ++i) HP[(size_t)j * r2 + i] += H2[(size_t)l * r2 + i] * P[(size_t)j * m2 + l];
+        dense::qr_thin(HP, r2, k, Q, R);
+        if (!dense::right_solve_upper(P, m2, k, R)) return fail("GMRES-DR: rank-deficient recycled block");
+        Mat PU((size_t)k * k), PV((size_t)p * k);
+        for (int j = 0; j < k; ++j) {
+            for (int i = 0; i < k; ++i) PU[(size_t)j * k + i] = P[(size_t)j * m2 + i] * dinv[i];
+            for (int i = 0; i < p; ++i) PV[(size_t)j * p + i] = P[(size_t)j * m2 + k + i];
+        }
+        st = mcomb(c, d, d.U, k, PU, k, d.T1, false);
+        if (!st) st = mcomb(c, d, V, p, PV, k, d.T1, true);
+        if (!st) st = mcomb(c, d, Cb, r2, Q, k, d.T2, false);
+        if (st) return done(st);
+        MARIE_HIP(hipMemcpyAsync(Cb, d.T2, (size_t)k * vb, hipMemcpyDeviceToDevice, c.s));
+        std::swap(d.U, d.T1);
+    }
+    // x = x0 + R1(x) (pgmresDR.m:254-269)
+    if (precond == 1) {
+        st = apply(h, OP_P, 0, PREC_F64, d.xacc, c.ws.pt, 1, c.s);
+        if (st) return done(st);
+        hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, x, 1.0, c.ws.pt,
+                           n);
+    } else {
+        hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, x, 1.0, d.xacc,
+                           n);
+    }
+    MARIE_HIP(hipGetLastError());
+    iter[0] = p;
+    iter[1] = outit;
+    st = matvec<double>(c, x, c.ws.t, false);
+    if (st) return done(st);
+    hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, c.ws.t, 1.0, b, -1.0, c.ws.t,
+                       n);
+    double tn = 0.0;
+    st = norm<double>(c, c.ws.t, &tn);
+    if (st) return done(st);
+    *relres = tn / bnorm_true;
+    *flag = (*relres <= tol) ? 0 : 1;
+    return done(OK);
+}
+
 }  // namespace marie
 
 using namespace marie;
@@ -710,5 +1168,50 @@ extern "C" int marie_vie_gmres_precond(marie_vie_op* hh, const double* b, double
     return host_solve(hh, b, x, [&](void* db, void* dx) {
         return marie_vie_gmres_precond_device(hh, db, dx, restart, tol, maxit, side, flag, relres, iter, resvec,
                                               resvec_cap, nres);
+    });
+}
+
+extern "C" int marie_vie_gmres_dr_device(marie_vie_op* hh, const void* b, void* x, int restart, double tol,
+                                         int maxit, int ritz, int side, int* flag, double* relres, int* iter,
+                                         double* resvec, int resvec_cap, int* nres) {
+    Op* h = reinterpret_cast<Op*>(hh);
+    if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
+    MARIE_HIP(hipSetDevice(h->device));
+    if (!b || !x || !flag || !relres || !iter || restart < 2 || restart > 64 || maxit < 1 || !(tol > 0) ||
+        ritz > DR_MAXK || (side != MARIE_PRECOND_NONE && side != MARIE_PRECOND_RIGHT && side != MARIE_PRECOND_LEFT)) {
+        set_error("invalid gmres_dr arguments (restart 2..64, ritz <= 32, side NONE/RIGHT/LEFT)");
+        return MARIE_ERR_ARG;
+    }
+    if (!(h->precisions & PREC_F64)) {
+        set_error("gmres_dr needs an FP64 operator");
+        return MARIE_ERR_STATE;
+    }
+    if (!h->have_fN || !h->have_mat || !h->have_idx) {
+        set_error("operator, index or material not set");
+        return MARIE_ERR_STATE;
+    }
+    if (h->nidx == 0) {
+        *flag = 0; *relres = 0.0; iter[0] = iter[1] = 0;
+        if (nres) *nres = 0;
+        return MARIE_OK;
+    }
+    std::vector<double> rv;
+    int st = gmres_dr(h, (const c2<double>*)b, (c2<double>*)x, restart, tol, maxit, ritz, side, flag, relres, iter,
+                      rv);
+    if (st) return st;
+    if (resvec && resvec_cap > 0) {
+        const int nc = (int)rv.size() < resvec_cap ? (int)rv.size() : resvec_cap;
+        for (int i = 0; i < nc; ++i) resvec[i] = rv[i];
+    }
+    if (nres) *nres = (int)rv.size();
+    return MARIE_OK;
+}
+
+extern "C" int marie_vie_gmres_dr(marie_vie_op* hh, const double* b, double* x, int restart, double tol, int maxit,
+                                  int ritz, int side, int* flag, double* relres, int* iter, double* resvec,
+                                  int resvec_cap, int* nres) {
+    return host_solve(hh, b, x, [&](void* db, void* dx) {
+        return marie_vie_gmres_dr_device(hh, db, dx, restart, tol, maxit, ritz, side, flag, relres, iter, resvec,
+                                         resvec_cap, nres);
     });
 }

@@ -57,6 +57,10 @@ SIGNATURES = [
                                             ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_gmres_precond', _I, [_P, _P, _P, _I, _D, _I, _I, ctypes.POINTER(_I), _DP,
                                      ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_gmres_dr_device', _I, [_P, _P, _P, _I, _D, _I, _I, _I, ctypes.POINTER(_I), _DP,
+                                       ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_gmres_dr', _I, [_P, _P, _P, _I, _D, _I, _I, _I, ctypes.POINTER(_I), _DP,
+                                ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_bicgstab_device', _I, [_P, _P, _P, _D, _I, _I, ctypes.POINTER(_I), _DP, _DP, _DP, _I,
                                        ctypes.POINTER(_I)]),
     ('marie_vie_bicgstab', _I, [_P, _P, _P, _D, _I, _I, ctypes.POINTER(_I), _DP, _DP, _DP, _I,
@@ -373,6 +377,34 @@ class Operator:
                                                     restart, tol, maxit, side, ctypes.byref(flag),
                                                     ctypes.byref(relres), it, resvec.ctypes.data_as(_DP),
                                                     resvec_cap, ctypes.byref(nres)))
+        return flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    def gmres_dr(self, b, x0=None, restart=50, tol=1e-6, maxit=200, ritz=5, side=PRECOND_NONE,
+                 resvec_cap=20000):
+        """Host-buffer GMRES with deflated restarts (pgmresDR); returns
+        (x, flag, relres, [inner, outer], resvec)."""
+        b = np.ascontiguousarray(np.asarray(b, np.complex128))
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(np.asarray(x0, np.complex128)).copy()
+        flag = ctypes.c_int()
+        relres = ctypes.c_double()
+        it = (ctypes.c_int * 2)()
+        resvec = np.zeros(resvec_cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_gmres_dr(self._h, _ptr(b), _ptr(x), restart, tol, maxit, ritz, side,
+                                        ctypes.byref(flag), ctypes.byref(relres), it, resvec.ctypes.data_as(_DP),
+                                        resvec_cap, ctypes.byref(nres)))
+        return x, flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
+
+    def gmres_dr_device(self, b_ptr, x_ptr, restart=50, tol=1e-6, maxit=200, ritz=5, side=PRECOND_NONE,
+                        resvec_cap=20000):
+        flag = ctypes.c_int()
+        relres = ctypes.c_double()
+        it = (ctypes.c_int * 2)()
+        resvec = np.zeros(resvec_cap)
+        nres = ctypes.c_int()
+        check(load().marie_vie_gmres_dr_device(self._h, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr), restart, tol,
+                                               maxit, ritz, side, ctypes.byref(flag), ctypes.byref(relres), it,
+                                               resvec.ctypes.data_as(_DP), resvec_cap, ctypes.byref(nres)))
         return flag.value, relres.value, [it[0], it[1]], resvec[:min(nres.value, resvec_cap)].copy()
 
     def bicgstab(self, b, x0=None, tol=1e-6, maxit=1000, precond=False):

@@ -15,6 +15,8 @@ Same names, argument meaning and return values as the reference functions
   H_field_Kop(J, fK, Gram, Hinc)                                src_vie/src_fields/src_op/H_field_Kop.m:1
   Scat_Solver(Einc, Hinc, freq, r, epsilon_r, sigma_e, rho, ...) src_vie/src_solver/Scat_Solver.m:1
   local_SAR / absorbed_power                                    src_vie/src_solver/Scat_Solver.m:179-228
+  fVIEsolver_Assembly(RHBM, freq, gpu_flag)                     src_vie/src_solver/fVIEsolver_Assembly.m:1
+  jvie2solver(Vrhs, fVIE, fVIEGPU, tol, Vini)                   src_vie/src_solver/jvie2solver.m:1
 
 Every product runs in libmarie_vie.so on the GPU. `fN` may be the host array
 returned by getOPERATORS or an already-resident `lib.Operator`; device
@@ -297,8 +299,9 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
     solver 'G': restarted GMRES (pgmres semantics), precond 'R' / 'L' with the
     preconditioner mv_AN(., 1./Mr, 1./Mr - 1) (JVIE_Solver.m:316-372);
     precision='fp32-refined' runs complex64 matvecs inside FP64 GMRES with
-    FP64 iterative refinement (no preconditioner). Any other solver letter
-    except 'D' runs BiCGStab (JVIE_Solver.m:451-492, the default), preconditioned
+    FP64 iterative refinement (no preconditioner); 'D': GMRES with deflated
+    restarts keeping `ritz` harmonic Ritz vectors (JVIE_Solver.m:378-450).
+    Any other solver letter runs BiCGStab (JVIE_Solver.m:451-492, the default), preconditioned
     when `precond` is 'L' or 'R'; iter is then in half steps / 2 and resvec absolute.
     Forms I and II (JVIE_Solver.m:176-300).
     """
@@ -307,8 +310,6 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
         tol = 1e-3
     if solver not in ('G', 'D'):         # JVIE_Solver.m:85-91: anything else is BiCGStab
         solver = 'B'
-    if solver == 'D':
-        raise NotImplementedError("solver 'D' (GMRES-DR) is not on the device path")
     use_pc = precond in ('L', 'R')       # JVIE_Solver.m:92-98
     if use_pc and precision != 'fp64':
         raise ValueError('the preconditioned solvers run in FP64')
@@ -323,9 +324,13 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
     op.set_material(Mr, Mc, Gram)
     op.set_index(idx, one_based=True)
     t0 = time.perf_counter()
-    if solver == 'G':
+    side = lib.PRECOND_NONE if not use_pc else (lib.PRECOND_RIGHT if precond == 'R' else lib.PRECOND_LEFT)
+    if solver == 'D':                    # JVIE_Solver.m:378-450
+        if precision != 'fp64':
+            raise ValueError('GMRES-DR runs in FP64')
+        x, flag, relres, it, resvec = op.gmres_dr(Vrhs, None, inner_it, tol, outer_it, ritz, side)
+    elif solver == 'G':
         if use_pc:
-            side = lib.PRECOND_RIGHT if precond == 'R' else lib.PRECOND_LEFT
             x, flag, relres, it, resvec = op.gmres_precond(Vrhs, None, inner_it, tol, outer_it, side=side)
         else:
             x, flag, relres, it, resvec = op.gmres(Vrhs, None, inner_it, tol, outer_it,
@@ -339,3 +344,83 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
     Je = np.zeros(3 * L * M * N, complex)
     Je[idx - 1] = x
     return Je.reshape(L, M, N, 3, order='F'), flag, relres, it, resvec, solvert
+
+
+def jvie2solver(Vrhs, op, tol, Vini=None, max_it=1000):
+    """Vsol of jvie2solver.m:1: BiCGStab (max_it 1000, initial guess Vini) on
+    the form II operator held by `op`; an all-zero right-hand side returns
+    itself (jvie2solver.m:30-33)."""
+    Vrhs = np.asarray(Vrhs, np.complex128)
+    if np.count_nonzero(Vrhs) == 0:
+        return Vrhs.copy()
+    x, flag, relres, it, resvec = op.bicgstab(Vrhs, Vini, tol, max_it)
+    return x
+
+
+def fVIEsolver_Assembly(RHBM, freq, gpu_flag=1, block_restart=20, block_maxit=200):
+    """[fVIEsolver, Gram, tau3, Scoord, dev] of fVIEsolver_Assembly.m:1.
+
+    RHBM: dict with 'r' (L x M x N x 3), 'epsilon_r', 'sigma_e' and optionally
+    'fN' / 'freqfN'. The body's bounding box plus one voxel is cut out
+    (fVIEsolver_Assembly.m:82-106) and its operator assembled on the device
+    when the box holds < 70 % of the grid, no fN is given or it was made at
+    another frequency (:110-124); form II material (:130-135); tau3 = j omega
+    eo Mc on the non-air voxels (:137-138). Scoord: the non-air voxel centres
+    (:71-77); they keep the column-major order of the cut-out box.
+
+    fVIEsolver(Vrhs, tol, Vini) solves A x = Vrhs on those voxels' DOFs: a
+    vector runs jvie2solver's BiCGStab; a matrix (DOFs x ports, the per-port
+    loop of VSIE_Solver.m:387-407 in one call) runs block GMRES with all
+    ports in each product. `dev` is the device-resident lib.Operator.
+    """
+    _require_gpu(gpu_flag)
+    em = EMconstants(freq)
+    r = np.asarray(RHBM['r'])
+    e_r = np.asarray(RHBM['epsilon_r']) - 1j * np.asarray(RHBM['sigma_e']) / (em['eo'] * em['omega'])
+    dx = r[1, 0, 0, 0] - r[0, 0, 0, 0]
+    Gram = dx ** 3
+    L, M, N = e_r.shape
+    ND = L * M * N
+    idxS = np.nonzero(np.abs(e_r.ravel(order='F') - 1) > 1e-12)[0]
+    Scoord = np.stack([r[..., a].ravel(order='F')[idxS] for a in range(3)], axis=1)
+    lo = [r[..., a].ravel(order='F')[idxS].min() - dx for a in range(3)]
+    hi = [r[..., a].ravel(order='F')[idxS].max() + dx for a in range(3)]
+    idxX = np.nonzero((r[:, 0, 0, 0] >= lo[0]) & (r[:, 0, 0, 0] <= hi[0]))[0]
+    idxY = np.nonzero((r[0, :, 0, 1] >= lo[1]) & (r[0, :, 0, 1] <= hi[1]))[0]
+    idxZ = np.nonzero((r[0, 0, :, 2] >= lo[2]) & (r[0, 0, :, 2] <= hi[2]))[0]
+    e_rnew = e_r[np.ix_(idxX, idxY, idxZ)]
+    fN = RHBM.get('fN')
+    if e_rnew.size < 0.7 * ND or fN is None or RHBM.get('freqfN') != freq:
+        rnew = r[np.ix_(idxX, idxY, idxZ)]
+        op = getOPERATORS(rnew, freq, 'N', return_operator=True)
+        e_r = e_rnew
+        L, M, N = e_r.shape
+        ND = L * M * N
+        idxS = np.nonzero(np.abs(e_r.ravel(order='F') - 1) > 1e-12)[0]
+    else:
+        op = fN if isinstance(fN, lib.Operator) else _operator(fN, (L, M, N))
+    idxS3 = np.concatenate([idxS, ND + idxS, 2 * ND + idxS])
+    Mr = e_r / e_r
+    Mc = (e_r - 1.0) / e_r
+    tau = 1j * em['omega'] * em['eo'] * Mc
+    tau3 = np.tile(tau.ravel(order='F')[idxS], 3)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idxS3)
+
+    def fVIEsolver(Vrhs, tol, Vini=None):
+        V = np.asarray(Vrhs, np.complex128)
+        if V.ndim == 1 or V.shape[1] == 1:
+            x = jvie2solver(V.ravel(), op, tol, None if Vini is None else np.asarray(Vini).ravel())
+            return x.reshape(V.shape)
+        X = np.zeros_like(V)
+        live = np.nonzero(np.any(V != 0, axis=0))[0]        # zero ports return zero (jvie2solver.m:30-33)
+        if live.size:
+            X0 = None if Vini is None else np.asarray(Vini, np.complex128)[:, live]
+            for c0 in range(0, live.size, 64):
+                cols = live[c0:c0 + 64]
+                Xc, flags, relres, it, rv = op.block_gmres(V[:, cols], None if X0 is None else X0[:, c0:c0 + 64],
+                                                           block_restart, tol, block_maxit)
+                X[:, cols] = Xc
+        return X
+
+    return fVIEsolver, Gram, tau3, Scoord, op

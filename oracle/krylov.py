@@ -84,6 +84,150 @@ def pgmres(A, b, restart=50, tol=1e-3, maxit=10, L1=None, L2=None, R1=None, R2=N
     return x, flag, relres, [p, outit], np.array(resvec)
 
 
+def pgmresDR_iter(A, r, m, L1, L2, R1, R2, C, tol):
+    """One cycle of pgmresDR_iter.m:22-142: Arnoldi on A (right preconditioners
+    R2 then R1, left L1 then L2), each new vector first orthogonalised against
+    the recycled basis C (B(:,k) = C^H w, :105-109) then against V by one pass
+    of classical Gram-Schmidt; the least-squares residual of [H] y = ||r|| e1
+    is the stopping test. Returns V (n x k+1), H (k+1 x k), B (kc x k), k,
+    resvec."""
+    n = r.shape[0]
+    V = np.zeros((n, m + 1), complex)
+    rnorm = np.linalg.norm(r)
+    V[:, 0] = r / rnorm
+    kc = 0 if C is None else C.shape[1]
+    B = np.zeros((kc, m), complex)
+    H = np.zeros((m + 1, m), complex)
+    resvec = []
+    for k in range(1, m + 1):
+        w = V[:, k - 1]
+        if R2 is not None:
+            w = R2(w)
+        if R1 is not None:
+            w = R1(w)
+        w = A(w)
+        if L1 is not None:
+            w = L1(w)
+        if L2 is not None:
+            w = L2(w)
+        if kc:
+            B[:, k - 1] = C.conj().T @ w
+            w = w - C @ B[:, k - 1]
+        h = V[:, :k].conj().T @ w
+        H[:k, k - 1] = h
+        w = w - V[:, :k] @ h
+        H[k, k - 1] = np.linalg.norm(w)
+        V[:, k] = w / H[k, k - 1]
+        rhs = np.zeros(k + 1, complex)
+        rhs[0] = rnorm
+        Hc = H[:k + 1, :k]
+        y = np.linalg.lstsq(Hc, rhs, rcond=None)[0]
+        resvec.append(np.linalg.norm(rhs - Hc @ y))
+        if resvec[-1] < tol:
+            return V[:, :k + 1], H[:k + 1, :k], B[:, :k], k, np.array(resvec)
+    return V, H, B, m, np.array(resvec)
+
+
+def harm_vecs1(m, k, H):
+    """getHarmVecs1.m:14-28: eigenvectors of H_m + h_{m+1,m}^2 H_m^{-H} e_m e_m^T
+    (the harmonic Ritz problem of a GMRES cycle) for the k eigenvalues of
+    smallest magnitude. Returns (vectors m x k, the k eigenvalues)."""
+    Hm = H[:m, :]
+    Z = np.linalg.solve(Hm.conj().T, np.eye(m))
+    Z[:m, :m - 1] = 0
+    Mat = Hm + H[m, m - 1] ** 2 * Z
+    vals, vecs = np.linalg.eig(Mat)
+    order = np.argsort(np.abs(vals), kind='stable')[:k]
+    return vecs[:, order], vals[order]
+
+
+def harm_vecs2(m, k, H2, V, U, C):
+    """getHarmVecs2.m:18-35: the pencil (H2^H [C^H U, 0; V^H U, I], H2^H H2)
+    whose eigenvalues are 1/theta for the harmonic Ritz values theta of A over
+    span([U V_p]); the k of largest magnitude (smallest |theta|). Returns
+    (vectors m x k, their eigenvalues)."""
+    import scipy.linalg as sl
+    Bm = H2.conj().T @ H2
+    Am = np.zeros((m + 1, m), complex)
+    Am[:k, :k] = C.conj().T @ U
+    Am[k:m + 1, :k] = V.conj().T @ U
+    Am[k:m, k:m] = np.eye(m - k)
+    Am = H2.conj().T @ Am
+    vals, vecs = sl.eig(Am, Bm)
+    order = np.argsort(np.abs(vals), kind='stable')[m - k:]
+    return vecs[:, order], vals[order]
+
+
+def pgmresDR(A, b, restart=50, tol=1e-3, maxit=10, ritz=None, L1=None, L2=None, R1=None, R2=None, x0=None):
+    """GMRES with deflated restarts as pgmresDR.m:73-282 (a GCRO-DR cycle:
+    the recycled pair U, C = A U with C orthonormal, harmonic Ritz vectors of
+    smallest magnitude kept between cycles). Returns (x, flag, relres,
+    [inner, outer], resvec), resvec relative to ||L b||.
+
+    The first cycle is plain GMRES(restart) followed by the least-squares
+    update y = H \\ (V^H r) (:175-177) and getHarmVecs1 (:180-197); later
+    cycles solve the (k + p + 1) x (k + p) system H2 = [D B; 0 H] (:223-235)
+    and refresh U, C with getHarmVecs2 (:238-250). When the first cycle stops
+    after p steps with p - 1 < ritz, the reference keeps p - 1 vectors but
+    then indexes `ritz` columns of U (:223-224), an error; here every later
+    cycle uses the number of columns U has.
+    """
+    if ritz is None:
+        ritz = min(10, round(restart / 2))
+    else:
+        ritz = min(ritz, restart - 1)
+    x0 = np.zeros_like(b, dtype=complex) if x0 is None else np.asarray(x0, complex)
+    r = b - A(x0)
+    bp = b
+    if L1 is not None:
+        r = L1(r); bp = L1(bp)
+    if L2 is not None:
+        r = L2(r); bp = L2(bp)
+    bnorm = np.linalg.norm(bp)
+    resvec = [np.linalg.norm(r) / bnorm]
+    outit = 0
+    V, H, _, p, rv = pgmresDR_iter(A, r, restart, L1, L2, R1, R2, None, tol * bnorm)
+    resvec.extend(list(rv / bnorm))
+    y = np.linalg.lstsq(H, V.conj().T @ r, rcond=None)[0]
+    x = V[:, :p] @ y
+    r = r - V @ (H @ y)
+    U = C = None
+    if resvec[-1] > tol:
+        k = p - 1 if ritz > p - 1 else ritz
+        P, _ = harm_vecs1(p, k, H)
+        U = V[:, :p] @ P
+        Q, R = np.linalg.qr(H @ P)
+        C = V @ Q
+        U = np.linalg.solve(R.T, U.T).T          # U / R
+    while resvec[-1] > tol and outit < maxit:
+        outit += 1
+        V, H, B, p, rv = pgmresDR_iter(A, r, restart, L1, L2, R1, R2, C, tol * bnorm)
+        resvec.extend(list(rv / bnorm))
+        k = U.shape[1]
+        d = np.linalg.norm(U, axis=0)
+        D = np.diag(1.0 / d)                     # pgmresDR.m:223-227
+        U = U @ D
+        H2 = np.block([[D, B], [np.zeros((H.shape[0], k)), H]])
+        CV = np.hstack([C, V])
+        y = np.linalg.lstsq(H2, CV.conj().T @ r, rcond=None)[0]
+        x = x + np.hstack([U, V[:, :p]]) @ y
+        r = r - CV @ (H2 @ y)
+        if resvec[-1] > tol:
+            P, _ = harm_vecs2(p + k, k, H2, V, U, C)
+            U = np.hstack([U, V[:, :p]]) @ P
+            Q, R = np.linalg.qr(H2 @ P)
+            C = CV @ Q
+            U = np.linalg.solve(R.T, U.T).T
+    if R2 is not None:
+        x = R2(x)
+    if R1 is not None:
+        x = R1(x)
+    x = x0 + x
+    relres = np.linalg.norm(b - A(x)) / np.linalg.norm(b)
+    flag = 0 if relres <= tol else 1
+    return x, flag, relres, [p, outit], np.array(resvec)
+
+
 def bicgstab(A, b, tol=1e-6, maxit=None, M=None, x0=None):
     """Preconditioned BiCGStab (van der Vorst 1992) with MATLAB's return
     convention (x, flag, relres, iter, resvec), the solver of

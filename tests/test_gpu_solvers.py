@@ -145,7 +145,66 @@ def test_jvie_solver_dispatch():
     fN = oop.get_operators(n, n, n, dx, f, 'N')
     ref, flag, *_ = vie.JVIE_Solver(Einc, e_r, r, f, fN, 1e-10, 1, 'G')
     assert flag == 0
-    for solver, precond in (('B', 0), ('M', 'L'), ('G', 'R'), ('G', 'L'), ('B', 'R')):
+    for solver, precond in (('B', 0), ('M', 'L'), ('G', 'R'), ('G', 'L'), ('B', 'R'), ('D', 0), ('D', 'R')):
         J, flag, relres, it, resvec, t = vie.JVIE_Solver(Einc, e_r, r, f, fN, 1e-8, 1, solver, precond)
         assert flag == 0 and relres <= 1e-8, (solver, precond)
         assert _rel(J, ref) <= 1e-6, (solver, precond)
+
+
+def _hetero_problem(n=12, dx=6.25e-3, f=298.2e6, form=2, seed=3):
+    """Two-shell sphere with seeded permittivities (forms I and II differ)."""
+    x = (np.arange(n) - (n - 1) / 2) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    r = np.stack([X, Y, Z], axis=-1)
+    R = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    rng = np.random.default_rng(seed)
+    em = fields.em_constants(f)
+    eps = np.where(R < 0.25 * n * dx, 20 + 60 * rng.random(), np.where(R < 0.45 * n * dx, 20 + 60 * rng.random(), 1.0))
+    sig = np.where(R < 0.45 * n * dx, 0.1 + 1.9 * rng.random(R.shape), 0.0)
+    e_r = eps - 1j * sig / (em['eo'] * em['omega'])
+    Einc, _ = fields.plane_wave_excitation(r, [0, 0, em['ko']], em['omega_mu'], [1, 0, 0])
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(Einc, e_r, dx, f, form=form)
+    return fN, Mr, Mc, Gram, idx, b
+
+
+@pytest.mark.parametrize("side,form", [(None, 2), ('R', 1), ('L', 1), (None, 1)])
+def test_gmres_dr_parity(side, form):
+    """GMRES with deflated restarts against the oracle's pgmresDR: same cycle
+    lengths, resvec to 1e-8 (the harmonic Ritz eigenproblems are solved by
+    different codes), solution to the tolerance."""
+    from marie_amd import lib
+    print("case gmres_dr", side, "form", form, flush=True)
+    fN, Mr, Mc, Gram, idx, b = _hetero_problem(form=form)
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx)
+    P = lambda v: mvp.mv_AN(v, fN, 1.0 / Mr, 1.0 / Mr - 1.0, Gram, 'notransp', idx)
+    kw = {} if side is None else ({'R1': P} if side == 'R' else {'L1': P})
+    xr, flr, rr, itr, rvr = krylov.pgmresDR(A, b, 10, 1e-8, 60, 4, **kw)
+    op = _op(fN, Mr, Mc, Gram, idx)
+    s = {None: lib.PRECOND_NONE, 'R': lib.PRECOND_RIGHT, 'L': lib.PRECOND_LEFT}[side]
+    x, fl, rel, it, rv = op.gmres_dr(b, restart=10, tol=1e-8, maxit=60, ritz=4, side=s)
+    assert flr == 0 and fl == 0 and rel <= 1e-8
+    assert rv.size == rvr.size and it == itr
+    assert np.allclose(rv[:11], rvr[:11], rtol=1e-10, atol=0)        # the first cycle is GMRES(10)
+    assert np.max(np.abs(rv - rvr)) <= 1e-8
+    assert _rel(x, xr) <= 1e-7
+
+
+def test_gmres_dr_arguments():
+    """ritz beyond restart - 1 is limited to it; ritz < 1 takes the default rule;
+    a first cycle that converges early needs no recycled block; zero rhs."""
+    from marie_amd import lib
+    print("case gmres_dr arguments", flush=True)
+    fN, Mr, Mc, Gram, idx, b = _hetero_problem(n=10)
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx)
+    op = _op(fN, Mr, Mc, Gram, idx)
+    for ritz, ref_ritz in ((9, 5), (0, None)):
+        xr, flr, rr, itr, rvr = krylov.pgmresDR(A, b, 6, 1e-7, 60, ref_ritz)
+        x, fl, rel, it, rv = op.gmres_dr(b, restart=6, tol=1e-7, maxit=60, ritz=ritz)
+        assert fl == flr == 0 and rv.size == rvr.size and np.max(np.abs(rv - rvr)) <= 1e-8
+    x, fl, rel, it, rv = op.gmres_dr(b, restart=60, tol=1e-3, maxit=5, ritz=5)
+    assert fl == 0 and it[1] == 0
+    x, fl, rel, it, rv = op.gmres_dr(np.zeros_like(b))
+    assert fl == 0 and np.all(x == 0)
+    with pytest.raises(lib.MarieError):
+        op.gmres_dr(b, restart=1)

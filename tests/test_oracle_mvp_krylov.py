@@ -188,3 +188,63 @@ def test_bicgstab_half_step_and_breakdown():
     A = np.array([[0.0, 1.0], [-1.0, 0.0]])
     x, flag, relres, it, resvec = krylov.bicgstab(lambda v: A @ v, np.array([1.0, 2.0]), 1e-10, 10)
     assert flag == 4 and it == 0 and relres == 1.0 and np.all(x == 0)
+
+
+def _dr_matrix(n=90, seed=1):
+    """A non-normal complex matrix with four small eigenvalues (what deflation removes)."""
+    rng = np.random.default_rng(seed)
+    Q = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+    T = np.triu(0.05 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))), 1)
+    ev = np.concatenate([[1e-3, 2e-3, 5e-3, 1e-2], 1 + rng.random(n - 4)]) * np.exp(0.3j * rng.standard_normal(n))
+    A = Q @ (np.diag(ev) + T) @ Q.conj().T
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    return A, b
+
+
+def test_pgmresdr_solves_and_first_cycle_is_gmres():
+    A, b = _dr_matrix()
+    xe = np.linalg.solve(A, b)
+    Af = lambda v: A @ v
+    x, flag, relres, it, rv = krylov.pgmresDR(Af, b, 20, 1e-10, 100, 5)
+    assert flag == 0 and relres <= 1e-10 and np.linalg.norm(x - xe) <= 1e-8 * np.linalg.norm(xe)
+    _, _, _, _, rvg = krylov.pgmres(Af, b, 20, 1e-10, 100)
+    assert np.allclose(rv[:21], rvg[:21], rtol=1e-10, atol=0)      # the first cycle is GMRES(20)
+    assert rv.size < rvg.size                                       # deflation pays
+    P = np.diag(1 / np.diag(A))
+    for kw in (dict(R1=lambda v: P @ v), dict(L1=lambda v: P @ v)):
+        x, flag, relres, it, rv = krylov.pgmresDR(Af, b, 20, 1e-10, 100, 5, **kw)
+        assert flag == 0 and np.linalg.norm(x - xe) <= 1e-8 * np.linalg.norm(xe)
+
+
+def test_harmonic_ritz_vectors_pinned():
+    """getHarmVecs1: the eigenvalues are the harmonic Ritz values of the cycle,
+    the pencil (Hbar^H Hbar, H_m^H); getHarmVecs2: 1/theta for the harmonic
+    Ritz values of the dense A over span([U V_p]), and A [U V_p] = [C V] H2."""
+    import scipy.linalg as sl
+    A, b = _dr_matrix(60, 4)
+    Af = lambda v: A @ v
+    m, k = 12, 4
+    V, H, _, p, _ = krylov.pgmresDR_iter(Af, b.astype(complex), m, None, None, None, None, None, 0.0)
+    P, vals = krylov.harm_vecs1(p, k, H)
+    theta = sl.eig(H.conj().T @ H, H[:p].conj().T, right=False)
+    ref = np.sort_complex(theta[np.argsort(np.abs(theta))[:k]])
+    assert np.allclose(np.sort_complex(vals), ref, rtol=1e-9)
+    # the first recycled pair, then one more cycle against it
+    y = np.linalg.lstsq(H, V.conj().T @ b, rcond=None)[0]
+    r = b - V @ (H @ y)
+    U = V[:, :p] @ P
+    Q, R = np.linalg.qr(H @ P)
+    C = V @ Q
+    U = np.linalg.solve(R.T, U.T).T
+    assert np.allclose(A @ U, C, atol=1e-10)
+    V, H, B, p, _ = krylov.pgmresDR_iter(Af, r, m, None, None, None, None, C, 0.0)
+    D = np.diag(1 / np.linalg.norm(U, axis=0))           # pgmresDR.m:223-227
+    U = U @ D
+    H2 = np.block([[D, B], [np.zeros((H.shape[0], k)), H]])
+    W = np.hstack([U, V[:, :p]])
+    CV = np.hstack([C, V])
+    assert np.allclose(A @ W, CV @ H2, atol=1e-9)
+    P2, vals2 = krylov.harm_vecs2(p + k, k, H2, V, U, C)
+    th = sl.eig(W.conj().T @ A.conj().T @ A @ W, W.conj().T @ A.conj().T @ W, right=False)
+    ref2 = np.sort_complex(th[np.argsort(np.abs(th))[:k]])
+    assert np.allclose(np.sort_complex(1 / vals2), ref2, rtol=1e-8)
