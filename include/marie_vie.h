@@ -289,7 +289,10 @@ int marie_vie_dgf_fields_device(const double* seg_center_dev, const void* seg_J_
  * the fastest per kernel for precision `prec`. A variant is cfg6[0..4], one
  * value per kernel (fwd_x, fwd_y, z, inv_y, inv_x): v > 0 selects the
  * LDS-stage FFT with v elements per thread, v < 0 the register-resident FFT
- * with -v points per thread (power-of-two sides); cfg6[5] is reserved and
+ * with -v points per thread (power-of-two sides); for the z kernel,
+ * v = -(100*TX + E) selects the variant with one thread group per field
+ * component (TX columns per workgroup, E points per thread, the three
+ * spectra exchanged through LDS for the tensor product); cfg6[5] is reserved and
  * returned as 0. cfg6 / best_ms5 (optional) receive the choice and its mean
  * kernel times. No counterpart in the reference (MATLAB's fftn plans
  * internally). */

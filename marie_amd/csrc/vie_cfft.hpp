@@ -11,6 +11,44 @@
 
 namespace marie {
 
+// Workgroup barrier for the LDS exchanges: waits for this wave's LDS
+// accesses (lgkmcnt) only, so global loads already issued (the next field
+// component) stay in flight across it. The kernels that use it communicate
+// between threads through LDS only. The empty asm statements keep the
+// compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);      // vmcnt, expcnt at their maximum; lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Raw buffer access to an array of at most 2^31 - 1 bytes: an offset at or
+// beyond the range reads zero without touching memory, which replaces a
+// branch around loads of padding (k >= N) or of columns past the grid.
+constexpr int kBufOob = (int)0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename T> __device__ __forceinline__ c2<T> buf_load(__amdgpu_buffer_rsrc_t rs, int off);
+template <> __device__ __forceinline__ c2<double> buf_load<double>(__amdgpu_buffer_rsrc_t rs, int off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    return c2<double>{__builtin_bit_cast(double, ((unsigned long long)(unsigned)v[1] << 32) | (unsigned)v[0]),
+                      __builtin_bit_cast(double, ((unsigned long long)(unsigned)v[3] << 32) | (unsigned)v[2])};
+}
+template <> __device__ __forceinline__ c2<float> buf_load<float>(__amdgpu_buffer_rsrc_t rs, int off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+    return c2<float>{__builtin_bit_cast(float, (unsigned)v[0]), __builtin_bit_cast(float, (unsigned)v[1])};
+}
+
+// offset `lin` when `ok`, else beyond the range (no branch)
+__device__ __forceinline__ int buf_off(int lin, bool ok) {
+    const int m = -(int)ok;
+    return (lin & m) | (kBufOob & ~m);
+}
+
 struct CPlanData {
     int n;
     int r[12];
@@ -67,7 +105,7 @@ __device__ __forceinline__ void c_bfly(c2<T> (&v)[E], const c2<T>* __restrict__ 
 template <typename T, int P, int E, int Rp, int Nsp, int R>
 __device__ __forceinline__ void c_exchange(c2<T> (&v)[E], c2<T>* lds, int sp, int off, int t) {
     constexpr int Tl = P / E;
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int q = m / Rp, r = m % Rp;
@@ -76,7 +114,7 @@ __device__ __forceinline__ void c_exchange(c2<T> (&v)[E], c2<T>* lds, int sp, in
         const int o = (jj - k) * Rp + k + r * Nsp;
         lds[o * sp + off] = v[m];
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int q = m / R, r = m % R;

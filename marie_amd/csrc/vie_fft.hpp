@@ -283,6 +283,41 @@ __device__ __forceinline__ void freq_product(const c2<T>* __restrict__ f, int64_
     }
 }
 
+// Row `a` of freq_product: y_a from x0, x1, x2 (same operations, same order,
+// for the thread group that owns component a).
+template <typename T, int PROD, bool OCT>
+__device__ __forceinline__ c2<T> freq_row(const c2<T>* __restrict__ f, int64_t fs, bool fx, bool fy, bool fz, int a,
+                                          c2<T> x0, c2<T> x1, c2<T> x2) {
+    if constexpr (PROD == 0) {
+        // rows {0,1,2}, {1,3,4}, {2,4,5} of the symmetric tensor
+        const int i0 = a == 0 ? 0 : (a == 1 ? 1 : 2);
+        const int i1 = a == 0 ? 1 : (a == 1 ? 3 : 4);
+        const int i2 = a == 0 ? 2 : (a == 1 ? 4 : 5);
+        c2<T> g0 = f[i0 * fs], g1 = f[i1 * fs], g2 = f[i2 * fs];
+        if (OCT) {
+            // component 1 (xy) flips when fx != fy, 2 (xz) when fx != fz, 4 (yz) when fy != fz
+            const bool s1 = fx != fy, s2 = fx != fz, s4 = fy != fz;
+            if ((i0 == 1 && s1) || (i0 == 2 && s2)) g0 = c2<T>{-g0.x, -g0.y};
+            if ((i1 == 1 && s1) || (i1 == 4 && s4)) g1 = c2<T>{-g1.x, -g1.y};
+            if ((i2 == 2 && s2) || (i2 == 4 && s4)) g2 = c2<T>{-g2.x, -g2.y};
+        }
+        return cadd(cadd(cmul(g0, x0), cmul(g1, x1)), cmul(g2, x2));
+    } else {
+        // y0 = k2 x1 - k1 x2, y1 = k0 x2 - k2 x0, y2 = k1 x0 - k0 x1
+        const int ip = a == 0 ? 2 : (a == 1 ? 0 : 1);      // multiplies the next component
+        const int im = a == 0 ? 1 : (a == 1 ? 2 : 0);      // multiplies the one after
+        c2<T> kp = f[ip * fs], km = f[im * fs];
+        if (OCT) {
+            const bool fl[3] = {fx, fy, fz};
+            if (fl[ip]) kp = c2<T>{-kp.x, -kp.y};
+            if (fl[im]) km = c2<T>{-km.x, -km.y};
+        }
+        const c2<T> xn = a == 0 ? x1 : (a == 1 ? x2 : x0);
+        const c2<T> xa = a == 0 ? x2 : (a == 1 ? x0 : x1);
+        return csub(cmul(kp, xn), cmul(km, xa));
+    }
+}
+
 // Dispatch order pairing mirror images: 0, 1, n-1, 2, n-2, ..., n/2 (n even).
 __device__ __forceinline__ int mirror_order2(int y, int n) {
     if (y == 0 || (n & 1)) return y;

@@ -630,7 +630,11 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     //           cfg < 0 -> register engine with -cfg points per thread (2^k sides)
     bool reg[5];
     const int Pp[5] = {Px, Py, Pz, Py, Px};
-    for (int k = 0; k < 5; ++k) reg[k] = em[k] < 0 && reg_supported<T>(Pp[k], -em[k]);
+    for (int k = 0; k < 5; ++k)
+        reg[k] = em[k] < 0 && (k == 2 ? reg_z_supported<T>(Pp[k], -em[k]) : reg_supported<T>(Pp[k], -em[k]));
+    // the register-engine z pass addresses one component through a 32-bit
+    // buffer offset: larger grids take the LDS-stage engine
+    if ((int64_t)Px * Py * N * (int64_t)sizeof(c2<T>) > (int64_t)0x7fffffff) reg[2] = false;
     const int dflt[5] = {dxy, dxy, dz, dxy, dxy};
     const AxisPlan* apl[5] = {&h->plan[0], &h->plan[1], &h->plan[2], &h->plan[1], &h->plan[0]};
     int e[5];
@@ -802,7 +806,7 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
     int st = OK;
     // candidate configurations: the LDS-stage engine's tile variants, then the
     // register engine's points-per-thread variants (power-of-two sides only)
-    int cands[8][5];
+    int cands[16][5];
     int nc = 0;
     for (int v = 0; v < nxy; ++v) {
         const int exy = vxy[v], ez = vz[v < nz ? v : nz - 1];
@@ -815,6 +819,15 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
     for (int v = 0; v < 3; ++v)
         for (int k = 0; k < 5; ++k) cands[nc + v][k] = -regE[v];
     nc += 3;
+    // z pass with one thread group per component (code 100 * TX + E)
+    const int zsplit[4] = {408, 404, 416, 808};
+    for (int v = 0; v < 4; ++v) {
+        if (!reg_z_supported<double>(h->Pz, zsplit[v]) && f64) continue;
+        if (!f64 && !reg_z_supported<float>(h->Pz, zsplit[v])) continue;
+        for (int k = 0; k < 5; ++k) cands[nc][k] = -8;
+        cands[nc][2] = -zsplit[v];
+        ++nc;
+    }
     const bool verbose = getenv("MARIE_TUNE_VERBOSE") != nullptr;
     for (int ci = 0; ci < nc && st == OK; ++ci) {
         const int cj = ci;

@@ -219,3 +219,50 @@ def test_every_tile_variant(prec, variants, shape):
         # round trip of the stored operator (octant expand or full copy)
         if prec == "f64":
             assert _rel(op.get_fN(), fN) <= 1e-15
+
+
+@pytest.mark.parametrize("prec,shape", [("f64", (16, 8, 32)), ("f32", (16, 8, 32)), ("f64", (12, 10, 16)),
+                                        ("f64", (5, 6, 64))])
+def test_z_split_variants(prec, shape):
+    """The z kernel with one thread group per component (codes 100*TX + E)
+    gives the oracle's products: N (both storage forms, both directions, two
+    right-hand sides) and K."""
+    from marie_amd import lib as L_
+    L, M, N = shape
+    print("case z split", prec, shape, flush=True)
+    p = L_.PREC_F64 if prec == "f64" else L_.PREC_F32
+    tol = TOL64 if prec == "f64" else TOL32
+    rng = np.random.default_rng(31)
+    G = rng.standard_normal((L, M, N, 6)) + 1j * rng.standard_normal((L, M, N, 6))
+    for c in range(6):
+        if oop.GP_COEFF['L'][c] < 0:
+            G[0, :, :, c] = 0
+        if oop.GP_COEFF['M'][c] < 0:
+            G[:, 0, :, c] = 0
+        if oop.GP_COEFF['N'][c] < 0:
+            G[:, :, 0, c] = 0
+    fN_sym = oop.fft_operator(oop.circulant_nop(G))
+    _, Mr, Mc, idx, x = _case(L, M, N, seed=32)
+    x2 = np.stack([x, rng.standard_normal(x.size) + 1j * rng.standard_normal(x.size)], axis=1)
+    fN_rand = _case(L, M, N, seed=33)[0]
+    codes = [408, 404, 416, 808]
+    for fN in (fN_sym, fN_rand):
+        op = _op(L, M, N, None, fN, Mr, Mc, 0.8, idx, prec=p)
+        yr = [mvp.mv_AN(x2[:, q], fN, Mr, Mc, 0.8, 'notransp', idx) for q in range(2)]
+        yt = mvp.mv_AN(x, fN, Mr, Mc, 0.8, 'transp', idx)
+        for code in codes:
+            op.set_tiles([-8, -8, -code, -8, -8, 0], p)
+            y = op.apply(x2)
+            for q in range(2):
+                assert _rel(y[:, q], yr[q]) <= tol, (code, q)
+            assert _rel(op.apply(x, L_.OP_AN, True), yt) <= tol, code
+    # K: cross product, octant storage of a physical fK
+    ko, dx = 2 * np.pi / 299792458.0 * 298e6, 6.25e-3
+    fK = oop.fft_operator(oop.circulant_kop(oop.assembly_kop(L, M, N, ko, dx)))
+    op = L_.Operator(L, M, N, precisions=p)
+    op.set_fK(fK)
+    op.set_index(idx)
+    yk = mvp.mv_K(x, fK, L, M, N, idx)
+    for code in codes:
+        op.set_tiles([-8, -8, -code, -8, -8, 0], p)
+        assert _rel(op.apply(x, L_.OP_K), yk) <= tol, ('K', code)
