@@ -43,49 +43,88 @@ template <typename TV> __device__ __forceinline__ double2 ldc(const c2<TV>* p) {
     return double2{(double)v.x, (double)v.y};
 }
 
-// partial[i * G + blk] = block sum of conj(V_i) * w over its range, i < k
-// (groups of KV basis vectors per sweep over the block's range of w)
+// partial[i * G + blk] = block sum of conj(V_i) * w over its range, i < k.
+// Grid (G, ceil(k / KV)): blockIdx.y picks a group of KV basis vectors, so the
+// groups are swept in parallel rather than one after another by each block.
 template <typename TV>
 __global__ __launch_bounds__(KT) void k_mdot(const c2<TV>* __restrict__ V, int64_t ldv, int k,
                                              const c2<TV>* __restrict__ w, int64_t n, c2<double>* __restrict__ partial,
                                              int G) {
     __shared__ double sh[KT / 64 * 2 * KV];
-    for (int i0 = 0; i0 < k; i0 += KV) {
-        const int nv = (k - i0) < KV ? (k - i0) : KV;
-        double acc[2 * KV];
+    const int i0 = blockIdx.y * KV;
+    const int nv = (k - i0) < KV ? (k - i0) : KV;
+    double acc[2 * KV];
 #pragma unroll
-        for (int q = 0; q < 2 * KV; ++q) acc[q] = 0.0;
-        const int64_t st = (int64_t)G * KT;
-        int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x;
-        // two elements per thread per trip: 2 * (nv + 1) independent loads in flight
-        for (; e + st < n; e += 2 * st) {
-            const double2 w0 = ldc(w + e), w1 = ldc(w + e + st);
+    for (int q = 0; q < 2 * KV; ++q) acc[q] = 0.0;
+    const int64_t st = (int64_t)G * KT;
+    int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x;
+    // two elements per thread per trip: 2 * (nv + 1) independent loads in flight
+    for (; e + st < n; e += 2 * st) {
+        const double2 w0 = ldc(w + e), w1 = ldc(w + e + st);
 #pragma unroll
-            for (int q = 0; q < KV; ++q) {
-                if (q < nv) {
-                    const c2<TV>* col = V + (i0 + q) * ldv + e;
-                    const double2 a = ldc(col), b = ldc(col + st);
-                    acc[2 * q] += a.x * w0.x + a.y * w0.y + b.x * w1.x + b.y * w1.y;
-                    acc[2 * q + 1] += a.x * w0.y - a.y * w0.x + b.x * w1.y - b.y * w1.x;
-                }
+        for (int q = 0; q < KV; ++q) {
+            if (q < nv) {
+                const c2<TV>* col = V + (i0 + q) * ldv + e;
+                const double2 a = ldc(col), b = ldc(col + st);
+                acc[2 * q] += a.x * w0.x + a.y * w0.y + b.x * w1.x + b.y * w1.y;
+                acc[2 * q + 1] += a.x * w0.y - a.y * w0.x + b.x * w1.y - b.y * w1.x;
             }
         }
-        if (e < n) {
-            const double2 wv = ldc(w + e);
-#pragma unroll
-            for (int q = 0; q < KV; ++q) {
-                if (q < nv) {
-                    const double2 a = ldc(V + (i0 + q) * ldv + e);
-                    acc[2 * q] += a.x * wv.x + a.y * wv.y;
-                    acc[2 * q + 1] += a.x * wv.y - a.y * wv.x;
-                }
-            }
-        }
-        block_reduce<2 * KV>(acc, sh);
-        if (threadIdx.x == 0)
-            for (int q = 0; q < nv; ++q)
-                partial[(int64_t)(i0 + q) * G + blockIdx.x] = c2<double>{acc[2 * q], acc[2 * q + 1]};
     }
+    if (e < n) {
+        const double2 wv = ldc(w + e);
+#pragma unroll
+        for (int q = 0; q < KV; ++q) {
+            if (q < nv) {
+                const double2 a = ldc(V + (i0 + q) * ldv + e);
+                acc[2 * q] += a.x * wv.x + a.y * wv.y;
+                acc[2 * q + 1] += a.x * wv.y - a.y * wv.x;
+            }
+        }
+    }
+    block_reduce<2 * KV>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < nv; ++q)
+            partial[(int64_t)(i0 + q) * G + blockIdx.x] = c2<double>{acc[2 * q], acc[2 * q + 1]};
+}
+
+// k_mdot for a complex64 basis with an even column stride: each thread reads
+// two adjacent elements of w and of every basis vector with one 16-byte load
+// (the last element of an odd length is added by thread 0 of block 0)
+__global__ __launch_bounds__(KT) void k_mdot_f2(const c2<float>* __restrict__ V, int64_t ldv, int k,
+                                                const c2<float>* __restrict__ w, int64_t n,
+                                                c2<double>* __restrict__ partial, int G) {
+    __shared__ double sh[KT / 64 * 2 * KV];
+    const int64_t n2 = n >> 1;
+    const float4* w4 = reinterpret_cast<const float4*>(w);
+    const int i0 = blockIdx.y * KV;
+    const int nv = (k - i0) < KV ? (k - i0) : KV;
+    double acc[2 * KV];
+#pragma unroll
+    for (int q = 0; q < 2 * KV; ++q) acc[q] = 0.0;
+    for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n2; e += (int64_t)G * KT) {
+        const float4 wv = w4[e];
+#pragma unroll
+        for (int q = 0; q < KV; ++q) {
+            if (q < nv) {
+                const float4 a = reinterpret_cast<const float4*>(V + (i0 + q) * ldv)[e];
+                acc[2 * q] += (double)a.x * wv.x + (double)a.y * wv.y + (double)a.z * wv.z + (double)a.w * wv.w;
+                acc[2 * q + 1] += (double)a.x * wv.y - (double)a.y * wv.x + (double)a.z * wv.w - (double)a.w * wv.z;
+            }
+        }
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const c2<float> wv = w[n - 1];
+        for (int q = 0; q < nv; ++q) {
+            const c2<float> a = V[(i0 + q) * ldv + n - 1];
+            acc[2 * q] += (double)a.x * wv.x + (double)a.y * wv.y;
+            acc[2 * q + 1] += (double)a.x * wv.y - (double)a.y * wv.x;
+        }
+    }
+    block_reduce<2 * KV>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < nv; ++q)
+            partial[(int64_t)(i0 + q) * G + blockIdx.x] = c2<double>{acc[2 * q], acc[2 * q + 1]};
 }
 
 // out[i] = sum_b partial[i*G + b], fixed order; one block per i
@@ -221,6 +260,7 @@ struct Ctx {
     int G;
     int transp = 0;              // 1: solve with A^H (mv_AN's 'transp' branch)
     int precond = 0;             // 0 none, 1 right (A P), 2 left (P A): pgmres R1 / L1
+    int64_t ldv = 0;             // column stride of the basis V (n, or n rounded up to even in mode 2)
     Ws ws;
 };
 
@@ -305,18 +345,28 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<TV>* r, int m, double abstol, b
     // Arnoldi step k (1-based): w = A V(:,k), Hessenberg column k-1 on the
     // device, V(:,k+1) = w / H(k+1,k), column copied to pinned memory + event
     auto enqueue = [&](int k) -> int {
-        int st2 = cycle_matvec<TV>(c, V + (int64_t)(k - 1) * n, w, f32op);
+        int st2 = cycle_matvec<TV>(c, V + (int64_t)(k - 1) * c.ldv, w, f32op);
         if (st2) return st2;
         c2<double>* hc = c.ws.h + (size_t)(k - 1) * ld;
         // h = V(:,1:k)^H w  (classical Gram-Schmidt, pgmres_iter.m:106)
-        hipLaunchKernelGGL(k_mdot<TV>, dim3(c.G), dim3(KT), 0, c.s, V, n, k, w, n, c.ws.part, c.G);
+        if constexpr (sizeof(TV) == 4) {
+            if ((c.ldv & 1) == 0)
+                hipLaunchKernelGGL(k_mdot_f2, dim3(c.G, (k + KV - 1) / KV), dim3(KT), 0, c.s, V, c.ldv, k, w, n, c.ws.part,
+                                   c.G);
+            else
+                hipLaunchKernelGGL(k_mdot<TV>, dim3(c.G, (k + KV - 1) / KV), dim3(KT), 0, c.s, V, c.ldv, k, w, n, c.ws.part,
+                                   c.G);
+        } else {
+            hipLaunchKernelGGL(k_mdot<TV>, dim3(c.G, (k + KV - 1) / KV), dim3(KT), 0, c.s, V, c.ldv, k, w, n, c.ws.part,
+                                   c.G);
+        }
         hipLaunchKernelGGL(k_reduce, dim3(k), dim3(KT), 0, c.s, c.ws.part, c.G, hc);
         // w = w - V h with ||w||^2 partials (pgmres_iter.m:107, :109)
-        hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, w, 1.0, V, n, k, hc, -1.0, n,
+        hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, w, 1.0, V, c.ldv, k, hc, -1.0, n,
                            c.ws.npart, c.G);
         hipLaunchKernelGGL(k_norm_entry, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, hc + k);
         // V(:,k+1) = w / H(k+1,k)  (pgmres_iter.m:110)
-        hipLaunchKernelGGL(k_scale_by_entry<TV>, dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)k * n, w, hc + k, n);
+        hipLaunchKernelGGL(k_scale_by_entry<TV>, dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)k * c.ldv, w, hc + k, n);
         MARIE_HIP(hipGetLastError());
         MARIE_HIP(hipMemcpyAsync(c.ws.hp + (size_t)(k - 1) * ld, hc, (k + 1) * sizeof(c2<double>),
                                  hipMemcpyDeviceToHost, c.s));
@@ -402,17 +452,17 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<TV>* r, int m, double abstol, b
     MARIE_HIP(hipMemcpyAsync(dr, cr, (k + 1) * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
     if (c.precond == 1) {
         // right preconditioning: x += P (V y)  (pgmres_iter.m:131-145)
-        hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, c.ws.t, 0.0, V, n, k, dy, 1.0, n,
+        hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, c.ws.t, 0.0, V, c.ldv, k, dy, 1.0, n,
                            (double*)nullptr, c.G);
         int st2 = apply(c.h, OP_P, 0, PREC_F64, c.ws.t, c.ws.pt, 1, c.s);
         if (st2) return st2;
         hipLaunchKernelGGL((k_axpby<double, double, double>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, x, 1.0, c.ws.pt,
                            n);
     } else {
-        hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, V, n, k, dy, 1.0, n,
+        hipLaunchKernelGGL((k_vcomb<double, TV>), dim3(c.G), dim3(KT), 0, c.s, x, 1.0, V, c.ldv, k, dy, 1.0, n,
                            (double*)nullptr, c.G);
     }
-    hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, r, 0.0, V, n, k + 1, dr, 1.0, n,
+    hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, r, 0.0, V, c.ldv, k + 1, dr, 1.0, n,
                        (double*)nullptr, c.G);
     MARIE_HIP(hipGetLastError());
     MARIE_HIP(hipStreamSynchronize(c.s));
@@ -432,7 +482,10 @@ static int alloc_ws(Ctx& c, int m, int mode) {
         }
         return OK;
     };
-    int st = A(&c.ws.V, (size_t)(m + 1) * n * ev);
+    // complex64 basis columns start on 16-byte boundaries (even stride), so the
+    // multi-dot reads two elements per 16-byte load
+    c.ldv = mode == 2 ? ((n + 1) & ~(int64_t)1) : n;
+    int st = A(&c.ws.V, (size_t)(m + 1) * c.ldv * ev);
     if (!st) st = A(&c.ws.w, n * ev);
     if (!st && mode != 0) st = A(&c.ws.rT, n * ev);
     if (!st) st = A((void**)&c.ws.r, n * sizeof(c2<double>));
@@ -678,7 +731,8 @@ static int mcomb(Ctx& c, DrWs& d, const c2<double>* In, int nin, const Mat& coef
 
 // g = [cols]^H v (ncols entries) to the host
 static int multidot(Ctx& c, const c2<double>* cols, int ncols, const c2<double>* v, std::vector<cd>& g) {
-    hipLaunchKernelGGL(k_mdot<double>, dim3(c.G), dim3(KT), 0, c.s, cols, c.n, ncols, v, c.n, c.ws.part, c.G);
+    hipLaunchKernelGGL(k_mdot<double>, dim3(c.G, (ncols + KV - 1) / KV), dim3(KT), 0, c.s, cols, c.n, ncols, v, c.n,
+                       c.ws.part, c.G);
     hipLaunchKernelGGL(k_reduce, dim3(ncols), dim3(KT), 0, c.s, c.ws.part, c.G, c.ws.h);
     MARIE_HIP(hipGetLastError());
     MARIE_HIP(hipMemcpyAsync(c.ws.hp, c.ws.h, ncols * sizeof(c2<double>), hipMemcpyDeviceToHost, c.s));
@@ -722,7 +776,8 @@ static int dr_cycle(Ctx& c, c2<double>* Cb, int k, c2<double>* V, const c2<doubl
         st = cycle_matvec<double>(c, V + (int64_t)(j - 1) * n, w, false);
         if (st) return st;
         const int nc = k + j;                 // [C V_1..j]
-        hipLaunchKernelGGL(k_mdot<double>, dim3(c.G), dim3(KT), 0, c.s, Cb, n, nc, w, n, c.ws.part, c.G);
+        hipLaunchKernelGGL(k_mdot<double>, dim3(c.G, (nc + KV - 1) / KV), dim3(KT), 0, c.s, Cb, n, nc, w, n, c.ws.part,
+                           c.G);
         hipLaunchKernelGGL(k_reduce, dim3(nc), dim3(KT), 0, c.s, c.ws.part, c.G, c.ws.h);
         hipLaunchKernelGGL((k_vcomb<double, double>), dim3(c.G), dim3(KT), 0, c.s, w, 1.0, Cb, n, nc, c.ws.h, -1.0,
                            n, c.ws.npart, c.G);
@@ -785,6 +840,7 @@ int gmres_dr(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int m
     c.s = h->stream;
     c.n = h->nidx;
     c.G = grid_for(c.n);
+    c.ldv = c.n;
     const int64_t n = c.n;
     // pgmresDR.m:112-116
     ritz = ritz < 1 ? std::min(10, (m + 1) / 2) : std::min(ritz, m - 1);
