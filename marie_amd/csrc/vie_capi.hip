@@ -124,6 +124,9 @@ int marie_vie_destroy(marie_vie_op* h) {
         if (b) (void)hipFree(b);
     for (int k = 0; k < 8; ++k)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+    for (auto& row : h->ring)
+        for (hipEvent_t e : row)
+            if (e) (void)hipEventDestroy(e);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return MARIE_OK;
@@ -361,6 +364,8 @@ int marie_vie_apply(marie_vie_op* h, int op, int transp, const double* x, double
 
 int marie_vie_enable_timing(marie_vie_op* h, int on) {
     CHECK_H(h);
+    MARIE_HIP(hipStreamSynchronize(h->stream));   // recorded rows are dropped
+    h->ring_used = 0;
     h->timing = on != 0;
     for (int k = 0; k < 8; ++k) h->kernel_ms[k] = 0.0;
     h->kernel_calls = 0;
@@ -369,6 +374,8 @@ int marie_vie_enable_timing(marie_vie_op* h, int on) {
 
 int marie_vie_get_timing(marie_vie_op* h, double* mean_ms5, int64_t* calls) {
     CHECK_H(h);
+    const int st = timing_flush(h);
+    if (st) return st;
     for (int k = 0; k < 5; ++k) mean_ms5[k] = h->kernel_calls ? h->kernel_ms[k] / h->kernel_calls : 0.0;
     if (calls) *calls = h->kernel_calls;
     return MARIE_OK;

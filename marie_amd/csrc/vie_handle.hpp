@@ -83,9 +83,15 @@ struct Op {
     int emax64[5] = {16, 16, 8, 16, 16};
     int emax32[5] = {32, 32, 16, 32, 32};
 
-    // optional per-kernel timing with HIP events on the handle's stream
+    // optional per-kernel timing with HIP events on the handle's stream: each
+    // product records 6 events into the next row of a ring, read back (with
+    // one synchronisation) when the ring is full or the timings are asked
+    // for, so timed products still run back to back
     bool timing = false;
     hipEvent_t ev[8] = {};
+    static constexpr int kTimingRing = 64;
+    hipEvent_t ring[kTimingRing][6] = {};
+    int ring_used = 0;
     double kernel_ms[8] = {};
     int64_t kernel_calls = 0;
 
@@ -102,5 +108,6 @@ int store_circ(Op* h, int kind, void* full64);   // kind 0: fN, 1: fK
 int load_circ(Op* h, int kind, void* out64);
 int assemble_K(Op* h, double dx, double ko);
 int autotune(Op* h, int prec, int iters, double* best_ms5);
+int timing_flush(Op* h);   // folds the recorded ring rows into kernel_ms
 
 }  // namespace marie

@@ -647,7 +647,14 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     }
     const int64_t nline = (int64_t)M * N;
     const bool tm = h->timing;
-    if (tm) MARIE_HIP(hipEventRecord(h->ev[0], s));
+    hipEvent_t* tev = h->ring[h->ring_used];
+    if (tm) {
+        // timing-only events: no system-scope fence (cache writeback and
+        // invalidation) between the kernels they bracket
+        for (int k = 0; k < 6; ++k)
+            if (!tev[k]) MARIE_HIP(hipEventCreateWithFlags(&tev[k], hipEventDisableSystemFence));
+        MARIE_HIP(hipEventRecord(tev[0], s));
+    }
     const int conj_in = (op == OP_AN && transp) ? 1 : 0;
 
     // forward x
@@ -661,7 +668,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
                    s, x, A, h->pos, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, conj_in)));
     }
     MARIE_HIP(hipGetLastError());
-    if (tm) MARIE_HIP(hipEventRecord(h->ev[1], s));
+    if (tm) MARIE_HIP(hipEventRecord(tev[1], s));
     // forward y: A[c][k][j<M][i] -> B[c][k][j<Py][i]
     if (reg[1]) {
         reg_strided<T>(false, Py, -em[1], s, A, B, twy, Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
@@ -675,7 +682,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
                    (int64_t)Py * Px, M, Py, lg, (int64_t)N * M * Px, (int64_t)N * Py * Px)));
     }
     MARIE_HIP(hipGetLastError());
-    if (tm) MARIE_HIP(hipEventRecord(h->ev[2], s));
+    if (tm) MARIE_HIP(hipEventRecord(tev[2], s));
     // z: fused forward, tensor product, inverse
     if (reg[2]) {
         reg_z<T>(oct, prod, Pz, -em[2], s, B, fN, twz, Px, Py, N, nrhs);
@@ -697,7 +704,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
                        dim3(NT), sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
     }
     MARIE_HIP(hipGetLastError());
-    if (tm) MARIE_HIP(hipEventRecord(h->ev[3], s));
+    if (tm) MARIE_HIP(hipEventRecord(tev[3], s));
     // inverse y: B[c][k][j<Py][i] -> A[c][k][j<M][i]
     if (reg[3]) {
         reg_strided<T>(true, Py, -em[3], s, B, A, twy, Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
@@ -711,7 +718,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
                    (int64_t)M * Px, Py, M, lg, (int64_t)N * Py * Px, (int64_t)N * M * Px)));
     }
     MARIE_HIP(hipGetLastError());
-    if (tm) MARIE_HIP(hipEventRecord(h->ev[4], s));
+    if (tm) MARIE_HIP(hipEventRecord(tev[4], s));
     // inverse x + crop + epilogue
     {
         const T scale = T(1.0 / ((double)Px * (double)Py * (double)Pz));
@@ -729,15 +736,23 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
         MARIE_HIP(hipGetLastError());
     }
     if (tm) {
-        MARIE_HIP(hipEventRecord(h->ev[5], s));
-        MARIE_HIP(hipEventSynchronize(h->ev[5]));
+        MARIE_HIP(hipEventRecord(tev[5], s));
+        if (++h->ring_used == Op::kTimingRing) return timing_flush(h);
+    }
+    return OK;
+}
+
+int timing_flush(Op* h) {
+    if (h->ring_used == 0) return OK;
+    MARIE_HIP(hipEventSynchronize(h->ring[h->ring_used - 1][5]));
+    for (int r = 0; r < h->ring_used; ++r)
         for (int k = 0; k < 5; ++k) {
             float ms = 0.f;
-            MARIE_HIP(hipEventElapsedTime(&ms, h->ev[k], h->ev[k + 1]));
+            MARIE_HIP(hipEventElapsedTime(&ms, h->ring[r][k], h->ring[r][k + 1]));
             h->kernel_ms[k] += ms;
         }
-        h->kernel_calls += 1;
-    }
+    h->kernel_calls += h->ring_used;
+    h->ring_used = 0;
     return OK;
 }
 
@@ -839,6 +854,7 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
         for (int k = 0; k < 8; ++k) h->kernel_ms[k] = 0.0;
         h->kernel_calls = 0;
         for (int it = 0; it < iters && st == OK; ++it) st = apply(h, OP_AN, 0, prec, x, y, 1, h->stream);
+        if (st == OK) st = timing_flush(h);
         for (int k = 0; k < 5 && st == OK; ++k) {
             const double ms = h->kernel_ms[k] / (h->kernel_calls ? h->kernel_calls : 1);
             if (ms < best[k]) {
