@@ -63,6 +63,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-block', action='store_true', help='skip the 16-excitation block solve')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the adjoint gradient timing')
+    ap.add_argument('--no-batch', action='store_true',
+                    help='skip the several-bodies batch (4 phantoms x 2 frequencies x 16 coil ports)')
     ap.add_argument('--profile-only', action='store_true',
                     help='assemble and run the timed FP64 matvecs only (for rocprofv3 runs)')
     ap.add_argument('--profile-solve', action='store_true',
@@ -285,6 +287,68 @@ def main():
             log('block gmres mode %d: %.3f s for %d excitations (sequential FP64 %.3f s), cycles %d, max relres %.2e'
                 % (mode, tb, s_blk, t_seq, it[1], rr.max()))
         out['batching_16_excitations'] = blk
+
+    if not args.no_solve and not args.no_batch and n1 == 128:
+        # several uneven bodies and frequencies: 64^3, 80^3, 96^3, 128^3 phantoms
+        # over the same 0.192 m domain (seeds 1-4) at 128 and 298 MHz, each with
+        # the 16 ports of a coil of 0.1 x 0.05 m loops in two rings of 8 on a
+        # cylinder of radius 0.15 m and length 0.2 m; each problem is one
+        # handle (device assembly, own autotuned tiles) solved by block GMRES,
+        # first one after the other, then all at once (marie_vie_block_gmres_batch)
+        from marie_amd import coil
+        ports = coil.cylinder_loop_array(16, 0.15, 0.1, 0.05, seg_per_side=8, rings=2, length=0.2)
+        probs = []
+        t = time.perf_counter()
+        for nb, seed in ((64, 1), (80, 2), (96, 3), (128, 4)):
+            dxb = 0.192 / nb
+            rb, epsb, sigb, _ = geometry.head_phantom(nb, nb, nb, dxb, seed=seed, scale=0.85)
+            for fb in (128e6, 298e6):
+                emb = vie.EMconstants(fb)
+                e_rb = epsb - 1j * sigb / (emb['eo'] * emb['omega'])
+                Es, _ = coil.CCL_Excitation(rb, emb['ko'], 1.0, ports)
+                Mrb, Mcb, Gb, idxb, v0 = vie.jvie_system(Es[0], e_rb, rb, fb, form=2)
+                # form II right-hand side Gram * j w eo (e_r - 1) / e_r * Einc on the
+                # DOFs (JVIE_Solver.m:151-156, :242-244), the same factor for every port
+                fac = np.tile((Gb * 1j * emb['omega'] * emb['eo'] * (e_rb - 1) / e_rb).ravel(order='F'), 3)[idxb - 1]
+                cols = [fac * Ep.ravel(order='F')[idxb - 1] for Ep in Es]
+                assert np.allclose(cols[0], v0, rtol=1e-12, atol=0)
+                opb = lib.Operator(nb, nb, nb)
+                opb.assemble_N(dxb, emb['ko'])
+                opb.set_material(Mrb, Mcb, Gb)
+                opb.set_index(idxb, one_based=True)
+                opb.autotune(lib.PREC_F64, iters=3)
+                Bb = torch.from_numpy(np.ascontiguousarray(np.stack(cols))).to(dev)
+                probs.append((nb, fb, opb, Bb))
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t
+        kw = dict(restart=20, tol=1e-6, maxit=100, mode=0, inner_tol=1e-7, use_graph=True)
+        seq = []
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for nb, fb, opb, Bb in probs:
+            Xb = torch.zeros_like(Bb)
+            t1 = time.perf_counter()
+            fl, rr, it, rv = opb.block_gmres_device(Bb.data_ptr(), Xb.data_ptr(), 16, **kw)
+            torch.cuda.synchronize()
+            seq.append({'voxels': nb, 'MHz': fb / 1e6, 'dof': int(Bb.shape[1]), 's': time.perf_counter() - t1,
+                        'cycles': it[1], 'max_relres': float(rr.max()), 'flags_max': int(fl.max())})
+        t_seq = time.perf_counter() - t
+        Xs = [torch.zeros_like(p[3]) for p in probs]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fls, rrs, its = lib.block_gmres_batch([p[2] for p in probs], [p[3].data_ptr() for p in probs],
+                                              [x.data_ptr() for x in Xs], [16] * len(probs), **kw)
+        torch.cuda.synchronize()
+        t_bat = time.perf_counter() - t
+        out['batching_bodies_frequencies'] = {
+            'problems': seq, 'ports_per_problem': 16, 'setup_s': t_setup, 'sequential_block_s': t_seq,
+            'concurrent_batch_s': t_bat, 'speedup': t_seq / t_bat,
+            'batch_max_relres': float(max(r.max() for r in rrs)), 'batch_flags_max': int(max(f.max() for f in fls))}
+        log('batch of %d problems x 16 ports: one after the other %.3f s, concurrent %.3f s (max relres %.2e; '
+            'setup %.1f s)' % (len(probs), t_seq, t_bat, max(r.max() for r in rrs), t_setup))
+        for p in probs:
+            p[2].close()
+        del probs, Xs
 
     if not args.no_solve and not args.no_adjoint:
         from marie_amd import adjoint

@@ -266,6 +266,21 @@ int marie_vie_block_gmres(marie_vie_op* h, const double* B, double* X, int nrhs,
                           int maxit, int mode, double inner_tol, int options, int* flag, double* relres,
                           int* iter, double* resvec, int resvec_cap, int* nres);
 
+/* Several problems (bodies, frequencies: separate handles), each with its
+ * block of right-hand sides, solved concurrently: one host worker per
+ * problem (at most 8 at a time, largest first) runs
+ * marie_vie_block_gmres_device on that handle, whose stream is its own, so
+ * the products and Gram-Schmidt kernels of different problems overlap on
+ * the device. ops[i], B[i], X[i], nrhs[i] as the single-problem entry;
+ * restart, tol, maxit, mode, inner_tol, options shared. flag and relres
+ * hold sum(nrhs) entries in problem order, iter 2 per problem, status the
+ * return code of each problem. The batch form of fVIEsolver_Assembly's
+ * per-port solves over several bodies (VSIE_Solver.m:387-407 per body). */
+int marie_vie_block_gmres_batch(marie_vie_op* const* ops, int nitems, const void* const* B_dev,
+                                void* const* X_dev, const int* nrhs, int restart, double tol, int maxit,
+                                int mode, double inner_tol, int options, int* flag, double* relres,
+                                int* iter, int* status);
+
 /* Incident E and H of current elements at observation points (eval_DGF.m:37-106
  * as called by CCL_Excitation.m:41-70 and Dipole_Excitation.m): segment
  * midpoints seg_center (nseg x 3, row per segment), current elements seg_J

@@ -31,16 +31,26 @@ def rect_loop(center, tangent, axis, width, height, seg_per_side=8):
     return np.array(rows)
 
 
-def cylinder_loop_array(n_ports=16, radius=0.15, width=0.1, height=0.05, z_center=0.0, seg_per_side=8):
-    """One rectangular loop per port, centred on a cylinder of `radius` about
-    the z axis at angles 2 pi p / n_ports: `width` along z, `height` along the
-    circumference (the chord of the cylinder). Returns a list of CCL arrays."""
+def cylinder_loop_array(n_ports=16, radius=0.15, width=0.1, height=0.05, z_center=0.0, seg_per_side=8, rings=1,
+                        length=None):
+    """Rectangular loops on a cylinder of `radius` about the z axis: `rings`
+    rings of n_ports / rings loops each, at angles 2 pi p / (n_ports / rings)
+    and, for several rings, z centres spread evenly over `length` (default
+    rings * width) about z_center; `width` along z, `height` along the
+    circumference (the chord of the cylinder). Returns a list of CCL arrays,
+    ring by ring."""
+    if n_ports % rings:
+        raise ValueError('n_ports must be a multiple of rings')
+    per = n_ports // rings
+    length = rings * width if length is None else length
     ports = []
-    for p in range(n_ports):
-        phi = 2 * np.pi * p / n_ports
-        center = [radius * np.cos(phi), radius * np.sin(phi), z_center]
-        tangent = [-np.sin(phi), np.cos(phi), 0.0]
-        ports.append(rect_loop(center, [0.0, 0.0, 1.0], tangent, width, height, seg_per_side))
+    for q in range(rings):
+        zc = z_center + (q - (rings - 1) / 2) * (length / rings)
+        for p in range(per):
+            phi = 2 * np.pi * p / per
+            center = [radius * np.cos(phi), radius * np.sin(phi), zc]
+            tangent = [-np.sin(phi), np.cos(phi), 0.0]
+            ports.append(rect_loop(center, [0.0, 0.0, 1.0], tangent, width, height, seg_per_side))
     return ports
 
 

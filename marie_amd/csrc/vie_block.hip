@@ -30,6 +30,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
+#include <string>
+#include <thread>
 #include <vector>
 
 namespace marie {
@@ -794,4 +798,58 @@ extern "C" int marie_vie_block_gmres(marie_vie_op* hh, const double* B, double* 
     if (db) (void)hipFree(db);
     if (dx) (void)hipFree(dx);
     return st;
+}
+
+extern "C" int marie_vie_block_gmres_batch(marie_vie_op* const* ops, int nitems, const void* const* B, void* const* X,
+                                           const int* nrhs, int restart, double tol, int maxit, int mode,
+                                           double inner_tol, int options, int* flag, double* relres, int* iter,
+                                           int* status) {
+    if (!ops || !B || !X || !nrhs || !flag || !relres || !iter || !status || nitems < 1 || nitems > 256) {
+        set_error("invalid batch arguments (1..256 problems)");
+        return MARIE_ERR_ARG;
+    }
+    std::vector<int64_t> off(nitems + 1, 0);
+    for (int i = 0; i < nitems; ++i) {
+        if (!ops[i] || nrhs[i] < 1) {
+            set_error("batch problem " + std::to_string(i) + ": null handle or no right-hand side");
+            return MARIE_ERR_ARG;
+        }
+        for (int j = 0; j < i; ++j)
+            if (ops[j] == ops[i]) {
+                set_error("batch problems " + std::to_string(j) + " and " + std::to_string(i) +
+                          " share a handle (each needs its own stream and work buffers)");
+                return MARIE_ERR_ARG;
+            }
+        off[i + 1] = off[i] + nrhs[i];
+    }
+    // largest problems first, so the small ones fill in around them
+    std::vector<int> order(nitems);
+    for (int i = 0; i < nitems; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const Op* ha = reinterpret_cast<const Op*>(ops[a]);
+        const Op* hb = reinterpret_cast<const Op*>(ops[b]);
+        return ha->nidx * nrhs[a] > hb->nidx * nrhs[b];
+    });
+    std::vector<std::string> msg(nitems);
+    std::atomic<int> next(0);
+    auto worker = [&]() {
+        for (int t = next++; t < nitems; t = next++) {
+            const int i = order[t];
+            status[i] = marie_vie_block_gmres_device(ops[i], B[i], X[i], nrhs[i], restart, tol, maxit, mode, inner_tol,
+                                                     options, flag + off[i], relres + off[i], iter + 2 * i, nullptr,
+                                                     0, nullptr);
+            if (status[i] != MARIE_OK) msg[i] = get_error();
+        }
+    };
+    const int nth = nitems < 8 ? nitems : 8;
+    std::vector<std::thread> th;
+    for (int k = 1; k < nth; ++k) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+    for (int i = 0; i < nitems; ++i)
+        if (status[i] != MARIE_OK) {
+            set_error("batch problem " + std::to_string(i) + ": " + msg[i]);
+            return status[i];
+        }
+    return MARIE_OK;
 }

@@ -70,6 +70,9 @@ SIGNATURES = [
                                           ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_block_gmres', _I, [_P, _P, _P, _I, _I, _D, _I, _I, _D, _I, ctypes.POINTER(_I), _DP,
                                    ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
+    ('marie_vie_block_gmres_batch', _I, [ctypes.POINTER(_P), _I, ctypes.POINTER(_P), ctypes.POINTER(_P),
+                                         ctypes.POINTER(_I), _I, _D, _I, _I, _D, _I, ctypes.POINTER(_I), _DP,
+                                         ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     ('marie_vie_dgf_fields', _I, [_P, _P, _P, _I, _P, _I64, _D, _D, _P, _P, _I]),
     ('marie_vie_dgf_fields_device', _I, [_P, _P, _P, _I, _P, _I64, _D, _D, _P, _P, _P]),
     ('marie_vie_autotune', _I, [_P, _I, _I, ctypes.POINTER(_I), _DP]),
@@ -155,6 +158,33 @@ def _block_options(use_graph, cgs_passes):
     if cgs_passes not in (1, 2):
         raise ValueError('cgs_passes must be 1 or 2')
     return (BLOCK_GRAPH if use_graph else 0) | (BLOCK_CGS2 if cgs_passes == 2 else 0)
+
+
+def block_gmres_batch(ops, B_ptrs, X_ptrs, nrhs, restart=20, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4,
+                      use_graph=False, cgs_passes=1):
+    """Block GMRES of several problems at once (one handle each, device
+    buffers B_ptrs[i], X_ptrs[i] of nrhs[i] stacked columns). Returns
+    (flags per problem, relres per problem, [steps, cycles] per problem)."""
+    n = len(ops)
+    tot = int(sum(nrhs))
+    P = ctypes.c_void_p
+    opa = (P * n)(*[o._h.value for o in ops])
+    ba = (P * n)(*B_ptrs)
+    xa = (P * n)(*X_ptrs)
+    na = (ctypes.c_int * n)(*nrhs)
+    flag = (ctypes.c_int * tot)()
+    relres = (ctypes.c_double * tot)()
+    it = (ctypes.c_int * (2 * n))()
+    status = (ctypes.c_int * n)()
+    check(load().marie_vie_block_gmres_batch(opa, n, ba, xa, na, restart, tol, maxit, mode, inner_tol,
+                                             _block_options(use_graph, cgs_passes), flag, relres, it, status))
+    fl, rr, its, o = [], [], [], 0
+    for i in range(n):
+        fl.append(np.array(flag[o:o + nrhs[i]]))
+        rr.append(np.array(relres[o:o + nrhs[i]]))
+        its.append([it[2 * i], it[2 * i + 1]])
+        o += nrhs[i]
+    return fl, rr, its
 
 
 class Operator:

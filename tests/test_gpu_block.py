@@ -141,3 +141,42 @@ def test_block_zero_column_rejected(problem):
     B[:, 1] = 0
     with pytest.raises(lib.MarieError):
         op.block_gmres(B)
+
+
+def test_block_batch_of_uneven_problems():
+    """Several bodies / frequencies (separate handles, uneven sizes) solved
+    concurrently give each problem's own block solution; a handle used twice
+    is rejected."""
+    import torch
+    from marie_amd import lib
+    print("case block batch", flush=True)
+    probs = [_problem(10, 6.25e-3, 298.2e6), _problem(14, 5e-3, 128e6, eps=40.0, sig=0.3),
+             _problem(12, 6.25e-3, 200e6, eps=60.0, sig=0.8)]
+    ops, Bs, Xs, nrhs, refs = [], [], [], [], []
+    for fN, Mr, Mc, Gram, idx, B in probs:
+        n = Mr.shape[0]
+        op = lib.Operator(n, n, n)
+        op.set_fN(fN)
+        op.set_material(Mr, Mc, Gram)
+        op.set_index(idx)
+        ops.append(op)
+        Bd = torch.from_numpy(np.ascontiguousarray(B.T)).to('cuda:0')
+        Bs.append(Bd)
+        Xs.append(torch.zeros_like(Bd))
+        nrhs.append(B.shape[1])
+        Xr, fl, rr, it, rv = op.block_gmres(B, restart=8, tol=1e-9, maxit=100)
+        assert np.all(fl == 0)
+        refs.append(Xr)
+    torch.cuda.synchronize()
+    flags, relres, iters = lib.block_gmres_batch(ops, [b.data_ptr() for b in Bs], [x.data_ptr() for x in Xs], nrhs,
+                                                 restart=8, tol=1e-9, maxit=100)
+    for i in range(len(probs)):
+        assert np.all(flags[i] == 0) and np.all(relres[i] <= 1e-9)
+        X = Xs[i].cpu().numpy().T
+        assert _rel(X, refs[i]) <= 1e-12          # the same deterministic solve
+        fN, Mr, Mc, Gram, idx, B = probs[i]
+        for q in range(B.shape[1]):
+            r = B[:, q] - mvp.mv_AN(X[:, q], fN, Mr, Mc, Gram, 'notransp', idx)
+            assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[:, q])
+    with pytest.raises(lib.MarieError):
+        lib.block_gmres_batch([ops[0], ops[0]], [Bs[0].data_ptr()] * 2, [Xs[0].data_ptr()] * 2, [nrhs[0]] * 2)
