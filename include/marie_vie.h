@@ -313,7 +313,10 @@ int marie_vie_dgf_fields_device(const double* seg_center_dev, const void* seg_J_
  * internally). */
 int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* cfg6, double* best_ms5);
 
-/* Set the kernel variants explicitly (cfg6 as above). */
+/* Set the kernel variants explicitly (cfg6 as above). A new handle starts at
+ * {-4, -16, -8, -8, -4} for FP64 and {-8, -16, -16, -16, -8} for FP32 (the
+ * autotuner's usual choice at 256-point sides); a kernel whose side the
+ * register engine does not cover takes the LDS-stage FFT instead. */
 int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* cfg6);
 
 /* Per-kernel timing with HIP events on the handle's stream. When enabled,

@@ -79,9 +79,10 @@ struct Op {
     void* B = nullptr;              // [3 nrhs][N][Py][Px]
     int nrhs_cap = 0;
 
-    // elements per thread of each pass (fwd_x, fwd_y, z, inv_y, inv_x) -> tile shape
-    int emax64[5] = {16, 16, 8, 16, 16};
-    int emax32[5] = {32, 32, 16, 32, 32};
+    // per-kernel variants (fwd_x, fwd_y, z, inv_y, inv_x): the register engine
+    // where the side allows it (apply falls back to the LDS-stage engine)
+    int emax64[5] = {-4, -16, -8, -8, -4};
+    int emax32[5] = {-8, -16, -16, -16, -8};
 
     // optional per-kernel timing with HIP events on the handle's stream: each
     // product records 6 events into the next row of a ring, read back (with
