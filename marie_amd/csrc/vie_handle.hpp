@@ -44,12 +44,14 @@ struct Op {
     AxisPlan plan[3];
 
     // device data
-    // fN: full c2<T>[6][Pz][Py][Px], or its octant [6][Pz/2+1][Py/2+1][Px/2+1]
-    // when fN_oct (every component has the circulant's per-axis parity)
+    // fN: full c2<T>[6][Pz][Py][Px], or its octant [Pz/2+1][Py/2+1][Px/2+1][6]
+    // (the 6 components of a point adjacent, so the z pass reads one 96-byte
+    // run per point) when fN_oct (every component has the circulant's per-axis
+    // parity)
     void* fN64 = nullptr;
     void* fN32 = nullptr;
     bool fN_oct = false;
-    // fK: the K operator, c2<T>[3][Pz][Py][Px] or its octant (fK_oct)
+    // fK: the K operator, c2<T>[3][Pz][Py][Px] or its octant [..][3] (fK_oct)
     void* fK64 = nullptr;
     void* fK32 = nullptr;
     bool fK_oct = false;

@@ -120,6 +120,7 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
     const int TX = 1 << lg;
     const int msk = TX - 1;
     const int P = pl.P;
+    constexpr int NC = PROD ? 3 : 6;    // operator components
     int bx, j, rhs;
     zpass_tile((int)blockIdx.x, (int)gridDim.x, (Px + TX - 1) / TX, Py, nrhs, bx, j, rhs);
     const int i0 = bx * TX;
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
     if (OCT) {
         hx = Px / 2; hy = Py / 2; hz = P / 2;
         fplane = (int64_t)(hy + 1) * (hx + 1);
-        fstride = fplane * (hz + 1);
+        fstride = 1;                // components interleaved per octant point
         fy = j > hy;
         frow = (int64_t)(fy ? Py - j : j) * (hx + 1);
     } else {
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
             if (b < nx) {
                 const int i = i0 + b;
                 const bool fx = OCT && i > hx, fz = OCT && k > hz;
-                const int64_t o = OCT ? (int64_t)(fz ? P - k : k) * fplane + frow + (fx ? Px - i : i)
+                const int64_t o = OCT ? ((int64_t)(fz ? P - k : k) * fplane + frow + (fx ? Px - i : i)) * NC
                                       : k * fplane + frow + b;
                 freq_product<T, PROD, OCT>(fN + o, fstride, fx, fy, fz, X[0][q], X[1][q], X[2][q]);
             }
@@ -256,10 +257,11 @@ __global__ void k_oct_compress(const c2<double>* __restrict__ F, c2<T>* __restri
     const int hx = Px / 2 + 1, hy = Py / 2 + 1, hz = Pz / 2 + 1;
     const int64_t tot = (int64_t)hx * hy * hz * nc;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(e % hx);
-        const int j = (int)((e / hx) % hy);
-        const int k = (int)((e / ((int64_t)hx * hy)) % hz);
-        const int m = (int)(e / ((int64_t)hx * hy * hz));
+        const int m = (int)(e % nc);
+        const int64_t r = e / nc;
+        const int i = (int)(r % hx);
+        const int j = (int)((r / hx) % hy);
+        const int k = (int)(r / ((int64_t)hx * hy));
         const c2<double> v = F[(((int64_t)m * Pz + k) * Py + j) * Px + i];
         O[e] = c2<T>{(T)v.x, (T)v.y};
     }
@@ -276,7 +278,7 @@ __global__ void k_oct_expand(const c2<double>* __restrict__ O, c2<double>* __res
         const int m = (int)(e / ((int64_t)Px * Py * Pz));
         const bool fx = i > hx, fy = j > hy, fz = k > hz;
         const int ie = fx ? Px - i : i, je = fy ? Py - j : j, ke = fz ? Pz - k : k;
-        c2<double> v = O[(((int64_t)m * (hz + 1) + ke) * (hy + 1) + je) * (hx + 1) + ie];
+        c2<double> v = O[((((int64_t)ke * (hy + 1) + je) * (hx + 1) + ie)) * nc + m];
         bool ox, oy, oz;
         parity(kind, m, ox, oy, oz);
         if ((ox && fx) != ((oy && fy) != (oz && fz))) v = c2<double>{-v.x, -v.y};
