@@ -101,8 +101,23 @@ __device__ __forceinline__ void c_bfly(c2<T> (&v)[E], const c2<T>* __restrict__ 
     }
 }
 
+// LDS slot of exchange position o. PAD inserts one element after every 8:
+// when consecutive lanes hold consecutive butterflies of one line (sp = 1),
+// a stage of radix Rp writes at a stride of Rp elements, which without the
+// padding lands every lane of a quarter-wave in the same banks.
+template <bool PAD>
+__device__ __forceinline__ int c_slot(int o) {
+    return PAD ? o + (o >> 3) : o;
+}
+
+// elements of exchange area one line needs
+template <int P, bool PAD>
+constexpr int c_line_len() {
+    return PAD ? P + P / 8 : P;
+}
+
 // outputs of a stage (radix Rp, sub-size Nsp) -> inputs of the next (radix R)
-template <typename T, int P, int E, int Rp, int Nsp, int R>
+template <typename T, int P, int E, int Rp, int Nsp, int R, bool PAD>
 __device__ __forceinline__ void c_exchange(c2<T> (&v)[E], c2<T>* lds, int sp, int off, int t) {
     constexpr int Tl = P / E;
     lds_barrier();
@@ -112,17 +127,17 @@ __device__ __forceinline__ void c_exchange(c2<T> (&v)[E], c2<T>* lds, int sp, in
         const int jj = t + Tl * q;
         const int k = jj & (Nsp - 1);
         const int o = (jj - k) * Rp + k + r * Nsp;
-        lds[o * sp + off] = v[m];
+        lds[c_slot<PAD>(o) * sp + off] = v[m];
     }
     lds_barrier();
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int q = m / R, r = m % R;
-        v[m] = lds[((t + Tl * q) + r * (P / R)) * sp + off];
+        v[m] = lds[c_slot<PAD>((t + Tl * q) + r * (P / R)) * sp + off];
     }
 }
 
-template <typename T, int P, int E, bool INV, bool REV, int S>
+template <typename T, int P, int E, bool INV, bool REV, int S, bool PAD>
 __device__ __forceinline__ void c_fft_rec(c2<T> (&v)[E], c2<T>* lds, int sp, int off, const c2<T>* tw, int t) {
     constexpr int n = make_cplan(P, E).n;
     if constexpr (S < n) {
@@ -130,18 +145,19 @@ __device__ __forceinline__ void c_fft_rec(c2<T> (&v)[E], c2<T>* lds, int sp, int
         constexpr int Ns = cplan_ns(P, E, REV, S);
         if constexpr (S > 0) {
             constexpr int Rp = cplan_radix(P, E, REV, S - 1);
-            c_exchange<T, P, E, Rp, Ns / Rp, R>(v, lds, sp, off, t);
+            c_exchange<T, P, E, Rp, Ns / Rp, R, PAD>(v, lds, sp, off, t);
         }
         c_bfly<T, P, E, R, Ns, INV>(v, tw, t);
-        c_fft_rec<T, P, E, INV, REV, S + 1>(v, lds, sp, off, tw, t);
+        c_fft_rec<T, P, E, INV, REV, S + 1, PAD>(v, lds, sp, off, tw, t);
     }
 }
 
 // Full transform (REV: radices in reverse order, used so an inverse can start
-// from the layout a forward transform ends in).
-template <typename T, int P, int E, bool INV, bool REV>
+// from the layout a forward transform ends in; PAD: padded exchange slots,
+// lines c_line_len<P, true>() elements apart).
+template <typename T, int P, int E, bool INV, bool REV, bool PAD = false>
 __device__ __forceinline__ void c_fft(c2<T> (&v)[E], c2<T>* lds, int sp, int off, const c2<T>* tw, int t) {
-    c_fft_rec<T, P, E, INV, REV, 0>(v, lds, sp, off, tw, t);
+    c_fft_rec<T, P, E, INV, REV, 0, PAD>(v, lds, sp, off, tw, t);
 }
 
 // position held by slot m in a first-stage input / last-stage output layout of radix R
