@@ -53,6 +53,91 @@ def kernel_bytes(L, M, N, Px, Py, Pz, nidx, s):
     }
 
 
+def other_workloads(torch, dev, geometry, lib, vie):
+    """The 32^3 sphere (radius 0.1 m, eps_r 50, sigma 0.5 S/m, 6.25 mm, 298 MHz)
+    and the 96x96x112 head phantom at 2 mm (128 and 298 MHz): GPU FP64
+    matvecs/s and GMRES(50) time to 1e-6, beside the CPU restatement's
+    (oracle mv_AN + pgmres, numpy + scipy.fft) matvecs/s and, where its
+    estimated duration stays under 200 s, its time to 1e-6."""
+    from oracle import krylov as okr, mvp as omvp     # CPU baseline only
+    cores = min(os.cpu_count() or 1, 16)
+    rows = []
+    for name, shape, dx, freq in (('sphere_32cubed_6p25mm_298MHz', (32, 32, 32), 6.25e-3, 298e6),
+                                  ('head_96x96x112_2mm_128MHz', (96, 96, 112), 2e-3, 128e6),
+                                  ('head_96x96x112_2mm_298MHz', (96, 96, 112), 2e-3, 298e6)):
+        L, M, N = shape
+        em = vie.EMconstants(freq)
+        if name.startswith('sphere'):
+            r = geometry.cell_grid(L, M, N, dx)
+            inside = np.linalg.norm(r, axis=-1) < 0.1
+            e_r = np.where(inside, 50.0 - 1j * 0.5 / (em['eo'] * em['omega']), 1.0 + 0j)
+        else:
+            r, eps, sig, _ = geometry.head_phantom(L, M, N, dx, seed=0, scale=1.0)
+            e_r = eps - 1j * sig / (em['eo'] * em['omega'])
+        Einc, _ = vie.PlaneWave_Excitation(r, [0, 0, em['ko']], em['omega_mu'], [1, 0, 0])
+        Mr, Mc, Gram, idx1, Vrhs = vie.jvie_system(Einc, e_r, r, freq, form=2)
+        op = lib.Operator(L, M, N)
+        op.assemble_N(dx, em['ko'])
+        op.set_material(Mr, Mc, Gram)
+        op.set_index(idx1, one_based=True)
+        cfg, _ = op.autotune(lib.PREC_F64, iters=5)
+        x = torch.randn(idx1.size, dtype=torch.complex128, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(20):
+            op.apply_device(x.data_ptr(), y.data_ptr(), lib.OP_AN, False, lib.PREC_F64)
+        torch.cuda.synchronize()
+        k, t = 200, time.perf_counter()
+        for _ in range(k):
+            op.apply_device(x.data_ptr(), y.data_ptr(), lib.OP_AN, False, lib.PREC_F64)
+        torch.cuda.synchronize()
+        gpu_mv = k / (time.perf_counter() - t)
+        b = torch.from_numpy(np.ascontiguousarray(Vrhs)).to(dev)
+        xs = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        flag, relres, it, _ = op.gmres_device(b.data_ptr(), xs.data_ptr(), restart=50, tol=1e-6, maxit=200)
+        torch.cuda.synchronize()
+        gpu_solve = time.perf_counter() - t
+        iters = (int(it[1]) - 1) * 50 + int(it[0])
+        # CPU restatement on the same operator
+        fN = op.get_fN()
+        xh = x.cpu().numpy()
+        idx0 = idx1 - 1
+        omvp.mv_AN(xh, fN, Mr, Mc, Gram, 'notransp', idx0, workers=cores)
+        reps, t = 0, time.perf_counter()
+        while reps < 50 and time.perf_counter() - t < 10.0:
+            omvp.mv_AN(xh, fN, Mr, Mc, Gram, 'notransp', idx0, workers=cores)
+            reps += 1
+        t_mv = (time.perf_counter() - t) / reps
+        row = {'workload': name, 'dof': int(idx1.size), 'tiles_fp64': cfg, 'gpu_matvecs_per_s': gpu_mv,
+               'gpu_time_to_1e-6_s': gpu_solve, 'gpu_iterations': iters, 'gpu_flag': int(flag),
+               'gpu_relres': float(relres), 'cpu_matvecs_per_s': 1.0 / t_mv, 'cpu_cores': cores,
+               'cpu_matvec_sample': reps}
+        if iters * t_mv < 200.0:
+            calls = [0]
+
+            def A(v):
+                calls[0] += 1
+                if calls[0] % 20 == 0:
+                    log('  CPU solve: %d products' % calls[0])
+                return omvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx0, workers=cores)
+            t = time.perf_counter()
+            xc, cflag, crel, cit, _ = okr.pgmres(A, Vrhs, restart=50, tol=1e-6, maxit=200)
+            row.update({'cpu_time_to_1e-6_s': time.perf_counter() - t, 'cpu_flag': int(cflag),
+                        'cpu_iterations': (int(cit[1]) - 1) * 50 + int(cit[0]),
+                        'gpu_vs_cpu_solution_rel_diff': float(np.linalg.norm(xs.cpu().numpy() - xc)
+                                                              / np.linalg.norm(xc))})
+        else:
+            row['cpu_time_to_1e-6_s_estimate'] = iters * t_mv
+        log('%s: GPU %.1f matvecs/s, solve %.3f s (%d it); CPU %.2f matvecs/s, solve %s' % (
+            name, gpu_mv, gpu_solve, iters, 1.0 / t_mv,
+            ('%.1f s' % row['cpu_time_to_1e-6_s']) if 'cpu_time_to_1e-6_s' in row
+            else 'est. %.0f s' % row['cpu_time_to_1e-6_s_estimate']))
+        rows.append(row)
+        op.close()
+    return {'metric': 'other workloads', 'workloads': rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -69,6 +154,9 @@ def main():
                     help='assemble and run the timed FP64 matvecs only (for rocprofv3 runs)')
     ap.add_argument('--profile-solve', action='store_true',
                     help='assemble, then one FP64 and one FP32-refined solve to 1e-6 (for rocprofv3 runs)')
+    ap.add_argument('--configs', action='store_true',
+                    help='the other workloads instead: 32^3 sphere and 96x96x112 phantom at 128/298 MHz, '
+                         'GPU and CPU-restatement matvecs/s and time to 1e-6')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='run a plain 1 GiB device copy five times (scale factor for FETCH/WRITE_SIZE)')
     args = ap.parse_args()
@@ -91,6 +179,9 @@ def main():
 
     dev = torch.device('cuda:0')
     torch.cuda.init()
+    if args.configs:
+        print(json.dumps(other_workloads(torch, dev, geometry, lib, vie)))
+        return
     n1 = args.size
     L = M = N = n1
     dx = 1.5e-3 * 128 / n1
