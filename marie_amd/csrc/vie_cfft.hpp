@@ -11,18 +11,6 @@
 
 namespace marie {
 
-// Workgroup barrier for the LDS exchanges: waits for this wave's LDS
-// accesses (lgkmcnt) only, so global loads already issued (the next field
-// component) stay in flight across it. The kernels that use it communicate
-// between threads through LDS only. The empty asm statements keep the
-// compiler from moving memory accesses across the barrier.
-__device__ __forceinline__ void lds_barrier() {
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt(0xc07f);      // vmcnt, expcnt at their maximum; lgkmcnt(0)
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
-
 // Raw buffer access to an array of at most 2^31 - 1 bytes: an offset at or
 // beyond the range reads zero without touching memory, which replaces a
 // branch around loads of padding (k >= N) or of columns past the grid.

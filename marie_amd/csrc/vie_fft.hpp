@@ -15,6 +15,18 @@
 
 namespace marie {
 
+// Workgroup barrier for the LDS exchanges: waits for this wave's LDS
+// accesses (lgkmcnt) only, so global loads already issued (the next field
+// component) stay in flight across it. The kernels that use it communicate
+// between threads through LDS only. The empty asm statements keep the
+// compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);      // vmcnt, expcnt at their maximum; lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 template <typename T> struct __align__(2 * sizeof(T)) c2 {
     T x, y;
 };
@@ -173,7 +185,7 @@ __device__ __forceinline__ void stage(c2<T>* buf, const c2<T>* tw, int P, int Ns
             for (int r = 0; r < R; ++r) v[q][r] = buf[((jj + r * nb) << lg) + b];
         }
     }
-    __syncthreads();
+    lds_barrier();
     const int tstride = P / (Ns * R);
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) {
@@ -194,7 +206,7 @@ __device__ __forceinline__ void stage(c2<T>* buf, const c2<T>* tw, int P, int Ns
             for (int r = 0; r < R; ++r) buf[((jo + r * Ns) << lg) + b] = v[q][r];
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // Radix sets: a kernel is instantiated for the smallest set its plan needs,
@@ -215,6 +227,8 @@ __host__ __device__ inline int plan_rset(const AxisPlan& pl) {
 
 // Full transform of the tile in LDS. buf and tw live in LDS or global; the
 // caller synchronises before (data loaded) and this returns synchronised.
+// The barriers order LDS accesses only (lds_barrier): callers communicate
+// between threads through LDS alone, so their global loads stay in flight.
 template <typename T, int EMAX, bool INV, int RSET>
 __device__ void tile_fft(c2<T>* buf, const c2<T>* tw, const AxisPlan& pl, int lg) {
     int Ns = 1;

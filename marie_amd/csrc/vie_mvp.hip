@@ -63,8 +63,10 @@ __global__ __launch_bounds__(NT) void k_fwd_x(const c2<T>* __restrict__ x, c2<T>
         }
         buf[i * TX + b] = v;
     }
-    __syncthreads();
-    tile_fft<T, EMAX, false, RSET>(buf, tw, pl, lg);
+    c2<T>* twl = buf + TX * P;          // the axis's twiddles, staged behind the tile
+    for (int e = threadIdx.x; e < P; e += NT) twl[e] = tw[e];
+    lds_barrier();
+    tile_fft<T, EMAX, false, RSET>(buf, twl, pl, lg);
     for (int e = threadIdx.x; e < TX * P; e += NT) {
         const int b = e / P, i = e - b * P;
         const int64_t line = line0 + b;
@@ -97,8 +99,10 @@ __global__ __launch_bounds__(NT) void k_strided(const c2<T>* __restrict__ src, c
         if (p < nin && b < nx) v = s[p * s_ls + b];
         buf[e] = v;
     }
-    __syncthreads();
-    tile_fft<T, EMAX, INV, RSET>(buf, tw, pl, lg);
+    c2<T>* twl = buf + TX * P;
+    for (int e = threadIdx.x; e < P; e += NT) twl[e] = tw[e];
+    lds_barrier();
+    tile_fft<T, EMAX, INV, RSET>(buf, twl, pl, lg);
     for (int e = threadIdx.x; e < TX * P; e += NT) {
         const int b = e & msk, p = e >> lg;
         if (p < nout && b < nx) d[p * d_ls + b] = buf[e];
@@ -130,6 +134,8 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
     const int tot = TX * P;
     c2<T>* base = B + (int64_t)rhs * 3 * cstride + (int64_t)j * Px + i0;
     c2<T> X[3][EMAX];
+    c2<T>* twl = buf + tot;             // the axis's twiddles, staged behind the tile
+    for (int e = threadIdx.x; e < P; e += NT) twl[e] = tw[e];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         for (int e = threadIdx.x; e < tot; e += NT) {
@@ -138,14 +144,14 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
             if (k < N && b < nx) v = base[c * cstride + k * plane + b];
             buf[e] = v;
         }
-        __syncthreads();
-        tile_fft<T, EMAX, false, RSET>(buf, tw, pl, lg);
+        lds_barrier();
+        tile_fft<T, EMAX, false, RSET>(buf, twl, pl, lg);
 #pragma unroll
         for (int q = 0; q < EMAX; ++q) {
             const int e = threadIdx.x + q * NT;
             if (e < tot) X[c][q] = buf[e];
         }
-        __syncthreads();
+        lds_barrier();
     }
     // fN addressing
     int64_t fplane, fstride, frow;
@@ -183,13 +189,13 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
             const int e = threadIdx.x + q * NT;
             if (e < tot) buf[e] = X[a][q];
         }
-        __syncthreads();
-        tile_fft<T, EMAX, true, RSET>(buf, tw, pl, lg);
+        lds_barrier();
+        tile_fft<T, EMAX, true, RSET>(buf, twl, pl, lg);
         for (int e = threadIdx.x; e < tot; e += NT) {
             const int b = e & msk, k = e >> lg;
             if (k < N && b < nx) base[a * cstride + k * plane + b] = buf[e];
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -403,8 +409,10 @@ __global__ __launch_bounds__(NT) void k_inv_x(const c2<T>* __restrict__ A, c2<T>
         if (line < nline) v = A[((int64_t)comp * nline + line) * P + i];
         buf[i * TX + b] = v;
     }
-    __syncthreads();
-    tile_fft<T, EMAX, true, RSET>(buf, tw, pl, lg);
+    c2<T>* twl = buf + TX * P;
+    for (int e = threadIdx.x; e < P; e += NT) twl[e] = tw[e];
+    lds_barrier();
+    tile_fft<T, EMAX, true, RSET>(buf, twl, pl, lg);
     const c2<T>* xr = x + rhs * nidx;
     c2<T>* yr = y + rhs * nidx;
     for (int e = threadIdx.x; e < TX * P; e += NT) {
@@ -665,7 +673,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     } else {
         const int lg = pick_lg(Px, e[0] * NT, 16);
         const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
-        const size_t sm = ((size_t)Px << lg) * sizeof(c2<T>);
+        const size_t sm = (((size_t)Px << lg) + Px) * sizeof(c2<T>);    // tile + twiddles
         MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e[0], RS, hipLaunchKernelGGL((k_fwd_x<T, EM, RS>), g, dim3(NT), sm,
                    s, x, A, h->pos, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, conj_in)));
     }
@@ -678,7 +686,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     } else {
         const int lg = pick_lg(Py, e[1] * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, ncomp);
-        const size_t sm = ((size_t)Py << lg) * sizeof(c2<T>);
+        const size_t sm = (((size_t)Py << lg) + Py) * sizeof(c2<T>);    // tile + twiddles
         MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e[1], RS, hipLaunchKernelGGL((k_strided<T, EM, false, RS>), g,
                    dim3(NT), sm, s, A, B, twy, h->plan[1], Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
                    (int64_t)Py * Px, M, Py, lg, (int64_t)N * M * Px, (int64_t)N * Py * Px)));
@@ -691,7 +699,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     } else {
         const int lg = pick_lg(Pz, e[2] * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg) * Py * nrhs);
-        const size_t sm = ((size_t)Pz << lg) * sizeof(c2<T>);
+        const size_t sm = (((size_t)Pz << lg) + Pz) * sizeof(c2<T>);    // tile + twiddles
         if (oct && !kop)
             MARIE_RSET(h->plan[2], MARIE_EMAX_Z(T, e[2], RS, hipLaunchKernelGGL((k_z<T, EM, RS, true, 0>), g,
                        dim3(NT), sm, s, B, fN, twz, h->plan[2], Px, Py, N, lg, nrhs)));
@@ -714,7 +722,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     } else {
         const int lg = pick_lg(Py, e[3] * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, ncomp);
-        const size_t sm = ((size_t)Py << lg) * sizeof(c2<T>);
+        const size_t sm = (((size_t)Py << lg) + Py) * sizeof(c2<T>);    // tile + twiddles
         MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e[3], RS, hipLaunchKernelGGL((k_strided<T, EM, true, RS>), g,
                    dim3(NT), sm, s, B, A, twy, h->plan[1], Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
                    (int64_t)M * Px, Py, M, lg, (int64_t)N * Py * Px, (int64_t)N * M * Px)));
@@ -730,7 +738,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
         } else {
             const int lg = pick_lg(Px, e[4] * NT, 16);
             const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
-            const size_t sm = ((size_t)Px << lg) * sizeof(c2<T>);
+            const size_t sm = (((size_t)Px << lg) + Px) * sizeof(c2<T>);    // tile + twiddles
             MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e[4], RS, hipLaunchKernelGGL((k_inv_x<T, EM, RS>), g, dim3(NT),
                        sm, s, A, y, x, h->pos, Mr, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, op, transp,
                        (T)h->Gram, scale)));
