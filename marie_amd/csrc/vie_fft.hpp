@@ -211,16 +211,20 @@ __device__ __forceinline__ void stage(c2<T>* buf, const c2<T>* tw, int P, int Ns
 
 // Radix sets: a kernel is instantiated for the smallest set its plan needs,
 // which keeps the register allocation of the common power-of-two case small.
+// (7 apart from 11 and 13: the largest radix sets the kernel's register
+// count, and 7-smooth sides are the common case)
 constexpr int RSET_POW2 = 0;   // 2, 4, 8
 constexpr int RSET_SMALL = 1;  // + 3, 5
-constexpr int RSET_ALL = 2;    // + 7, 11, 13
+constexpr int RSET_SEVEN = 2;  // + 7
+constexpr int RSET_ALL = 3;    // + 11, 13
 
 __host__ __device__ inline int plan_rset(const AxisPlan& pl) {
     int s = RSET_POW2;
     for (int i = 0; i < pl.nst; ++i) {
         const int R = pl.radix[i];
         if (R == 3 || R == 5) s = s < RSET_SMALL ? RSET_SMALL : s;
-        if (R == 7 || R == 11 || R == 13) s = RSET_ALL;
+        if (R == 7) s = s < RSET_SEVEN ? RSET_SEVEN : s;
+        if (R == 11 || R == 13) s = RSET_ALL;
     }
     return s;
 }
@@ -241,9 +245,11 @@ __device__ void tile_fft(c2<T>* buf, const c2<T>* tw, const AxisPlan& pl, int lg
             if (R == 3) stage<T, 3, EMAX, INV>(buf, tw, pl.P, Ns, lg);
             else if (R == 5) stage<T, 5, EMAX, INV>(buf, tw, pl.P, Ns, lg);
         }
-        if constexpr (RSET >= RSET_ALL) {
+        if constexpr (RSET >= RSET_SEVEN) {
             if (R == 7) stage<T, 7, EMAX, INV>(buf, tw, pl.P, Ns, lg);
-            else if (R == 11) stage<T, 11, EMAX, INV>(buf, tw, pl.P, Ns, lg);
+        }
+        if constexpr (RSET >= RSET_ALL) {
+            if (R == 11) stage<T, 11, EMAX, INV>(buf, tw, pl.P, Ns, lg);
             else if (R == 13) stage<T, 13, EMAX, INV>(buf, tw, pl.P, Ns, lg);
         }
         Ns *= R;
@@ -256,7 +262,8 @@ __device__ void tile_fft(c2<T>* buf, const c2<T>* tw, const AxisPlan& pl, int lg
         switch (::marie::plan_rset(plan)) {                                \
             case ::marie::RSET_POW2: { constexpr int RS = 0; STMT; } break; \
             case ::marie::RSET_SMALL: { constexpr int RS = 1; STMT; } break;\
-            default: { constexpr int RS = 2; STMT; } break;                 \
+            case ::marie::RSET_SEVEN: { constexpr int RS = 2; STMT; } break;\
+            default: { constexpr int RS = 3; STMT; } break;                 \
         }                                                                  \
     } while (0)
 

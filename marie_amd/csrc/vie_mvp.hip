@@ -518,7 +518,7 @@ static int pick_lg(int P, int cap, int txmax) {
 
 // Tile variants. Each pass runs with EMAX elements per thread (so a workgroup of
 // NT threads holds TX = EMAX*NT/P lines in LDS); the compiled set below is what
-// the autotuner chooses from. Radix sets beyond {3,5} only get the default.
+// the autotuner chooses from. The radix set with 11 and 13 only gets the default.
 template <typename T> struct Variants;
 template <> struct Variants<double> {
     static constexpr int xy[3] = {4, 8, 16};
@@ -534,22 +534,22 @@ template <> struct Variants<float> {
 #define MARIE_EMAX_XY(T, emax, RSv, STMT)                                                   \
     do {                                                                                    \
         if constexpr (sizeof(T) == 8) {                                                     \
-            if ((emax) == 4 && RSv < 2) { constexpr int EM = 4; STMT; }                     \
-            else if ((emax) == 8 && RSv < 2) { constexpr int EM = 8; STMT; }                \
+            if ((emax) == 4 && RSv < 3) { constexpr int EM = 4; STMT; }                     \
+            else if ((emax) == 8 && RSv < 3) { constexpr int EM = 8; STMT; }                \
             else { constexpr int EM = 16; STMT; }                                           \
         } else {                                                                            \
-            if ((emax) == 8 && RSv < 2) { constexpr int EM = 8; STMT; }                     \
-            else if ((emax) == 16 && RSv < 2) { constexpr int EM = 16; STMT; }              \
+            if ((emax) == 8 && RSv < 3) { constexpr int EM = 8; STMT; }                     \
+            else if ((emax) == 16 && RSv < 3) { constexpr int EM = 16; STMT; }              \
             else { constexpr int EM = 32; STMT; }                                           \
         }                                                                                   \
     } while (0)
 #define MARIE_EMAX_Z(T, emax, RSv, STMT)                                                    \
     do {                                                                                    \
         if constexpr (sizeof(T) == 8) {                                                     \
-            if ((emax) == 4 && RSv < 2) { constexpr int EM = 4; STMT; }                     \
+            if ((emax) == 4 && RSv < 3) { constexpr int EM = 4; STMT; }                     \
             else { constexpr int EM = 8; STMT; }                                            \
         } else {                                                                            \
-            if ((emax) == 8 && RSv < 2) { constexpr int EM = 8; STMT; }                     \
+            if ((emax) == 8 && RSv < 3) { constexpr int EM = 8; STMT; }                     \
             else { constexpr int EM = 16; STMT; }                                           \
         }                                                                                   \
     } while (0)
