@@ -110,7 +110,7 @@ __global__ __launch_bounds__(NT) void k_strided(const c2<T>* __restrict__ src, c
 }
 
 // Fused z pass: forward z FFT of the three components, tensor product, inverse.
-// OCT: fN is stored as its octant [6][Pz/2+1][Py/2+1][Px/2+1]. The circulant of
+// OCT: fN is stored as its octant [Pz/2+1][Py/2+1][Px/2+1][6] (K: [..][3]). The circulant of
 // each tensor component is even or odd under reflection of each axis
 // (gperiodic_coeff_nop.m), so its DFT has the same parity:
 // F(P-k) = s F(k) with s = -1 iff exactly one of the component's two indices is
