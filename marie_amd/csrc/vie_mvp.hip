@@ -646,10 +646,15 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     // buffer offset: larger grids take the LDS-stage engine
     if ((int64_t)Px * Py * N * (int64_t)sizeof(c2<T>) > (int64_t)0x7fffffff) reg[2] = false;
     const int dflt[5] = {dxy, dxy, dz, dxy, dxy};
+    // LDS-stage variants for a register-engine code whose side the register
+    // engine does not cover (the tuner's usual choice on 2-3-7 sides)
+    const int fb64[5] = {8, 8, 4, 8, 4}, fb32[5] = {8, 8, 8, 16, 8};
     const AxisPlan* apl[5] = {&h->plan[0], &h->plan[1], &h->plan[2], &h->plan[1], &h->plan[0]};
     int e[5];
     for (int k = 0; k < 5; ++k) {
-        e[k] = effective_emax(em[k] > 0 ? em[k] : dflt[k], *apl[k], dflt[k]);
+        int want = em[k] > 0 ? em[k] : (f64 ? fb64[k] : fb32[k]);
+        if (em[k] <= 0 && Pp[k] > want * NT) want = dflt[k];
+        e[k] = effective_emax(want, *apl[k], dflt[k]);
         if (!reg[k] && Pp[k] > e[k] * NT) {
             set_error("circulant side exceeds the single-tile FFT limit of the selected tile variant");
             return ERR_SIZE;
