@@ -20,14 +20,24 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, in
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
 
-template <typename T> __device__ __forceinline__ c2<T> buf_load(__amdgpu_buffer_rsrc_t rs, int off);
-template <> __device__ __forceinline__ c2<double> buf_load<double>(__amdgpu_buffer_rsrc_t rs, int off) {
+// AUX: the instruction's cache-policy bits (0 default, 2 non-temporal)
+template <typename T, int AUX = 0> __device__ __forceinline__ c2<T> buf_load(__amdgpu_buffer_rsrc_t rs, int off);
+template <> __device__ __forceinline__ c2<double> buf_load<double, 0>(__amdgpu_buffer_rsrc_t rs, int off) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
     return c2<double>{__builtin_bit_cast(double, ((unsigned long long)(unsigned)v[1] << 32) | (unsigned)v[0]),
                       __builtin_bit_cast(double, ((unsigned long long)(unsigned)v[3] << 32) | (unsigned)v[2])};
 }
-template <> __device__ __forceinline__ c2<float> buf_load<float>(__amdgpu_buffer_rsrc_t rs, int off) {
+template <> __device__ __forceinline__ c2<float> buf_load<float, 0>(__amdgpu_buffer_rsrc_t rs, int off) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+    return c2<float>{__builtin_bit_cast(float, (unsigned)v[0]), __builtin_bit_cast(float, (unsigned)v[1])};
+}
+template <> __device__ __forceinline__ c2<double> buf_load<double, 2>(__amdgpu_buffer_rsrc_t rs, int off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 2);
+    return c2<double>{__builtin_bit_cast(double, ((unsigned long long)(unsigned)v[1] << 32) | (unsigned)v[0]),
+                      __builtin_bit_cast(double, ((unsigned long long)(unsigned)v[3] << 32) | (unsigned)v[2])};
+}
+template <> __device__ __forceinline__ c2<float> buf_load<float, 2>(__amdgpu_buffer_rsrc_t rs, int off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 2);
     return c2<float>{__builtin_bit_cast(float, (unsigned)v[0]), __builtin_bit_cast(float, (unsigned)v[1])};
 }
 
