@@ -109,6 +109,31 @@ __global__ __launch_bounds__(NT) void k_strided(const c2<T>* __restrict__ src, c
     }
 }
 
+// Non-temporal access to the z pass's work buffer in FP64, where it is read
+// once and written once: the operator octant, read by up to 8 mirror tiles,
+// keeps its place in the caches. (In complex64 the buffer is half the size
+// and is still partly cached from the y pass that wrote it; there plain
+// accesses measured faster.)
+template <typename T>
+__device__ __forceinline__ c2<T> ld_nt(const c2<T>* p) {
+    if constexpr (sizeof(T) == 8) {
+        typedef T v2 __attribute__((ext_vector_type(2)));
+        const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+        return c2<T>{v.x, v.y};
+    } else {
+        return *p;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void st_nt(c2<T>* p, c2<T> a) {
+    if constexpr (sizeof(T) == 8) {
+        typedef T v2 __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(v2{a.x, a.y}, reinterpret_cast<v2*>(p));
+    } else {
+        *p = a;
+    }
+}
+
 // Fused z pass: forward z FFT of the three components, tensor product, inverse.
 // OCT: fN is stored as its octant [Pz/2+1][Py/2+1][Px/2+1][6] (K: [..][3]). The circulant of
 // each tensor component is even or odd under reflection of each axis
@@ -141,7 +166,7 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
         for (int e = threadIdx.x; e < tot; e += NT) {
             const int b = e & msk, k = e >> lg;
             c2<T> v = {T(0), T(0)};
-            if (k < N && b < nx) v = base[c * cstride + k * plane + b];
+            if (k < N && b < nx) v = ld_nt(base + c * cstride + k * plane + b);
             buf[e] = v;
         }
         lds_barrier();
@@ -193,7 +218,7 @@ __global__ __launch_bounds__(NT) void k_z(c2<T>* __restrict__ B, const c2<T>* __
         tile_fft<T, EMAX, true, RSET>(buf, twl, pl, lg);
         for (int e = threadIdx.x; e < tot; e += NT) {
             const int b = e & msk, k = e >> lg;
-            if (k < N && b < nx) base[a * cstride + k * plane + b] = buf[e];
+            if (k < N && b < nx) st_nt(base + a * cstride + k * plane + b, buf[e]);
         }
         lds_barrier();
     }
