@@ -187,6 +187,30 @@ def block_gmres_batch(ops, B_ptrs, X_ptrs, nrhs, restart=20, tol=1e-6, maxit=200
     return fl, rr, its
 
 
+def _largest_prime_factor(n):
+    f, p = 1, 2
+    while p * p <= n:
+        while n % p == 0:
+            f, n = p, n // p
+        p += 1
+    return max(f, n) if n > 1 else f
+
+
+def circulant_side(n):
+    """Circulant side for a grid side of n voxels: 2n, as getOPERATORS.m
+    builds it, when 2n factors into primes up to 13 (the FFT radices); else
+    the smallest even side >= 2n with factors up to 7. Any side >= 2n - 1
+    embeds the same linear convolution, so the product is the same to
+    rounding; e.g. the 102-voxel side of a 5 mm head model gets 210 instead of
+    204 = 4 * 3 * 17."""
+    m = 2 * n
+    if _largest_prime_factor(m) <= 13:
+        return m
+    while _largest_prime_factor(m) > 7:
+        m += 2
+    return m
+
+
 class Operator:
     """A device-resident JVIE operator: fN, material, DOF index and work buffers.
 
@@ -196,7 +220,7 @@ class Operator:
 
     def __init__(self, L, M, N, P=None, precisions=PREC_F64, device=0):
         lib = load()
-        P = P or (2 * L, 2 * M, 2 * N)
+        P = P or (circulant_side(L), circulant_side(M), circulant_side(N))
         self.L, self.M, self.N = L, M, N
         self.P = tuple(P)
         self.precisions = precisions

@@ -152,7 +152,9 @@ def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operat
 
     circulant_shape=(Px, Py, Pz), each side >= 2L-1, embeds the same Toeplitz
     operator in another circulant (e.g. 2*3*11 = 66 instead of 2*43 = 86 for
-    L = 43); the matvec result is the same.
+    L = 43); the matvec result is the same. By default each side is 2L unless
+    2L has a prime factor above 13 (lib.circulant_side: e.g. 210 for a
+    102-voxel side, where 204 = 4*3*17).
 
     `singular` 'DIRECTFN' / 'DEMCEM' select the reference's self-term method by
     name; both are served by the library's own machine-precision quadrature.

@@ -69,3 +69,17 @@ def test_device_pointers_carry_resolved_values():
     assert np.array_equal(raw[:, 1], -g.imag.numpy())
     n = adjoint._dense(torch.conj(g).imag)
     assert np.array_equal(n.numpy(), -g.imag.numpy())
+
+
+def test_circulant_side_rule():
+    """2L when its prime factors are at most 13 (getOPERATORS.m's size), else
+    the smallest even 7-smooth side >= 2L."""
+    from marie_amd import lib
+    assert [lib.circulant_side(n) for n in (8, 11, 39, 81, 96, 112, 128)] == [16, 22, 78, 162, 192, 224, 256]
+    assert lib.circulant_side(17) == 36          # 34 = 2 * 17
+    assert lib.circulant_side(102) == 210        # 204 = 4 * 3 * 17
+    assert lib.circulant_side(133) == 270        # 266 = 2 * 7 * 19
+    for n in range(1, 400):
+        m = lib.circulant_side(n)
+        assert m >= 2 * n and m % 2 == 0
+        assert lib._largest_prime_factor(m) <= 13

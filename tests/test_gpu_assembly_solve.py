@@ -181,3 +181,26 @@ def test_gmres_refined_modes_reach_tolerance(mode):
     assert em_ <= max(5 * e64, 1e-6), (em_, e64)
     # resvec ends with the refined FP64 true residual
     assert abs(rvm[-1] - relm) <= 1e-9
+
+
+def test_default_circulant_for_large_prime_factor():
+    """A side whose 2L has a prime factor above 13 (L = 17: 34 = 2 * 17) gets
+    the next 7-smooth circulant side (36) by default; the product equals the
+    oracle's mv_AN on the reference's 2L embedding."""
+    from marie_amd import lib
+    L, M, N = 17, 6, 5
+    dx, f = 6.25e-3, 298.2e6
+    print("case default circulant", (L, M, N), flush=True)
+    ko = 2 * np.pi / C0 * f
+    op = lib.Operator(L, M, N)
+    assert op.P == (36, 12, 10)
+    op.assemble_N(dx, ko)
+    fN = oop.get_operators(L, M, N, dx, f, 'N')          # 34 x 12 x 10 x 6
+    rng = np.random.default_rng(4)
+    e_r = 1 + rng.random((L, M, N)) * 10 - 1j * rng.random((L, M, N))
+    Mr, Mc = np.ones((L, M, N)), (e_r - 1) / e_r
+    idx = np.sort(rng.choice(3 * L * M * N, 900, replace=False))
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    op.set_material(Mr, Mc, dx ** 3)
+    op.set_index(idx)
+    assert _rel(op.apply(x), mvp.mv_AN(x, fN, Mr, Mc, dx ** 3, 'notransp', idx)) <= 1e-12
