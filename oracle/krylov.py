@@ -7,6 +7,10 @@ least-squares problem solved from scratch at every inner step
 restart loop on resvec, and flag/relres from the TRUE residual
 ||b - A x|| / ||b|| (pgmres.m:175-180). Preconditioner handles return the
 preconditioned vector directly (iterapp 'mtimes' on a function handle).
+Also GMRES-DR (pgmresDR.m, getHarmVecs1/2.m) and BiCGStab (MATLAB's
+built-in, not in the reference tree: its iterates are pinned by scipy's
+BiCGStab, while MATLAB's exact iterate sequence and internal stagnation
+tests are parity unpinned).
 Test infrastructure only (see oracle/__init__.py).
 """
 import numpy as np
