@@ -1,4 +1,4 @@
-"""The configurations' real shapes on the GPU.
+"""The configurations' real shapes on the GPU, and the largest grid.
 
 96x96x112 voxels at 2 mm (a 192x192x224 circulant: radices 2, 3 and 7, the
 LDS-stage engine) at 128 and 298 MHz: one FP64 product against the oracle's
@@ -72,3 +72,34 @@ def test_config3_identities():
     assert abs(np.vdot(Ahu, v) - np.vdot(u, Av)) <= 1e-12 * np.linalg.norm(u) * np.linalg.norm(Av)
     w = op.apply(2.0 * u - 0.5j * v)
     assert _rel(w, 2.0 * op.apply(u) - 0.5j * Av) <= 1e-13
+
+
+def test_largest_grid_identities():
+    """256^3 voxels (a 512^3 circulant: register engine on the 512-point x and
+    y lines, the LDS-stage z pass, multi-GB work buffers and a 1.6 GB octant):
+    complex symmetry of N, the adjoint identity of A, linearity, and a solve
+    whose true residual is recomputed."""
+    from marie_amd import lib, vie
+    n = 256
+    dx, f = 0.75e-3, 298e6
+    print("case largest grid", n, flush=True)
+    r, e_r, Einc, em = _phantom(n, n, n, dx, f, 0.85)
+    Mr, Mc, Gram, idx1, b = vie.jvie_system(Einc, e_r, r, f, form=2)
+    del r, Einc
+    op = lib.Operator(n, n, n)
+    assert op.P == (512, 512, 512)
+    op.assemble_N(dx, em['ko'])
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx1, one_based=True)
+    rng = np.random.default_rng(9)
+    u = rng.standard_normal(idx1.size) + 1j * rng.standard_normal(idx1.size)
+    v = rng.standard_normal(idx1.size) + 1j * rng.standard_normal(idx1.size)
+    Nu, Nv = op.apply(u, lib.OP_N), op.apply(v, lib.OP_N)
+    assert abs(u @ Nv - v @ Nu) <= 1e-12 * np.linalg.norm(u) * np.linalg.norm(Nv)
+    Av, Ahu = op.apply(v), op.apply(u, lib.OP_AN, True)
+    assert abs(np.vdot(Ahu, v) - np.vdot(u, Av)) <= 1e-12 * np.linalg.norm(u) * np.linalg.norm(Av)
+    w = op.apply(2.0 * u - 0.5j * v)
+    assert _rel(w, 2.0 * op.apply(u) - 0.5j * Av) <= 1e-13
+    xs, flag, relres, it, resvec = op.gmres(b, restart=50, tol=1e-3, maxit=20)
+    assert flag == 0 and relres <= 1e-3
+    assert _rel(op.apply(xs), b) <= 1e-3
