@@ -75,8 +75,8 @@ def test_config3_identities():
 
 
 def test_largest_grid_identities():
-    """256^3 voxels (a 512^3 circulant: register engine on the 512-point x and
-    y lines, the LDS-stage z pass, multi-GB work buffers and a 1.6 GB octant):
+    """256^3 voxels (a 512^3 circulant: the register engine on 512-point lines,
+    multi-GB work buffers and a 1.6 GB octant):
     complex symmetry of N, the adjoint identity of A, linearity, and a solve
     whose true residual is recomputed."""
     from marie_amd import lib, vie
