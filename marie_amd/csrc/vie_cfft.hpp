@@ -1,4 +1,5 @@
-// Register-resident Stockham FFT with a compile-time plan (power-of-two P).
+// Register-resident Stockham FFT with a compile-time plan (P a power of two,
+// or 2^a times 3, 5 or 7 with the odd factor dividing E).
 //
 // Same data movement as vie_regfft.hpp (thread t of Tl = P/E owns butterflies
 // jj = t + Tl*q of each stage, slot v[q*R + r] holds input jj + r*P/R; LDS only
@@ -52,18 +53,31 @@ struct CPlanData {
     int r[12];
 };
 
-// greedy radices from {16, 8, 4, 2} that divide E and what is left of P
+// greedy radices from {16, 8, 4, 2} that divide E and what is left of P,
+// then 3, 5, 7 for the odd factors (mixed-radix sides such as 192 = 4*4*4*3)
 constexpr CPlanData make_cplan(int P, int E) {
     CPlanData d{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
     int rem = P;
     while (rem > 1 && d.n < 12) {
         int R = 16;
         while (R > 1 && (R > E || E % R != 0 || rem % R != 0)) R >>= 1;
+        if (R == 1) {
+            R = (E % 3 == 0 && rem % 3 == 0) ? 3 : (E % 5 == 0 && rem % 5 == 0) ? 5
+              : (E % 7 == 0 && rem % 7 == 0) ? 7 : 1;
+            if (R == 1) break;
+        }
         d.r[d.n] = R;
         d.n += 1;
         rem /= R;
     }
     return d;
+}
+
+// the plan's radices multiply to P (every instantiated (P, E) must have one)
+constexpr bool cplan_ok(int P, int E) {
+    int prod = 1;
+    for (int i = 0; i < make_cplan(P, E).n; ++i) prod *= make_cplan(P, E).r[i];
+    return prod == P && P % E == 0;
 }
 
 constexpr int cplan_radix(int P, int E, bool rev, int s) {
@@ -82,20 +96,24 @@ constexpr int cplan_last(int P, int E, bool rev) { return cplan_radix(P, E, rev,
 
 template <typename T, int P, int E, int R, int Ns, bool INV>
 __device__ __forceinline__ void c_bfly(c2<T> (&v)[E], const c2<T>* __restrict__ tw, int t) {
+    static_assert(cplan_ok(P, E), "no radix plan for this line length and points per thread");
     constexpr int Tl = P / E;
     constexpr int Q = E / R;
     constexpr int ts = P / (Ns * R);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         if constexpr (Ns > 1) {
-            const int k = (t + Tl * q) & (Ns - 1);
+            const int k = (int)((unsigned)(t + Tl * q) % (unsigned)Ns);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 const c2<T> w = tw[r * k * ts];
                 v[q * R + r] = INV ? cmulc(v[q * R + r], w) : cmul(v[q * R + r], w);
             }
         }
-        rdft<T, INV, R>(&v[q * R]);
+        if constexpr (R == 3) dft3<T, INV>(&v[q * R]);
+        else if constexpr (R == 5) dft5<T, INV>(&v[q * R]);
+        else if constexpr (R == 7) dft_odd<T, INV, 7>(&v[q * R], tw, P);   // P/7-strided entries of the table
+        else rdft<T, INV, R>(&v[q * R]);
     }
 }
 
@@ -123,7 +141,7 @@ __device__ __forceinline__ void c_exchange(c2<T> (&v)[E], c2<T>* lds, int sp, in
     for (int m = 0; m < E; ++m) {
         const int q = m / Rp, r = m % Rp;
         const int jj = t + Tl * q;
-        const int k = jj & (Nsp - 1);
+        const int k = (int)((unsigned)jj % (unsigned)Nsp);
         const int o = (jj - k) * Rp + k + r * Nsp;
         lds[c_slot<PAD>(o) * sp + off] = v[m];
     }

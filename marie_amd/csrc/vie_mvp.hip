@@ -667,6 +667,20 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     const int Pp[5] = {Px, Py, Pz, Py, Px};
     for (int k = 0; k < 5; ++k)
         reg[k] = em[k] < 0 && (k == 2 ? reg_z_supported<T>(Pp[k], -em[k]) : reg_supported<T>(Pp[k], -em[k]));
+    // register-engine points per thread of each pass; an x or y pass whose
+    // mixed-radix side (192, 224) has no plan for the requested count takes
+    // that side's own (12 and 14 points: the tuner's usual choice)
+    int rc[5];
+    for (int k = 0; k < 5; ++k) {
+        rc[k] = -em[k];
+        if (k != 2 && em[k] < 0 && !reg[k]) {
+            const int mix = Pp[k] == 192 ? 12 : Pp[k] == 224 ? 14 : 0;
+            if (mix && reg_supported<T>(Pp[k], mix)) {
+                rc[k] = mix;
+                reg[k] = true;
+            }
+        }
+    }
     // the register-engine z pass addresses one component through a 32-bit
     // buffer offset: larger grids take the LDS-stage engine
     if ((int64_t)Px * Py * N * (int64_t)sizeof(c2<T>) > (int64_t)0x7fffffff) reg[2] = false;
@@ -699,7 +713,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
 
     // forward x
     if (reg[0]) {
-        reg_fwd_x<T>(Px, -em[0], s, x, A, h->pos, Mc, twx, L, M, N, h->nidx, conj_in, ncomp);
+        reg_fwd_x<T>(Px, rc[0], s, x, A, h->pos, Mc, twx, L, M, N, h->nidx, conj_in, ncomp);
     } else {
         const int lg = pick_lg(Px, e[0] * NT, 16);
         const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
@@ -711,7 +725,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     if (tm) MARIE_HIP(hipEventRecord(tev[1], s));
     // forward y: A[c][k][j<M][i] -> B[c][k][j<Py][i]
     if (reg[1]) {
-        reg_strided<T>(false, Py, -em[1], s, A, B, twy, Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
+        reg_strided<T>(false, Py, rc[1], s, A, B, twy, Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
                        (int64_t)Py * Px, M, Py, (int64_t)N * M * Px, (int64_t)N * Py * Px, N, ncomp);
     } else {
         const int lg = pick_lg(Py, e[1] * NT, txmax);
@@ -725,7 +739,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     if (tm) MARIE_HIP(hipEventRecord(tev[2], s));
     // z: fused forward, tensor product, inverse
     if (reg[2]) {
-        reg_z<T>(oct, prod, Pz, -em[2], s, B, fN, twz, Px, Py, N, nrhs);
+        reg_z<T>(oct, prod, Pz, rc[2], s, B, fN, twz, Px, Py, N, nrhs);
     } else {
         const int lg = pick_lg(Pz, e[2] * NT, txmax);
         const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg) * Py * nrhs);
@@ -747,7 +761,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     if (tm) MARIE_HIP(hipEventRecord(tev[3], s));
     // inverse y: B[c][k][j<Py][i] -> A[c][k][j<M][i]
     if (reg[3]) {
-        reg_strided<T>(true, Py, -em[3], s, B, A, twy, Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
+        reg_strided<T>(true, Py, rc[3], s, B, A, twy, Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
                        (int64_t)M * Px, Py, M, (int64_t)N * Py * Px, (int64_t)N * M * Px, N, ncomp);
     } else {
         const int lg = pick_lg(Py, e[3] * NT, txmax);
@@ -763,7 +777,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     {
         const T scale = T(1.0 / ((double)Px * (double)Py * (double)Pz));
         if (reg[4]) {
-            reg_inv_x<T>(Px, -em[4], s, A, y, x, h->pos, Mr, Mc, twx, L, M, N, h->nidx, op, transp, (T)h->Gram, scale,
+            reg_inv_x<T>(Px, rc[4], s, A, y, x, h->pos, Mr, Mc, twx, L, M, N, h->nidx, op, transp, (T)h->Gram, scale,
                          ncomp);
         } else {
             const int lg = pick_lg(Px, e[4] * NT, 16);
@@ -874,6 +888,19 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
     for (int v = 0; v < 3; ++v)
         for (int k = 0; k < 5; ++k) cands[nc + v][k] = -regE[v];
     nc += 3;
+    // register-engine plans of the mixed-radix sides (192: 12 or 24 points
+    // per thread, 224: 14 or 28); a pass whose side has no such plan runs its
+    // LDS-stage fallback in these candidates
+    const int mixE[4] = {12, 24, 14, 28};
+    const int Pp3[3] = {h->Px, h->Py, h->Pz};
+    for (int v = 0; v < 4; ++v) {
+        bool any = false;
+        for (int a = 0; a < 3; ++a)
+            any = any || (f64 ? reg_supported<double>(Pp3[a], mixE[v]) : reg_supported<float>(Pp3[a], mixE[v]));
+        if (!any) continue;
+        for (int k = 0; k < 5; ++k) cands[nc][k] = -mixE[v];
+        ++nc;
+    }
     // z pass with one thread group per component (code 100 * TX + E)
     const int zsplit[4] = {408, 404, 416, 808};
     for (int v = 0; v < 4; ++v) {
