@@ -668,13 +668,14 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     for (int k = 0; k < 5; ++k)
         reg[k] = em[k] < 0 && (k == 2 ? reg_z_supported<T>(Pp[k], -em[k]) : reg_supported<T>(Pp[k], -em[k]));
     // register-engine points per thread of each pass; an x or y pass whose
-    // mixed-radix side (192, 224) has no plan for the requested count takes
-    // that side's own (12 and 14 points: the tuner's usual choice)
+    // mixed-radix side (96, 112, 160, 192, 224) has no plan for the requested
+    // count takes that side's own (12, 14, 20 points)
     int rc[5];
     for (int k = 0; k < 5; ++k) {
         rc[k] = -em[k];
         if (k != 2 && em[k] < 0 && !reg[k]) {
-            const int mix = Pp[k] == 192 ? 12 : Pp[k] == 224 ? 14 : 0;
+            const int mix = (Pp[k] == 96 || Pp[k] == 192) ? 12 : (Pp[k] == 112 || Pp[k] == 224) ? 14
+                          : Pp[k] == 160 ? 20 : 0;
             if (mix && reg_supported<T>(Pp[k], mix)) {
                 rc[k] = mix;
                 reg[k] = true;
@@ -888,12 +889,12 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
     for (int v = 0; v < 3; ++v)
         for (int k = 0; k < 5; ++k) cands[nc + v][k] = -regE[v];
     nc += 3;
-    // register-engine plans of the mixed-radix sides (192: 12 or 24 points
-    // per thread, 224: 14 or 28); a pass whose side has no such plan runs its
-    // LDS-stage fallback in these candidates
-    const int mixE[4] = {12, 24, 14, 28};
+    // register-engine plans of the mixed-radix sides (96, 192: 12 or 24
+    // points per thread; 112, 224: 14 or 28; 160: 10 or 20); a pass whose
+    // side has no such plan runs its default in these candidates
+    const int mixE[6] = {12, 24, 14, 28, 10, 20};
     const int Pp3[3] = {h->Px, h->Py, h->Pz};
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < 6 && nc < 16; ++v) {
         bool any = false;
         for (int a = 0; a < 3; ++a)
             any = any || (f64 ? reg_supported<double>(Pp3[a], mixE[v]) : reg_supported<float>(Pp3[a], mixE[v]));

@@ -268,22 +268,31 @@ def test_z_split_variants(prec, shape):
         assert _rel(op.apply(x, L_.OP_K), yk) <= tol, ('K', code)
 
 
+MIXED = [
+    ((112, 96, 10), ([-14, -12, 8, -12, -14], [-28, -24, 8, -24, -28], [-28, -12, 8, -24, -14])),  # 224, 192
+    ((48, 56, 10), ([-12, -14, 8, -14, -12], [-24, -28, 8, -28, -24], [-24, -14, 8, -28, -12])),    # 96, 112
+    ((80, 48, 10), ([-10, -12, 8, -12, -10], [-20, -24, 8, -24, -20])),                             # 160, 96
+]
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_mixed_radix_register_engine(prec):
-    """Register-engine x and y kernels on mixed-radix sides (224 = 2^5 * 7 with
-    14 or 28 points per thread, 192 = 2^6 * 3 with 12 or 24) give the oracle's
-    products, both directions; the z side (20) runs on the LDS-stage engine."""
+@pytest.mark.parametrize("shape,codeset", MIXED)
+def test_mixed_radix_register_engine(prec, shape, codeset):
+    """Register-engine x and y kernels on mixed-radix sides (96, 192 = 2^k * 3
+    with 12 or 24 points per thread; 112, 224 = 2^k * 7 with 14 or 28; 160 =
+    2^5 * 5 with 10 or 20) give the oracle's products, both directions; the z
+    side (20) runs on the LDS-stage engine."""
     from marie_amd import lib as L_
-    L, M, N = 112, 96, 10
-    print("case mixed radix register engine", prec, flush=True)
+    L, M, N = shape
+    print("case mixed radix register engine", prec, shape, flush=True)
     p = L_.PREC_F64 if prec == "f64" else L_.PREC_F32
     tol = TOL64 if prec == "f64" else TOL32
     fN, Mr, Mc, idx, x = _case(L, M, N, frac=0.3, seed=51)
     op = _op(L, M, N, None, fN, Mr, Mc, 0.7, idx, prec=p)
-    assert op.P == (224, 192, 20)
+    assert op.P == (2 * L, 2 * M, 2 * N)
     yr = mvp.mv_AN(x, fN, Mr, Mc, 0.7, 'notransp', idx)
     yt = mvp.mv_AN(x, fN, Mr, Mc, 0.7, 'transp', idx)
-    for codes in ([-14, -12, 8, -12, -14], [-28, -24, 8, -24, -28], [-28, -12, 8, -24, -14]):
+    for codes in codeset:
         op.set_tiles(codes + [0], p)
         assert _rel(op.apply(x), yr) <= tol, codes
         assert _rel(op.apply(x, L_.OP_AN, True), yt) <= tol, codes
