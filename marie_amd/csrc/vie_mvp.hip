@@ -667,16 +667,17 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     const int Pp[5] = {Px, Py, Pz, Py, Px};
     for (int k = 0; k < 5; ++k)
         reg[k] = em[k] < 0 && (k == 2 ? reg_z_supported<T>(Pp[k], -em[k]) : reg_supported<T>(Pp[k], -em[k]));
-    // register-engine points per thread of each pass; an x or y pass whose
+    // register-engine points per thread of each pass; a pass whose
     // mixed-radix side (96, 112, 160, 192, 224) has no plan for the requested
-    // count takes that side's own (12, 14, 20 points)
+    // count takes that side's own (12, 14, 20 points; z: 12, 10, and 14 in
+    // complex64, where its three components fit in registers)
     int rc[5];
     for (int k = 0; k < 5; ++k) {
         rc[k] = -em[k];
-        if (k != 2 && em[k] < 0 && !reg[k]) {
+        if (em[k] < 0 && !reg[k]) {
             const int mix = (Pp[k] == 96 || Pp[k] == 192) ? 12 : (Pp[k] == 112 || Pp[k] == 224) ? 14
-                          : Pp[k] == 160 ? 20 : 0;
-            if (mix && reg_supported<T>(Pp[k], mix)) {
+                          : Pp[k] == 160 ? (k == 2 ? 10 : 20) : 0;
+            if (mix && (k == 2 ? reg_z_supported<T>(Pp[k], mix) : reg_supported<T>(Pp[k], mix))) {
                 rc[k] = mix;
                 reg[k] = true;
             }

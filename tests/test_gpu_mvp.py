@@ -296,3 +296,50 @@ def test_mixed_radix_register_engine(prec, shape, codeset):
         op.set_tiles(codes + [0], p)
         assert _rel(op.apply(x), yr) <= tol, codes
         assert _rel(op.apply(x, L_.OP_AN, True), yt) <= tol, codes
+
+
+@pytest.mark.parametrize("prec,shape,zcode", [
+    ("f64", (10, 12, 80), 10),      # z side 160 = 2^5 * 5
+    ("f64", (10, 12, 96), 12),      # z side 192 = 2^6 * 3
+    ("f32", (10, 12, 112), 14),     # z side 224 = 2^5 * 7 (complex64 only)
+    ("f32", (10, 12, 48), 12),      # z side 96
+])
+def test_mixed_radix_z(prec, shape, zcode):
+    """The register-engine z kernel on mixed-radix sides gives the oracle's N
+    products (two right-hand sides, both directions) and K product; an untuned
+    handle takes the same plan by default."""
+    from marie_amd import lib as L_
+    L, M, N = shape
+    print("case mixed radix z", prec, shape, zcode, flush=True)
+    p = L_.PREC_F64 if prec == "f64" else L_.PREC_F32
+    tol = TOL64 if prec == "f64" else TOL32
+    rng = np.random.default_rng(61)
+    G = rng.standard_normal((L, M, N, 6)) + 1j * rng.standard_normal((L, M, N, 6))
+    for c in range(6):
+        if oop.GP_COEFF['L'][c] < 0:
+            G[0, :, :, c] = 0
+        if oop.GP_COEFF['M'][c] < 0:
+            G[:, 0, :, c] = 0
+        if oop.GP_COEFF['N'][c] < 0:
+            G[:, :, 0, c] = 0
+    fN = oop.fft_operator(oop.circulant_nop(G))
+    _, Mr, Mc, idx, x = _case(L, M, N, frac=0.5, seed=62)
+    x2 = np.stack([x, rng.standard_normal(x.size) + 1j * rng.standard_normal(x.size)], axis=1)
+    op = _op(L, M, N, None, fN, Mr, Mc, 0.8, idx, prec=p)
+    yr = [mvp.mv_AN(x2[:, q], fN, Mr, Mc, 0.8, 'notransp', idx) for q in range(2)]
+    yt = mvp.mv_AN(x, fN, Mr, Mc, 0.8, 'transp', idx)
+    for cfg in ([8, 8, -zcode, 8, 8, 0], None):
+        if cfg is not None:
+            op.set_tiles(cfg, p)
+        y = op.apply(x2)
+        for q in range(2):
+            assert _rel(y[:, q], yr[q]) <= tol, (cfg, q)
+        assert _rel(op.apply(x, L_.OP_AN, True), yt) <= tol, cfg
+        op = _op(L, M, N, None, fN, Mr, Mc, 0.8, idx, prec=p)     # a fresh, untuned handle next
+    ko, dx = 2 * np.pi / 299792458.0 * 298e6, 6.25e-3
+    fK = oop.fft_operator(oop.circulant_kop(oop.assembly_kop(L, M, N, ko, dx)))
+    opk = L_.Operator(L, M, N, precisions=p)
+    opk.set_fK(fK)
+    opk.set_index(idx)
+    opk.set_tiles([8, 8, -zcode, 8, 8, 0], p)
+    assert _rel(opk.apply(x, L_.OP_K), mvp.mv_K(x, fK, L, M, N, idx)) <= tol
