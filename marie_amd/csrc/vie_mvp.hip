@@ -669,13 +669,15 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
         reg[k] = em[k] < 0 && (k == 2 ? reg_z_supported<T>(Pp[k], -em[k]) : reg_supported<T>(Pp[k], -em[k]));
     // register-engine points per thread of each pass; a pass whose
     // mixed-radix side (96, 112, 160, 192, 224) has no plan for the requested
-    // count takes that side's own (12, 14, 20 points; z: 12, 10, and 14 in
-    // complex64, where its three components fit in registers)
+    // count takes that side's own (12, 14, 20 points; z: 12 and 10 where its
+    // three components fit in registers, at 224 one thread group per
+    // component with 14 points, code 414)
     int rc[5];
     for (int k = 0; k < 5; ++k) {
         rc[k] = -em[k];
         if (em[k] < 0 && !reg[k]) {
-            const int mix = (Pp[k] == 96 || Pp[k] == 192) ? 12 : (Pp[k] == 112 || Pp[k] == 224) ? 14
+            const int mix = (Pp[k] == 96 || Pp[k] == 192) ? 12
+                          : Pp[k] == 224 ? (k == 2 ? 414 : 14) : Pp[k] == 112 ? 14
                           : Pp[k] == 160 ? (k == 2 ? 10 : 20) : 0;
             if (mix && (k == 2 ? reg_z_supported<T>(Pp[k], mix) : reg_supported<T>(Pp[k], mix))) {
                 rc[k] = mix;
@@ -904,8 +906,8 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
         ++nc;
     }
     // z pass with one thread group per component (code 100 * TX + E)
-    const int zsplit[4] = {408, 404, 416, 808};
-    for (int v = 0; v < 4; ++v) {
+    const int zsplit[5] = {408, 404, 416, 808, 414};
+    for (int v = 0; v < 5 && nc < 16; ++v) {
         if (!reg_z_supported<double>(h->Pz, zsplit[v]) && f64) continue;
         if (!f64 && !reg_z_supported<float>(h->Pz, zsplit[v])) continue;
         for (int k = 0; k < 5; ++k) cands[nc][k] = -8;

@@ -303,6 +303,8 @@ def test_mixed_radix_register_engine(prec, shape, codeset):
     ("f64", (10, 12, 96), 12),      # z side 192 = 2^6 * 3
     ("f32", (10, 12, 112), 14),     # z side 224 = 2^5 * 7 (complex64 only)
     ("f32", (10, 12, 48), 12),      # z side 96
+    ("f64", (10, 12, 112), 414),    # z side 224: one thread group per component, 14 points, 4 columns
+    ("f32", (10, 12, 112), 414),
 ])
 def test_mixed_radix_z(prec, shape, zcode):
     """The register-engine z kernel on mixed-radix sides gives the oracle's N
