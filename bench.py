@@ -176,8 +176,8 @@ def main():
         print(json.dumps({'copy_bytes': a.numel() * 16, 'copies': 5}))
         return
     if args.gpus != 1:
-        print('bench.py measures one GPU: this build has no multi-GPU path (the sharding design is '
-              'described in DESIGN.md); run with --gpus 1', file=sys.stderr)
+        print('bench.py measures one GPU: the package runs one handle per GPU and has no multi-GPU path; '
+              'run with --gpus 1', file=sys.stderr)
         sys.exit(2)
 
     import torch
