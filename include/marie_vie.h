@@ -304,7 +304,9 @@ int marie_vie_dgf_fields_device(const double* seg_center_dev, const void* seg_J_
  * the fastest per kernel for precision `prec`. A variant is cfg6[0..4], one
  * value per kernel (fwd_x, fwd_y, z, inv_y, inv_x): v > 0 selects the
  * LDS-stage FFT with v elements per thread, v < 0 the register-resident FFT
- * with -v points per thread (power-of-two sides); for the z kernel,
+ * with -v points per thread (power-of-two sides 32-512 with 4, 8 or 16
+ * points; mixed-radix sides 96/192 with 12 or 24, 112/224 with 14 or 28, 160
+ * with 10 or 20 -- z: 12, 10, and 14 in complex64); for the z kernel,
  * v = -(100*TX + E) selects the variant with one thread group per field
  * component (TX columns per workgroup, E points per thread, the three
  * spectra exchanged through LDS for the tensor product); cfg6[5] is reserved and
@@ -315,8 +317,10 @@ int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* cfg6, double* 
 
 /* Set the kernel variants explicitly (cfg6 as above). A new handle starts at
  * {-4, -16, -8, -8, -4} for FP64 and {-8, -16, -16, -16, -8} for FP32 (the
- * autotuner's usual choice at 256-point sides); a kernel whose side the
- * register engine does not cover takes the LDS-stage FFT instead. */
+ * autotuner's usual choice at 256-point sides); on a mixed-radix side a
+ * register code takes that side's own plan (12, 14 or 20 points; the z
+ * kernel at 224 one thread group per component, -414), and a kernel whose
+ * side the register engine does not cover takes the LDS-stage FFT. */
 int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* cfg6);
 
 /* Per-kernel timing with HIP events on the handle's stream. When enabled,
