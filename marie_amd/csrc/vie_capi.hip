@@ -409,6 +409,23 @@ extern "C" int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* cfg6) {
         set_error("invalid tile arguments");
         return MARIE_ERR_ARG;
     }
+    // accepted codes: the LDS-stage engine's elements per thread (positive;
+    // a value between compiled variants runs the next larger one), the
+    // register engine's points per thread (negative) and, for the z pass, its
+    // one-group-per-component codes -(100 * TX + E)
+    static const int kReg[] = {4, 8, 16, 10, 12, 14, 20, 24, 28};
+    static const int kZ3[] = {404, 408, 416, 804, 808, 816, 414};
+    for (int k = 0; k < 5; ++k) {
+        const int v = cfg6[k];
+        bool ok = v == 4 || v == 8 || v == 16 || v == 32;
+        for (int r : kReg) ok = ok || v == -r;
+        if (k == 2)
+            for (int r : kZ3) ok = ok || v == -r;
+        if (!ok) {
+            set_error("tile code " + std::to_string(v) + " of pass " + std::to_string(k) + " is not a compiled variant");
+            return MARIE_ERR_ARG;
+        }
+    }
     int* em = prec == MARIE_PREC_F64 ? h->emax64 : h->emax32;
     for (int k = 0; k < 5; ++k) em[k] = cfg6[k];
     return MARIE_OK;

@@ -583,6 +583,18 @@ static int effective_emax(int emax, const AxisPlan& pl, int dflt) {
     return plan_rset(pl) == RSET_ALL ? dflt : emax;
 }
 
+// The elements per thread that MARIE_EMAX_XY / MARIE_EMAX_Z dispatch for a
+// requested value: the tile size is derived from this, never from the raw
+// request, so that the tile always fits the instantiated kernel.
+static int dispatched_emax(bool f64, bool zpass, int e, int rs) {
+    if (zpass) {
+        if (f64) return (e == 4 && rs < 3) ? 4 : 8;
+        return (e == 8 && rs < 3) ? 8 : 16;
+    }
+    if (f64) return (e == 4 && rs < 3) ? 4 : (e == 8 && rs < 3) ? 8 : 16;
+    return (e == 8 && rs < 3) ? 8 : (e == 16 && rs < 3) ? 16 : 32;
+}
+
 // Preconditioner material of JVIE_Solver.m:278, :297 (mv_AN called with
 // 1./Mr and 1./Mr - 1), derived from Mr in FP64, rounded once for complex64.
 __global__ void k_precond_material(const c2<double>* __restrict__ Mr, c2<double>* __restrict__ Pr,
@@ -599,6 +611,19 @@ __global__ void k_precond_material(const c2<double>* __restrict__ Mr, c2<double>
             Pr32[v] = c2<float>{(float)r.x, (float)r.y};
             Pc32[v] = c2<float>{(float)c.x, (float)c.y};
         }
+    }
+}
+
+// The same material for a handle without FP64 data: derived from the complex64
+// Mr, computed in FP64 and rounded once.
+__global__ void k_precond_material32(const c2<float>* __restrict__ Mr, c2<float>* __restrict__ Pr32,
+                                     c2<float>* __restrict__ Pc32, int64_t n) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        const double mx = Mr[v].x, my = Mr[v].y;
+        const double d = mx * mx + my * my;
+        const double rx = mx / d, ry = -my / d;
+        Pr32[v] = c2<float>{(float)rx, (float)ry};
+        Pc32[v] = c2<float>{(float)(rx - 1.0), (float)ry};
     }
 }
 
@@ -623,17 +648,25 @@ static int ensure_precond(Op* h, hipStream_t s) {
         }
         return OK;
     };
-    int st = A(&h->Pr64, n * sizeof(c2<double>));
-    if (!st) st = A(&h->Pc64, n * sizeof(c2<double>));
+    // the FP64 material exists only on handles created with MARIE_PREC_F64
+    const bool f64 = h->precisions & PREC_F64;
     const bool f32 = h->precisions & PREC_F32;
+    int st = OK;
+    if (f64) st = A(&h->Pr64, n * sizeof(c2<double>));
+    if (!st && f64) st = A(&h->Pc64, n * sizeof(c2<double>));
     if (!st && f32) st = A(&h->Pr32, n * sizeof(c2<float>));
     if (!st && f32) st = A(&h->Pc32, n * sizeof(c2<float>));
     if (st) return st;
     int64_t g = ((int64_t)n + 255) / 256;
     if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(k_precond_material, dim3((unsigned)g), dim3(256), 0, s, (const c2<double>*)h->Mr64,
-                       (c2<double>*)h->Pr64, (c2<double>*)h->Pc64, f32 ? (c2<float>*)h->Pr32 : nullptr,
-                       f32 ? (c2<float>*)h->Pc32 : nullptr, (int64_t)n);
+    if (g < 1) g = 1;
+    if (f64)
+        hipLaunchKernelGGL(k_precond_material, dim3((unsigned)g), dim3(256), 0, s, (const c2<double>*)h->Mr64,
+                           (c2<double>*)h->Pr64, (c2<double>*)h->Pc64, f32 ? (c2<float>*)h->Pr32 : nullptr,
+                           f32 ? (c2<float>*)h->Pc32 : nullptr, (int64_t)n);
+    else
+        hipLaunchKernelGGL(k_precond_material32, dim3((unsigned)g), dim3(256), 0, s, (const c2<float>*)h->Mr32,
+                           (c2<float>*)h->Pr32, (c2<float>*)h->Pc32, (int64_t)n);
     MARIE_HIP(hipGetLastError());
     h->have_P = true;
     return OK;
@@ -697,7 +730,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     for (int k = 0; k < 5; ++k) {
         int want = em[k] > 0 ? em[k] : (f64 ? fb64[k] : fb32[k]);
         if (em[k] <= 0 && Pp[k] > want * NT) want = dflt[k];
-        e[k] = effective_emax(want, *apl[k], dflt[k]);
+        e[k] = dispatched_emax(f64, k == 2, effective_emax(want, *apl[k], dflt[k]), plan_rset(*apl[k]));
         if (!reg[k] && Pp[k] > e[k] * NT) {
             set_error("circulant side exceeds the single-tile FFT limit of the selected tile variant");
             return ERR_SIZE;

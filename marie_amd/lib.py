@@ -129,6 +129,20 @@ def _cplx(a):
     return np.ascontiguousarray(a.ravel(order='F'))
 
 
+def _vec_arg(b, x0, nidx, name='b'):
+    """b and x0 (or zeros) as contiguous complex128 vectors of exactly nidx
+    entries: the C entry points read and write nidx values through them."""
+    b = np.ascontiguousarray(np.asarray(b, np.complex128).ravel())
+    if b.size != nidx:
+        raise ValueError('%s must hold nidx = %d entries (got %d)' % (name, nidx, b.size))
+    if x0 is None:
+        return b, np.zeros_like(b)
+    x = np.ascontiguousarray(np.asarray(x0, np.complex128).ravel()).copy()
+    if x.size != nidx:
+        raise ValueError('x0 must hold nidx = %d entries (got %d)' % (nidx, x.size))
+    return b, x
+
+
 def dgf_fields(seg_center, seg_J, port_start, points, ko, omega_mu, device=0):
     """E, H (nport x npoints x 3 complex) of current elements (eval_DGF.m) on the GPU.
 
@@ -308,6 +322,8 @@ class Operator:
     def apply(self, x, op=OP_AN, transp=False):
         """Host-to-host product (complex128); x is nidx or nidx x nrhs (columns)."""
         x = np.asarray(x, np.complex128)
+        if x.ndim not in (1, 2) or x.shape[0] != self.nidx:
+            raise ValueError('x must be nidx = %d long (or nidx x nrhs); got shape %s' % (self.nidx, x.shape))
         nrhs = 1 if x.ndim == 1 else x.shape[1]
         xb = np.ascontiguousarray(x.T if x.ndim == 2 else x)
         y = np.empty_like(xb)
@@ -375,8 +391,12 @@ class Operator:
         B = np.asarray(B, np.complex128)
         if B.ndim == 1:
             B = B[:, None]
+        if B.ndim != 2 or B.shape[0] != self.nidx:
+            raise ValueError('B must be nidx = %d x nrhs; got shape %s' % (self.nidx, B.shape))
         nrhs = B.shape[1]
         Bc = np.ascontiguousarray(B.T)
+        if X0 is not None and np.asarray(X0).size != B.size:
+            raise ValueError('X0 must hold nidx x nrhs = %d entries' % B.size)
         Xc = (np.zeros_like(Bc) if X0 is None else
               np.ascontiguousarray(np.asarray(X0, np.complex128).reshape(B.shape).T).copy())
         flag = (ctypes.c_int * nrhs)()
@@ -393,8 +413,7 @@ class Operator:
 
     def gmres(self, b, x0=None, restart=50, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4, resvec_cap=20000):
         """Host-buffer GMRES; returns (x, flag, relres, [inner, outer], resvec)."""
-        b = np.ascontiguousarray(np.asarray(b, np.complex128))
-        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(np.asarray(x0, np.complex128)).copy()
+        b, x = _vec_arg(b, x0, self.nidx)
         flag = ctypes.c_int()
         relres = ctypes.c_double()
         it = (ctypes.c_int * 2)()
@@ -408,8 +427,7 @@ class Operator:
     def gmres_precond(self, b, x0=None, restart=50, tol=1e-6, maxit=200, side=PRECOND_RIGHT, resvec_cap=20000):
         """Host-buffer preconditioned GMRES (pgmres with R1 or L1 = the JVIE
         preconditioner); returns (x, flag, relres, [inner, outer], resvec)."""
-        b = np.ascontiguousarray(np.asarray(b, np.complex128))
-        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(np.asarray(x0, np.complex128)).copy()
+        b, x = _vec_arg(b, x0, self.nidx)
         flag = ctypes.c_int()
         relres = ctypes.c_double()
         it = (ctypes.c_int * 2)()
@@ -437,8 +455,7 @@ class Operator:
                  resvec_cap=20000):
         """Host-buffer GMRES with deflated restarts (pgmresDR); returns
         (x, flag, relres, [inner, outer], resvec)."""
-        b = np.ascontiguousarray(np.asarray(b, np.complex128))
-        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(np.asarray(x0, np.complex128)).copy()
+        b, x = _vec_arg(b, x0, self.nidx)
         flag = ctypes.c_int()
         relres = ctypes.c_double()
         it = (ctypes.c_int * 2)()
@@ -465,8 +482,7 @@ class Operator:
         """Host-buffer BiCGStab (MATLAB bicgstab semantics); returns
         (x, flag, relres, iter, resvec) with iter in half steps / 2 and resvec
         the absolute residual norms of every half step."""
-        b = np.ascontiguousarray(np.asarray(b, np.complex128))
-        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(np.asarray(x0, np.complex128)).copy()
+        b, x = _vec_arg(b, x0, self.nidx)
         flag = ctypes.c_int()
         relres = ctypes.c_double()
         it = ctypes.c_double()

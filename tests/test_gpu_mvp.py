@@ -163,7 +163,12 @@ def test_error_paths():
     with pytest.raises(L_.MarieError):
         op.set_index(np.array([3 * 64]))         # out of range
     with pytest.raises(L_.MarieError):
-        op.apply(np.ones(1))                     # operator not set
+        op.apply(np.ones(0))                     # operator and index not set
+    op.set_index(np.array([0, 5, 9], np.int64))
+    with pytest.raises(ValueError):
+        op.apply(np.ones(2))                     # x shorter than the index: refused before the C call
+    with pytest.raises(ValueError):
+        op.gmres(np.ones(4))
     op.set_index(np.array([], np.int64))
     fN = np.ones((8, 8, 8, 6), complex)
     op.set_fN(fN)
@@ -345,3 +350,26 @@ def test_mixed_radix_z(prec, shape, zcode):
     opk.set_index(idx)
     opk.set_tiles([8, 8, -zcode, 8, 8, 0], p)
     assert _rel(opk.apply(x, L_.OP_K), mvp.mv_K(x, fK, L, M, N, idx)) <= tol
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_tile_request_between_variants_and_invalid_codes(prec):
+    """A positive tile request above the compiled variants (32 elements per
+    thread in FP64 on a 512-point side) runs the largest compiled variant with
+    a tile that fits it, and gives the oracle's product; codes that name no
+    compiled variant are refused."""
+    from marie_amd import lib as L_
+    L, M, N = 256, 6, 5
+    print("case tile request between variants", prec, flush=True)
+    p = L_.PREC_F64 if prec == "f64" else L_.PREC_F32
+    tol = TOL64 if prec == "f64" else TOL32
+    fN, Mr, Mc, idx, x = _case(L, M, N, frac=0.4, seed=71)
+    op = _op(L, M, N, None, fN, Mr, Mc, 0.9, idx, prec=p)
+    assert op.P == (512, 12, 10)
+    yr = mvp.mv_AN(x, fN, Mr, Mc, 0.9, 'notransp', idx)
+    for cfg in ([32, 32, 16, 32, 32], [16, 16, 8, 16, 16], [4, 8, 4, 8, 4]):
+        op.set_tiles(cfg, p)
+        assert _rel(op.apply(x), yr) <= tol, cfg
+    for bad in ([3, 8, 8, 8, 8], [8, 8, -414 - 1, 8, 8], [8, -404, 8, 8, 8], [0, 8, 8, 8, 8]):
+        with pytest.raises(L_.MarieError):
+            op.set_tiles(bad, p)

@@ -62,6 +62,27 @@ def test_preconditioner_operator(form):
     assert _rel(y, yr) <= 1e-12
 
 
+def test_preconditioner_operator_complex64_only_handle():
+    """OP_P on a handle created with complex64 only (no FP64 material on the
+    device): the preconditioner's material is derived from the complex64 Mr,
+    and the product matches the oracle's FP64 mv_AN(., 1./Mr, 1./Mr - 1) to
+    complex64 accuracy, in both directions."""
+    from marie_amd import lib
+    print("case OP_P complex64-only handle", flush=True)
+    fN, Mr, Mc, Gram, idx, b = _problem(form=1)
+    n = Mr.shape[0]
+    op = lib.Operator(n, n, n, precisions=lib.PREC_F32)
+    op.set_fN(fN)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    for transp in ('notransp', 'transp'):
+        y = op.apply(x, lib.OP_P, transp=transp == 'transp')
+        yr = mvp.mv_AN(x, fN, 1.0 / Mr, 1.0 / Mr - 1.0, Gram, transp, idx)
+        assert _rel(y, yr) <= 2e-6, transp
+
+
 @pytest.mark.parametrize("side,form", [('R', 1), ('L', 1), ('R', 2), ('L', 2)])
 def test_preconditioned_gmres_parity(side, form):
     from marie_amd import lib
