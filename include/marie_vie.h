@@ -22,6 +22,7 @@
 #ifndef MARIE_VIE_H
 #define MARIE_VIE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -127,6 +128,23 @@ int marie_vie_apply(marie_vie_op* h, int op, int transp, const double* x, double
 /* Same on device buffers of precision `prec` (MARIE_PREC_F64 or _F32). */
 int marie_vie_apply_device(marie_vie_op* h, int op, int transp, int prec, const void* x_dev,
                            void* y_dev, int nrhs);
+
+/* marie_vie_apply_device without any allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc). The
+ * product's work buffers live in one file-scope device array per GPU of
+ * marie_vie_noalloc_scratch_bytes() bytes (128 MiB), held until the process
+ * ends; a product whose buffers, 16 (complex128) or 8 (complex64) bytes x
+ * 3 nrhs Px N (M + Py) plus alignment, exceed it returns MARIE_ERR_SIZE.
+ * Same kernels and results as marie_vie_apply_device; no timing events.
+ * MARIE_OP_P needs Mr = 1 (form II) or its material derived by an earlier
+ * MARIE_OP_P product of marie_vie_apply_device, else MARIE_ERR_STATE. Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU race
+ * (serialise them, e.g. on one stream). The array needs no clearing: each
+ * product writes every element it reads. */
+int marie_vie_apply_device_noalloc(marie_vie_op* h, int op, int transp, int prec, const void* x_dev,
+                                   void* y_dev, int nrhs);
+size_t marie_vie_noalloc_scratch_bytes(void);
 
 /* Restarted GMRES(restart) on A_N with MARIE's pgmres semantics
  * (src_numeric/src_iterative/pgmres.m:70-181, pgmres_iter.m:22-150):

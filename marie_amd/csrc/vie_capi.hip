@@ -103,6 +103,7 @@ int marie_vie_create(marie_vie_op** out, int L, int M, int N, int Px, int Py, in
         st = MARIE_ERR_HIP;
     }
     if (st == OK) h->own_stream = true;
+    if (st == OK) st = scratch_init(device);     // the allocation-free product's per-device array
     if (st == OK) st = upload_twiddles(h);
     for (int k = 0; k < 8 && st == OK; ++k)
         if (hipEventCreate(&h->ev[k]) != hipSuccess) { set_error("hipEventCreate failed"); st = MARIE_ERR_HIP; }
@@ -305,6 +306,30 @@ int marie_vie_apply_device(marie_vie_op* h, int op, int transp, int prec, const 
     if (h->nidx == 0) return MARIE_OK;
     return apply(h, op, transp ? 1 : 0, prec, x, y, nrhs, h->stream);
 }
+
+int marie_vie_apply_device_noalloc(marie_vie_op* h, int op, int transp, int prec, const void* x, void* y,
+                                   int nrhs) {
+    CHECK_H(h);
+    if (op < MARIE_OP_AN || op > MARIE_OP_P || (prec != MARIE_PREC_F64 && prec != MARIE_PREC_F32) || nrhs < 1 ||
+        (h->nidx > 0 && (!x || !y))) {
+        set_error("invalid apply arguments");
+        return MARIE_ERR_ARG;
+    }
+    if (noalloc_bytes(h, prec, nrhs) > scratch_capacity()) {
+        set_error("the product's work buffers (" + std::to_string(noalloc_bytes(h, prec, nrhs)) +
+                  " bytes) exceed the allocation-free scratch of " + std::to_string(scratch_capacity()) + " bytes");
+        return MARIE_ERR_SIZE;
+    }
+    if (!(op == MARIE_OP_K ? h->have_fK : h->have_fN) || !h->have_idx ||
+        ((op == MARIE_OP_AN || op == MARIE_OP_P) && !h->have_mat)) {
+        set_error("operator, index or material not set");
+        return MARIE_ERR_STATE;
+    }
+    if (h->nidx == 0) return MARIE_OK;
+    return apply_noalloc(h, op, transp ? 1 : 0, prec, x, y, nrhs, h->stream);
+}
+
+size_t marie_vie_noalloc_scratch_bytes(void) { return scratch_capacity(); }
 
 int marie_vie_apply(marie_vie_op* h, int op, int transp, const double* x, double* y, int nrhs) {
     CHECK_H(h);

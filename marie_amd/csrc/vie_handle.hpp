@@ -103,6 +103,11 @@ struct Op {
 
 int ensure_work(Op* h, int nrhs);
 int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s);
+// allocation-free product on the per-device scratch array (vie_mvp.hip)
+int scratch_init(int device);
+size_t scratch_capacity();
+size_t noalloc_bytes(const Op* h, int prec, int nrhs);
+int apply_noalloc(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s);
 int build_plan(int P, AxisPlan* pl);
 int upload_twiddles(Op* h);
 int store_fN(Op* h, void* full64);   // takes ownership of a device complex128 fN

@@ -19,6 +19,7 @@
 #include "vie_reg.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -672,8 +673,13 @@ static int ensure_precond(Op* h, hipStream_t s) {
     return OK;
 }
 
+// One product on work buffers Abuf ([3 nrhs][N][M][Px]) and Bbuf
+// ([3 nrhs][N][Py][Px]); every element a pass reads was written by an earlier
+// pass of the same product, so the buffers need no clearing. `timed`: record
+// the per-kernel events when the handle's timing is on.
 template <typename T>
-static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs, hipStream_t s) {
+static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs, hipStream_t s, void* Abuf,
+                   void* Bbuf, bool timed) {
     const bool f64 = sizeof(T) == 8;
     const bool kop = (op == OP_K);
     const c2<T>* fN = reinterpret_cast<const c2<T>*>(kop ? (f64 ? h->fK64 : h->fK32) : (f64 ? h->fN64 : h->fN32));
@@ -687,8 +693,8 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     const c2<T>* twx = tw;
     const c2<T>* twy = tw + h->Px;
     const c2<T>* twz = tw + h->Px + h->Py;
-    c2<T>* A = reinterpret_cast<c2<T>*>(h->A);
-    c2<T>* B = reinterpret_cast<c2<T>*>(h->B);
+    c2<T>* A = reinterpret_cast<c2<T>*>(Abuf);
+    c2<T>* B = reinterpret_cast<c2<T>*>(Bbuf);
     const int L = h->L, M = h->M, N = h->N, Px = h->Px, Py = h->Py, Pz = h->Pz;
     const int ncomp = 3 * nrhs;
     const int* em = f64 ? h->emax64 : h->emax32;
@@ -737,7 +743,7 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
         }
     }
     const int64_t nline = (int64_t)M * N;
-    const bool tm = h->timing;
+    const bool tm = timed && h->timing;
     hipEvent_t* tev = h->ring[h->ring_used];
     if (tm) {
         // timing-only events: no system-scope fence (cache writeback and
@@ -847,7 +853,8 @@ int timing_flush(Op* h) {
     return OK;
 }
 
-int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s) {
+// state and argument checks shared by both product entry points
+static int apply_check(Op* h, int op, int transp, int prec, int nrhs) {
     const bool have_op = (op == OP_K) ? h->have_fK : h->have_fN;
     if (!have_op || !h->have_idx || ((op == OP_AN || op == OP_P) && !h->have_mat)) {
         set_error(op == OP_K ? "K operator or index not set" : "operator, index or material not set");
@@ -867,28 +874,105 @@ int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs,
         set_error("too many right-hand sides for one product on this grid");
         return ERR_ARG;
     }
+    return OK;
+}
+
+// y = Gram x on the compact vectors (OP_P when every Mr is 1)
+static int gram_scale(Op* h, int prec, const void* x, void* y, int nrhs, hipStream_t s) {
+    const int64_t n = h->nidx * nrhs;
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    if (prec == PREC_F64)
+        hipLaunchKernelGGL(k_gram_scale<double>, dim3((unsigned)g), dim3(256), 0, s, (const c2<double>*)x,
+                           (c2<double>*)y, h->Gram, n);
+    else
+        hipLaunchKernelGGL(k_gram_scale<float>, dim3((unsigned)g), dim3(256), 0, s, (const c2<float>*)x,
+                           (c2<float>*)y, (float)h->Gram, n);
+    MARIE_HIP(hipGetLastError());
+    return OK;
+}
+
+int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s) {
+    const int chk = apply_check(h, op, transp, prec, nrhs);
+    if (chk) return chk;
     if (op == OP_P) {
-        if (h->mr_unit) {
-            const int64_t n = h->nidx * nrhs;
-            int64_t g = (n + 255) / 256;
-            if (g > 4096) g = 4096;
-            if (g < 1) g = 1;
-            if (prec == PREC_F64)
-                hipLaunchKernelGGL(k_gram_scale<double>, dim3((unsigned)g), dim3(256), 0, s, (const c2<double>*)x,
-                                   (c2<double>*)y, h->Gram, n);
-            else
-                hipLaunchKernelGGL(k_gram_scale<float>, dim3((unsigned)g), dim3(256), 0, s, (const c2<float>*)x,
-                                   (c2<float>*)y, (float)h->Gram, n);
-            MARIE_HIP(hipGetLastError());
-            return OK;
-        }
+        if (h->mr_unit) return gram_scale(h, prec, x, y, nrhs, s);
         const int stp = ensure_precond(h, s);
         if (stp) return stp;
     }
     int st = ensure_work(h, nrhs);
     if (st) return st;
-    if (prec == PREC_F64) return apply_t<double>(h, op, transp, (const c2<double>*)x, (c2<double>*)y, nrhs, s);
-    return apply_t<float>(h, op, transp, (const c2<float>*)x, (c2<float>*)y, nrhs, s);
+    if (prec == PREC_F64)
+        return apply_t<double>(h, op, transp, (const c2<double>*)x, (c2<double>*)y, nrhs, s, h->A, h->B, true);
+    return apply_t<float>(h, op, transp, (const c2<float>*)x, (c2<float>*)y, nrhs, s, h->A, h->B, true);
+}
+
+// ------------------------------------------------- allocation-free product
+// Work buffers of marie_vie_apply_device_noalloc: one file-scope device array
+// per GPU, held until the process ends. The host name `scratch` is not a
+// device address; its address on each device comes from hipGetSymbolAddress
+// (scratch_init, called by marie_vie_create with that device current) and
+// only that pointer is passed to kernels. Every call on a device shares its
+// array, so calls that overlap on one GPU race.
+constexpr size_t kScratchBytes = (size_t)128 << 20;     // 128 MiB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device ordinal beyond the scratch table");
+        return ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return OK;
+    void* p = nullptr;
+    MARIE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+    return OK;
+}
+
+size_t scratch_capacity() { return kScratchBytes; }
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// bytes of A and B for one product (offset of B = align256 of A's bytes)
+size_t noalloc_bytes(const Op* h, int prec, int nrhs) {
+    const size_t es = prec == PREC_F64 ? sizeof(c2<double>) : sizeof(c2<float>);
+    const size_t na = (size_t)3 * nrhs * h->Px * h->M * h->N;
+    const size_t nb = (size_t)3 * nrhs * h->Px * h->Py * h->N;
+    return align256(na * es) + nb * es;
+}
+
+int apply_noalloc(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s) {
+    if (nrhs >= 1 && noalloc_bytes(h, prec, nrhs) > kScratchBytes) {
+        set_error("the product's work buffers (" + std::to_string(noalloc_bytes(h, prec, nrhs)) +
+                  " bytes) exceed the allocation-free scratch of " + std::to_string(kScratchBytes) + " bytes");
+        return ERR_SIZE;
+    }
+    const int chk = apply_check(h, op, transp, prec, nrhs);
+    if (chk) return chk;
+    if (op == OP_P) {
+        if (h->mr_unit) return gram_scale(h, prec, x, y, nrhs, s);
+        if (!h->have_P) {
+            set_error("the preconditioner's material is derived on the first MARIE_OP_P product of "
+                      "marie_vie_apply_device (it allocates); run one there before this entry point");
+            return ERR_STATE;
+        }
+    }
+    void* base = (h->device >= 0 && h->device < kMaxDevices) ? g_scratch_ptr[h->device] : nullptr;
+    if (!base) {
+        set_error("allocation-free scratch not initialised on this device");
+        return ERR_STATE;
+    }
+    const size_t es = prec == PREC_F64 ? sizeof(c2<double>) : sizeof(c2<float>);
+    void* A = base;
+    void* B = static_cast<unsigned char*>(base) + align256((size_t)3 * nrhs * h->Px * h->M * h->N * es);
+    if (prec == PREC_F64)
+        return apply_t<double>(h, op, transp, (const c2<double>*)x, (c2<double>*)y, nrhs, s, A, B, false);
+    return apply_t<float>(h, op, transp, (const c2<float>*)x, (c2<float>*)y, nrhs, s, A, B, false);
 }
 
 // Times every compiled tile variant of each pass with HIP events on the

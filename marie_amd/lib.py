@@ -47,6 +47,8 @@ SIGNATURES = [
     ('marie_vie_set_index', _I, [_P, _P, _I64, _I]),
     ('marie_vie_apply', _I, [_P, _I, _I, _P, _P, _I]),
     ('marie_vie_apply_device', _I, [_P, _I, _I, _I, _P, _P, _I]),
+    ('marie_vie_apply_device_noalloc', _I, [_P, _I, _I, _I, _P, _P, _I]),
+    ('marie_vie_noalloc_scratch_bytes', ctypes.c_size_t, []),
     ('marie_vie_gmres_device', _I, [_P, _P, _P, _I, _D, _I, _I, _D, ctypes.POINTER(_I), _DP,
                                     ctypes.POINTER(_I), _DP, _I, ctypes.POINTER(_I)]),
     ('marie_vie_gmres', _I, [_P, _P, _P, _I, _D, _I, _I, _D, ctypes.POINTER(_I), _DP,
@@ -336,6 +338,13 @@ class Operator:
         check(load().marie_vie_apply_device(self._h, op, 1 if transp else 0, prec, ctypes.c_void_p(x_ptr),
                                             ctypes.c_void_p(y_ptr), nrhs))
 
+    def apply_device_noalloc(self, x_ptr, y_ptr, op=OP_AN, transp=False, prec=PREC_F64, nrhs=1):
+        """apply_device without any allocation: the work buffers live in the
+        library's per-device scratch array (noalloc_scratch_bytes()); calls
+        that overlap on one GPU race."""
+        check(load().marie_vie_apply_device_noalloc(self._h, op, 1 if transp else 0, prec, ctypes.c_void_p(x_ptr),
+                                                    ctypes.c_void_p(y_ptr), nrhs))
+
     def gmres_device(self, b_ptr, x_ptr, restart=50, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4,
                      resvec_cap=20000):
         flag = ctypes.c_int()
@@ -530,6 +539,11 @@ class Operator:
         calls = ctypes.c_int64()
         check(load().marie_vie_get_timing(self._h, ms, ctypes.byref(calls)))
         return list(ms), calls.value
+
+
+def noalloc_scratch_bytes():
+    """Size of the per-device scratch array of apply_device_noalloc."""
+    return int(load().marie_vie_noalloc_scratch_bytes())
 
 
 def singular_terms(dx, ko):
