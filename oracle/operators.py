@@ -208,11 +208,18 @@ def singular_terms(ko, dx, method='own', ref=None):
 
     'own' uses this package's polar/pyramid quadrature (singular.py);
     'DIRECTFN'/'DEMCEM' call the reference's C++ compiled under oracle/_ref
-    (singular_st.m:12-20, singular_ea.m:61-76, singular_va.m:131-147).
+    (singular_st.m:12-20, singular_ea.m:61-76, singular_va.m:131-147);
+    a sequence of five values is used as given (e.g. a fixture's DIRECTFN
+    values).
     """
-    if method == 'own':
-        return singular.all_terms(ko, dx)
-    return ref.singular_terms(ko, dx, method)
+    if isinstance(method, str):
+        if method == 'own':
+            return singular.all_terms(ko, dx)
+        return ref.singular_terms(ko, dx, method)
+    vals = tuple(complex(v) for v in method)
+    if len(vals) != 5:
+        raise ValueError('five self-term values expected')
+    return vals
 
 
 def assembly_nop(L, M, N, ko, dx, method='own', ref=None):

@@ -10,6 +10,12 @@ Run in a container that has the reference tree and oracle/_ref built
 and tests/golden/ref_cubature_kop.npz:
   * cubature_kop 3-vectors (cubature_kop.cpp) at the orders 4/10 of
     assembly_kop.m:32-66 for offsets without face contact.
+and tests/golden/ref_singular.npz:
+  * the five self-term integrals from DIRECTFN and DEMCEM at Np = 10, 20, 30
+    for the four (dx, f) cases (getOPERATORS.m:40-42 defaults to DIRECTFN;
+    singular_st.m:13-19, singular_ea.m:16-29, singular_va.m);
+  * the C and P tables of nearby_correct.m:5-133, per offset: C as codes
+    0 = zero, 1 = ST, 2 = EAc, 3 = EAo, 4 = VAc, 5 = VAo, P as 0/1.
 """
 import os
 import re
@@ -41,8 +47,35 @@ def gauss_table(order, path='/root/reference/src_vie/src_operators/cubatures/gau
     return np.array([xs[i + 1] for i in range(n)]), np.array([ws[i + 1] for i in range(n)])
 
 
+def nearby_tables_from_reference(path='/root/reference/src_vie/src_operators/nearby_correct.m'):
+    """(offsets 8 x 3 one-based, C codes 8 x 6 x 6, P 8 x 6 x 6) parsed from nearby_correct.m."""
+    txt = open(path).read()
+    names = {'0.0': 0, 'ST': 1, 'EAc': 2, 'EAo': 3, 'VAc': 4, 'VAo': 5}
+    offs, cc, pp = [], [], []
+    for b in re.split(r'isequal\(m,\s*\[', txt)[1:]:
+        offs.append([int(v) for v in re.match(r'([0-9,\s]+)\]', b).group(1).split(',')])
+        C = re.search(r'C\s*=\s*\[(.*?)\];', b, re.S).group(1)
+        P = re.search(r'P\s*=\s*\[(.*?)\];', b, re.S).group(1)
+        cc.append([[names[t] for t in row.split()] for row in C.strip().split('\n') if row.strip()])
+        pp.append([[int(float(t)) for t in row.split()] for row in P.strip().split('\n') if row.strip()])
+    return np.array(offs, np.int64), np.array(cc, np.int8), np.array(pp, np.int8)
+
+
+def singular_sweep(R, nps=(10, 20, 30)):
+    dfn = np.array([[R.singular_terms(2 * np.pi / C0 * f, dx, 'DIRECTFN', n) for n in nps] for dx, f in CASES])
+    dem = np.array([[R.singular_terms(2 * np.pi / C0 * f, dx, 'DEMCEM', n) for n in nps] for dx, f in CASES])
+    offs, cc, pp = nearby_tables_from_reference()
+    np.savez_compressed(os.path.join(os.path.dirname(__file__), 'ref_singular.npz'), np=np.array(nps),
+                        cases=np.array(CASES), directfn=dfn, demcem=dem, nearby_offsets=offs, nearby_C=cc,
+                        nearby_P=pp)
+    print('wrote self-terms at Np', nps, 'and', len(offs), 'nearby tables')
+
+
 def main():
     R = ref.Ref()
+    if len(sys.argv) > 1 and sys.argv[1] == 'singular':
+        singular_sweep(R)
+        return
     out = {}
     blocks, meta = [], []
     for dx, f in CASES:
