@@ -95,6 +95,16 @@ int marie_vie_assemble_N(marie_vie_op* h, double dx, double ko, const double* si
 /* The five self-term integrals the library uses (10 doubles out). */
 int marie_vie_singular_terms(double dx, double ko, double* out10);
 
+/* The five self-terms as the reference computes them by default: method
+ * 'DIRECTFN' of singular_st.m:15-18, singular_ea.m:17-28, singular_va.m:20-31
+ * (getOPERATORS.m:40-42 default; the 20 triangle-pair integrals of
+ * directfn_ws_st_const.cpp / directfn_ws_ea_const.cpp / directfn_ws_va_const.cpp
+ * with the vertex orders of their MEX gateways), Np_1D = np Gauss points per
+ * direction (assembly_nop.m:16 uses 20; 1 <= np <= 64). Evaluated on the
+ * current device. Passing these values to marie_vie_assemble_N reproduces
+ * getOPERATORS(r,f,'N') with its default self-term method. */
+int marie_vie_singular_terms_directfn(double dx, double ko, int np, double* out10);
+
 /* The K operator (curl of the vector potential: H = Hinc + K J / Gram,
  * H_field_Kop.m:69-76): fK is 2L x 2M x 2N x 3 complex, the FFT of the
  * circulant of getOPERATORS(r,f,'K') (getOPERATORS.m:101-115, assembly_kop.m,

@@ -602,7 +602,7 @@ __global__ __launch_bounds__(AT) void k_axis(c2<double>* __restrict__ data, cons
 
 // ------------------------------------------------------------------ host side
 // Gauss-Legendre on [-1,1], ascending, Newton in extended precision
-static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
     x.resize(n);
     w.resize(n);
     for (int i = 0; i < n; ++i) {

@@ -40,6 +40,7 @@ SIGNATURES = [
     ('marie_vie_get_fN', _I, [_P, _P]),
     ('marie_vie_assemble_N', _I, [_P, _D, _D, _P]),
     ('marie_vie_singular_terms', _I, [_D, _D, _P]),
+    ('marie_vie_singular_terms_directfn', _I, [_D, _D, _I, _P]),
     ('marie_vie_set_fK', _I, [_P, _P]),
     ('marie_vie_get_fK', _I, [_P, _P]),
     ('marie_vie_assemble_K', _I, [_P, _D, _D]),
@@ -550,4 +551,12 @@ def singular_terms(dx, ko):
     """The library's five self-term integrals (I_ST, I_EAc, I_EAo, I_VAc, I_VAo)."""
     out = np.zeros(5, np.complex128)
     check(load().marie_vie_singular_terms(float(dx), float(ko), _ptr(out)))
+    return out
+
+
+def singular_terms_directfn(dx, ko, Np=20):
+    """The five self-terms by DIRECTFN with Np Gauss points per direction, as
+    the reference's default method computes them (singular_st/ea/va.m)."""
+    out = np.zeros(5, np.complex128)
+    check(load().marie_vie_singular_terms_directfn(float(dx), float(ko), int(Np), _ptr(out)))
     return out

@@ -156,12 +156,15 @@ def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operat
     2L has a prime factor above 13 (lib.circulant_side: e.g. 210 for a
     102-voxel side, where 204 = 4*3*17).
 
-    `singular` 'DIRECTFN' / 'DEMCEM' select the reference's self-term method by
-    name; both are served by the library's own machine-precision quadrature.
-    A length-5 complex vector supplies the five self-terms verbatim (e.g. the
-    values of the reference's mexdirectfn_* routines). With
-    return_operator=True the device-resident lib.Operator is returned instead
-    of the host array.
+    `singular` selects the self-terms as getOPERATORS.m:36-42 does:
+    'DIRECTFN' (the reference's default) evaluates the reference's DIRECTFN
+    integrals on the device with Np_1D = 20 (assembly_nop.m:16), so the
+    operator is the reference's default one; DIRECTFN is not converged at
+    that order (its I_ST, I_EAc, I_EAo differ from the exact integrals by
+    1e-5..1e-4). 'DEMCEM' uses the library's machine-precision quadrature,
+    which equals DEMCEM's values to 1e-10. A length-5 complex vector supplies
+    the five self-terms verbatim. With return_operator=True the
+    device-resident lib.Operator is returned instead of the host array.
     """
     if op not in ('N', 'K'):
         raise NotImplementedError("operator '%s' is not on this package's path" % op)
@@ -175,7 +178,15 @@ def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operat
     if op == 'K':                              # getOPERATORS.m:101-115
         o.assemble_K(dx, ko)
         return o if return_operator else o.get_fK()
-    sv = None if isinstance(singular, str) else np.asarray(singular, np.complex128)
+    if isinstance(singular, str):
+        if singular == 'DIRECTFN':
+            sv = lib.singular_terms_directfn(dx, ko, 20)
+        elif singular == 'DEMCEM':
+            sv = None
+        else:
+            raise ValueError("singular must be 'DIRECTFN', 'DEMCEM' or five complex values")
+    else:
+        sv = np.asarray(singular, np.complex128)
     o.assemble_N(dx, ko, sv)
     if return_operator:
         return o
