@@ -2,9 +2,10 @@
  * marie_vie.h -- C-ABI of the MI355X (gfx950) FFT-VIE library.
  *
  * Drop-in boundary for MARIE's JVIE hot path. The MATLAB-facing functions
- * keep their signatures (a MEX gateway forwards to these entry points; see
- * the mex/ sources of the package); every entry point below names the
- * reference interface it replaces.
+ * keep their signatures: the MEX gateways marie_amd/mex/mv_AN_hip.cpp
+ * (mv_AN.m's GPU branch) and marie_amd/mex/jvie_gmres_hip.cpp (pgmres on
+ * the JVIE operator) forward to these entry points; every entry point below
+ * names the reference interface it replaces.
  *
  * Conventions
  *  - Every function returns an int status: 0 = OK, otherwise one of

@@ -83,3 +83,30 @@ def test_circulant_side_rule():
         m = lib.circulant_side(n)
         assert m >= 2 * n and m % 2 == 0
         assert lib._largest_prime_factor(m) <= 13
+
+
+def test_mex_gateways_call_only_declared_entry_points():
+    """Every marie_vie_* function the MEX gateway sources call is declared in
+    include/marie_vie.h (and so exported by the library)."""
+    src = os.path.join(ROOT, 'marie_amd', 'mex')
+    declared = set(declared_functions())
+    used = set()
+    for name in os.listdir(src):
+        if name.endswith(('.cpp', '.hpp')):
+            used |= set(re.findall(r'\b(marie_vie_\w+)\s*\(', open(os.path.join(src, name)).read()))
+    assert used and used <= declared, used - declared
+
+
+def test_mex_gateways_build_and_check_arguments_without_gpu():
+    """The gateways compile against the MEX API emulation (tests/mex) and
+    reject a wrong argument count as MATLAB errors before any device call."""
+    import subprocess
+    import sys
+    subprocess.run(['make', '-s', '-C', os.path.join(ROOT, 'tests', 'mex')], check=True)
+    sys.path.insert(0, os.path.join(ROOT, 'tests', 'mex'))
+    import emul
+    for name, nargs in (('mv_AN_hip', 3), ('jvie_gmres_hip', 4)):
+        g = emul.Gateway(name)
+        with pytest.raises(emul.MexError, match='MARIE:hip:nargin'):
+            g(*([np.zeros(2)] * nargs))
+        g('clear', nargout=0)

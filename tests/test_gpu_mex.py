@@ -1,0 +1,83 @@
+"""The MEX gateways (marie_amd/mex) called as MATLAB calls them, through the
+MEX API emulation of tests/mex, against the oracle.
+
+mv_AN_hip(JIn0, fN, Mr, Mc, Gram, transp_flag, idx) must return mv_AN.m's
+column for both flags, keep its device operator across calls with the same
+arrays and follow a changed material; jvie_gmres_hip must return pgmres'
+[x, flag, relres, iter, resvec] for the JVIE system (pgmres.m:70-181).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import fields, krylov, mvp
+from oracle import operators as oop
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), 'mex'))
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _problem(n=10, dx=6.25e-3, f=298.2e6, form=1):
+    x = (np.arange(n) - (n - 1) / 2) * dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    r = np.stack([X, Y, Z], axis=-1)
+    em = fields.em_constants(f)
+    e_r = np.where(X ** 2 + Y ** 2 + Z ** 2 < (0.45 * n * dx) ** 2, 50 - 1j * 0.5 / (em['eo'] * em['omega']), 1.0)
+    Einc, _ = fields.plane_wave_excitation(r, [0, 0, em['ko']], em['omega_mu'], [1, 0, 0])
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(Einc, e_r, dx, f, form=form)
+    return fN, Mr, Mc, Gram, idx, b
+
+
+def test_mv_AN_gateway():
+    import emul
+    print("case MEX gateway mv_AN_hip", flush=True)
+    g = emul.Gateway('mv_AN_hip')
+    fN, Mr, Mc, Gram, idx, b = _problem(form=1)
+    idx1 = (idx + 1).astype(float)                       # MATLAB's 1-based indices
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+    for flag in ('notransp', 'transp'):
+        y = g(x, fN, Mr, Mc, Gram, flag, idx1)
+        assert y.shape == (idx.size, 1)
+        assert _rel(y[:, 0], mvp.mv_AN(x, fN, Mr, Mc, Gram, flag, idx)) <= 1e-12, flag
+    # the same arrays again (cached operator), then a new material
+    y2 = g(x, fN, Mr, Mc, Gram, 'notransp', idx1)
+    assert _rel(y2[:, 0], mvp.mv_AN(x, fN, Mr, Mc, Gram, 'notransp', idx)) <= 1e-12
+    Mc2 = 0.5 * Mc
+    y3 = g(x, fN, Mr, Mc2, Gram, 'notransp', idx1)
+    assert _rel(y3[:, 0], mvp.mv_AN(x, fN, Mr, Mc2, Gram, 'notransp', idx)) <= 1e-12
+    # MATLAB-style errors: arity, flag, sizes
+    with pytest.raises(emul.MexError, match='MARIE:hip:nargin'):
+        g(x, fN, Mr, Mc, Gram, 'notransp')
+    with pytest.raises(emul.MexError, match='MARIE:hip:transp'):
+        g(x, fN, Mr, Mc, Gram, 'sideways', idx1)
+    with pytest.raises(emul.MexError, match='MARIE:hip:size'):
+        g(x[:-1], fN, Mr, Mc, Gram, 'notransp', idx1)
+    g('clear', nargout=0)
+
+
+def test_jvie_gmres_gateway():
+    import emul
+    print("case MEX gateway jvie_gmres_hip", flush=True)
+    g = emul.Gateway('jvie_gmres_hip')
+    fN, Mr, Mc, Gram, idx, b = _problem(form=2)
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx)
+    xr, flr, rr, itr, rvr = krylov.pgmres(A, b, 20, 1e-8, 20)
+    x, flag, relres, it, resvec = g(b, fN, Mr, Mc, Gram, (idx + 1).astype(float), 20, 1e-8, 20, nargout=5)
+    assert flag[0, 0] == flr == 0
+    assert list(it.ravel().astype(int)) == list(itr)
+    assert resvec.shape[1] == 1 and resvec.size == rvr.size
+    assert np.max(np.abs(resvec[:, 0] - rvr)) <= 1e-9
+    assert _rel(x[:, 0], xr) <= 1e-9 and abs(relres[0, 0] - rr) <= 1e-9
+    # a start vector
+    x2, flag2 = g(b, fN, Mr, Mc, Gram, (idx + 1).astype(float), 20, 1e-8, 20, xr, nargout=2)
+    assert flag2[0, 0] == 0 and _rel(x2[:, 0], xr) <= 1e-7
+    g('clear', nargout=0)
