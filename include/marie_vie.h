@@ -157,7 +157,7 @@ int marie_vie_apply_device_noalloc(marie_vie_op* h, int op, int transp, int prec
                                    void* y_dev, int nrhs);
 size_t marie_vie_noalloc_scratch_bytes(void);
 
-/* Restarted GMRES(restart) on A_N with MARIE's pgmres semantics
+/* Restarted GMRES(restart), 1 <= restart <= 511, on A_N with MARIE's pgmres semantics
  * (src_numeric/src_iterative/pgmres.m:70-181, pgmres_iter.m:22-150):
  * stop when the least-squares residual < tol*||b||; flag/relres from the
  * true residual ||b - A x||/||b||; resvec relative to ||b|| (capacity

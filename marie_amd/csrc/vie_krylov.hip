@@ -141,12 +141,15 @@ __global__ __launch_bounds__(KT) void k_reduce(const c2<double>* __restrict__ pa
     if (threadIdx.x == 0) out[blockIdx.x] = c2<double>{acc[0], acc[1]};
 }
 
-// y = beta*y + csign * sum_{i<k} V_i c_i ; optional partial ||y||^2 into npart[blk]
+// most basis columns one combination reads (restart + 1 of GMRES(restart))
+constexpr int kMaxCols = 512;
+
+// y = beta*y + csign * sum_{i<k} V_i c_i (k <= kMaxCols); optional partial ||y||^2 into npart[blk]
 template <typename TY, typename TV>
 __global__ __launch_bounds__(KT) void k_vcomb(c2<TY>* __restrict__ y, double beta, const c2<TV>* __restrict__ V,
                                               int64_t ldv, int k, const c2<double>* __restrict__ coef, double csign,
                                               int64_t n, double* __restrict__ npart, int G) {
-    __shared__ c2<double> cs[128];
+    __shared__ c2<double> cs[kMaxCols];
     __shared__ double sh[KT / 64];
     for (int i = threadIdx.x; i < k; i += KT) cs[i] = coef[i];
     __syncthreads();
@@ -1114,9 +1117,9 @@ static int gmres_entry(marie_vie_op* hh, const void* b, void* x, int restart, do
     Op* h = reinterpret_cast<Op*>(hh);
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
     MARIE_HIP(hipSetDevice(h->device));
-    if (!b || !x || !flag || !relres || !iter || restart < 1 || restart > 64 || maxit < 1 || !(tol > 0) ||
+    if (!b || !x || !flag || !relres || !iter || restart < 1 || restart > kMaxCols - 1 || maxit < 1 || !(tol > 0) ||
         (mode < 0 || mode > 2)) {
-        set_error("invalid gmres arguments (restart must be 1..64, mode 0..2)");
+        set_error("invalid gmres arguments (restart must be 1..511, mode 0..2)");
         return MARIE_ERR_ARG;
     }
     if (!(h->precisions & PREC_F64) || (mode != 0 && !(h->precisions & PREC_F32))) {
