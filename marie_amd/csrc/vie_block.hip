@@ -18,8 +18,9 @@
 // The tall-skinny products (V^H W with K = n ~ 10^6 rows, and V C with a
 // small C) are this file's kernels: a library GEMM tiles the small P x s
 // output and leaves most of the chip idle on K; here k_vhw splits the rows
-// over workgroups (register-tiled 4 x 4 per thread, operands staged in LDS)
-// and k_vc streams V once per output tile. The multi-RHS product is this
+// over workgroups and runs its 16 x 16 output tiles on the matrix cores
+// (operands staged in LDS), and k_vc streams V once per 16 output columns
+// with the coefficients as uniform operands. The multi-RHS product is this
 // library's FFT path, and may be replayed from HIP graphs captured per block
 // step.
 #include "../../include/marie_vie.h"
@@ -104,83 +105,108 @@ int grid_of(int64_t n) {
 }
 
 // ---- tall-skinny products ----------------------------------------------------
-// C (P x Q, partial per row block) = V^H W over rows; V: P columns of length n
-// (stride ldv), W: Q columns (stride ldw). Workgroup (pt, qt, rb): output tile
-// of VP = 64 NI columns of V by VQ = 16 columns of W, rows in chunks of VR = 8
-// taken rb, rb + G, ...; thread (tp, tq) accumulates p = p0 + tp + 64 i (i < NI),
-// q = q0 + tq + 4 j (j < 4) over all its chunks. Partials go to
-// part[(q P + p) G + rb].
-constexpr int VQ = 16, VR = 8;
+// C (P x Q, partial per row block) = V^H W over rows, on the matrix cores
+// (v_mfma_f64_16x16x4f64). V: P columns of length n (stride ldv), W: Q
+// columns (stride ldw). A workgroup (4 waves) takes a tile of TP = 16 NSUB
+// columns of V and the 16 columns q0.. of W (blockIdx.z), rows in chunks of
+// KR taken rb, rb + G, ... (blockIdx.y). Each chunk of V and W is staged in
+// LDS through loads of KR consecutive rows per column; each wave then runs
+// the 4-row k-steps of its 16 x 16 output tile: with A = V^T (16 columns x 4
+// rows) and B = W (4 rows x 16 columns) split into real and imaginary parts,
+// conj(V)^T W = (Ar Br + Ai Bi) + j (Ar Bi - Ai Br): four real products on
+// independent accumulators. When the tile has fewer columns than waves
+// (WPT waves per 16-column subtile), the waves split the k-steps and add
+// their sums in LDS. Partials go to part[(q P + p) G + rb].
+//
+// Operand layout of the instruction (16 x 4 A, 4 x 16 B, 16 x 16 D): lane
+// i + 16 k holds A[i][k] and B[k][i]; D[r][c] is element r / 4 of lane
+// c + 16 (r % 4).
+constexpr int VQ = 16, KR = 32;
+typedef double dx4 __attribute__((ext_vector_type(4)));
 
-template <int NI>
+template <int WPT>
 __global__ __launch_bounds__(BT) void k_vhw(const c2<double>* __restrict__ V, int64_t ldv, int P,
                                             const c2<double>* __restrict__ W, int64_t ldw, int Q, int64_t n,
                                             c2<double>* __restrict__ part, int G) {
-    constexpr int VP = 64 * NI;
-    __shared__ c2<double> Vs[VR][VP + 1];
-    __shared__ c2<double> Ws[VR][VQ + 1];
-    const int ptn = (P + VP - 1) / VP;
-    const int pt = blockIdx.x % ptn, qt = blockIdx.x / ptn;
+    constexpr int NSUB = 4 / WPT, TP = 16 * NSUB;
+    __shared__ c2<double> Vs[KR][TP + 1];
+    __shared__ c2<double> Ws[KR][VQ + 1];
+    __shared__ dx4 red[(WPT > 1) ? 4 * 64 * 2 : 1];
+    const int p0 = blockIdx.x * TP;
     const int rb = blockIdx.y;
-    const int p0 = pt * VP, q0 = qt * VQ;
-    const int tq = threadIdx.x & 3, tp = threadIdx.x >> 2;
-    double ar[NI][4] = {}, ai[NI][4] = {};
-    const int64_t nchunk = (n + VR - 1) / VR;
-    const int pv = P - p0 < VP ? P - p0 : VP;     // live columns of this tile
-    // the next chunk is loaded into registers while the current one is used
-    constexpr int LV = VR * VP / BT;              // V elements staged per thread
-    c2<double> vreg[LV], wreg = {0.0, 0.0};
+    const int q0 = blockIdx.z * VQ;
+    const int pv = P - p0 < TP ? P - p0 : TP;
+    const int qv = Q - q0 < VQ ? Q - q0 : VQ;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int psub = wave / WPT, kpart = wave % WPT;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t nchunk = (n + KR - 1) / KR;
+    constexpr int LV = KR * TP / BT;                  // V elements staged per thread and chunk
+    constexpr int LW = KR * VQ / BT;                  // W elements staged per thread and chunk
+    c2<double> vreg[LV], wreg[LW];
     auto fetch = [&](int64_t ch) {
-        const int64_t r0 = ch * VR;
+        const int64_t r0 = ch * KR;
 #pragma unroll
         for (int k = 0; k < LV; ++k) {
             const int i = threadIdx.x + BT * k;
-            const int r = i % VR, p = i / VR;
+            const int r = i % KR, pc = i / KR;
             const int64_t e = r0 + r;
-            vreg[k] = (p < pv && e < n) ? V[(int64_t)(p0 + p) * ldv + e] : c2<double>{0.0, 0.0};
+            vreg[k] = (pc < pv && e < n) ? V[(int64_t)(p0 + pc) * ldv + e] : c2<double>{0.0, 0.0};
         }
-        if (threadIdx.x < VR * VQ) {
-            const int r = threadIdx.x % VR, q = threadIdx.x / VR;
+#pragma unroll
+        for (int k = 0; k < LW; ++k) {
+            const int i = threadIdx.x + BT * k;
+            const int r = i % KR, qc = i / KR;
             const int64_t e = r0 + r;
-            wreg = (q0 + q < Q && e < n) ? W[(int64_t)(q0 + q) * ldw + e] : c2<double>{0.0, 0.0};
+            wreg[k] = (qc < qv && e < n) ? W[(int64_t)(q0 + qc) * ldw + e] : c2<double>{0.0, 0.0};
         }
     };
+    dx4 crr = {0, 0, 0, 0}, cii = {0, 0, 0, 0}, cri = {0, 0, 0, 0}, cir = {0, 0, 0, 0};
     if (rb < nchunk) fetch(rb);
     for (int64_t ch = rb; ch < nchunk; ch += G) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < LV; ++k) {
             const int i = threadIdx.x + BT * k;
-            Vs[i % VR][i / VR] = vreg[k];
+            Vs[i % KR][i / KR] = vreg[k];
         }
-        if (threadIdx.x < VR * VQ) Ws[threadIdx.x % VR][threadIdx.x / VR] = wreg;
+#pragma unroll
+        for (int k = 0; k < LW; ++k) {
+            const int i = threadIdx.x + BT * k;
+            Ws[i % KR][i / KR] = wreg[k];
+        }
         __syncthreads();
         if (ch + G < nchunk) fetch(ch + G);
-        if (tp >= pv) continue;
-#pragma unroll 2
-        for (int r = 0; r < VR; ++r) {
-            c2<double> v[NI], w[4];
 #pragma unroll
-            for (int i = 0; i < NI; ++i) v[i] = Vs[r][tp + 64 * i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) w[j] = Ws[r][tq + 4 * j];
-#pragma unroll
-            for (int i = 0; i < NI; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    // conj(v) * w (columns beyond pv read stale LDS and are not stored)
-                    ar[i][j] += v[i].x * w[j].x + v[i].y * w[j].y;
-                    ai[i][j] += v[i].x * w[j].y - v[i].y * w[j].x;
-                }
+        for (int ks = kpart; ks < KR / 4; ks += WPT) {
+            const c2<double> a = Vs[4 * ks + lk][16 * psub + li];
+            const c2<double> b = Ws[4 * ks + lk][li];
+            crr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, crr, 0, 0, 0);
+            cii = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, cii, 0, 0, 0);
+            cri = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, cri, 0, 0, 0);
+            cir = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, cir, 0, 0, 0);
         }
     }
-#pragma unroll
-    for (int i = 0; i < NI; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int p = p0 + tp + 64 * i, q = q0 + tq + 4 * j;
-            if (tp + 64 * i < pv && q < Q) part[((int64_t)q * P + p) * G + rb] = c2<double>{ar[i][j], ai[i][j]};
+    dx4 cr = crr + cii, ci = cri - cir;
+    if constexpr (WPT > 1) {
+        // the k-step partial sums of the waves of one subtile, added in a fixed order
+        __syncthreads();
+        red[(wave * 64 + lane) * 2] = cr;
+        red[(wave * 64 + lane) * 2 + 1] = ci;
+        __syncthreads();
+        if (kpart != 0) return;
+        for (int w2 = 1; w2 < WPT; ++w2) {
+            cr += red[((wave + w2) * 64 + lane) * 2];
+            ci += red[((wave + w2) * 64 + lane) * 2 + 1];
         }
+    }
+    // lane (c + 16 (r % 4)), element e: output row r = 4 e + lk (a column p of V), column li (a column q of W)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int pr = 16 * psub + 4 * e + lk;
+        if (pr < pv && li < qv)
+            part[((int64_t)(q0 + li) * P + p0 + pr) * G + rb] = c2<double>{cr[e], ci[e]};
+    }
 }
 
 // C[q * ldc + p] = sum_rb part[(q P + p) G + rb]: one wave per output, lanes
@@ -206,48 +232,44 @@ __global__ __launch_bounds__(BT) void k_vhw_reduce(const c2<double>* __restrict_
     }
 }
 
-// out(:, q) = beta * out(:, q) + alpha * sum_p V(:, p) C(p, q); thread per row,
-// 16 output columns per workgroup, C streamed through LDS 64 rows at a time
-constexpr int CQ = 16, CP = 64;
+// out(:, q) = beta * out(:, q) + alpha * sum_p V(:, p) C(p, q): thread per row,
+// CQ output columns per workgroup. Every element of V meets CQ coefficients
+// C(p, q0..q0+CQ-1), the same for all lanes: they are read as uniform
+// (scalar) loads and enter the FMAs as SGPR operands, with no LDS in the loop.
+constexpr int CQ = 16;
 __global__ __launch_bounds__(BT) void k_vc(c2<double>* __restrict__ out, int64_t ldo, const c2<double>* __restrict__ V,
                                            int64_t ldv, int P, const c2<double>* __restrict__ C, int64_t ldc, int Q,
                                            int64_t n, double alpha, double beta) {
-    __shared__ c2<double> Cs[CP][CQ];
     const int q0 = blockIdx.y * CQ;
+    const int nq = Q - q0 < CQ ? Q - q0 : CQ;
     const int64_t e = blockIdx.x * (int64_t)BT + threadIdx.x;
+    const bool live = e < n;
+    const int64_t el = live ? e : 0;
     double ax[CQ] = {}, ay[CQ] = {};
-    for (int pc = 0; pc < P; pc += CP) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < CP * CQ; i += BT) {
-            const int p = i / CQ, q = i % CQ;
-            Cs[p][q] = (pc + p < P && q0 + q < Q) ? C[(int64_t)(q0 + q) * ldc + pc + p] : c2<double>{0.0, 0.0};
-        }
-        __syncthreads();
-        if (e < n) {
-            const int np = P - pc < CP ? P - pc : CP;
-#pragma unroll 4
-            for (int p = 0; p < np; ++p) {
-                const c2<double> v = V[(int64_t)(pc + p) * ldv + e];
+    const c2<double>* cb = C + (int64_t)q0 * ldc;
+    for (int p = 0; p < P; ++p) {
+        const c2<double> v = V[(int64_t)p * ldv + el];
 #pragma unroll
-                for (int q = 0; q < CQ; ++q) {
-                    const c2<double> cc = Cs[p][q];
-                    ax[q] += v.x * cc.x - v.y * cc.y;
-                    ay[q] += v.x * cc.y + v.y * cc.x;
-                }
-            }
+        for (int q = 0; q < CQ; ++q) {
+            // columns past nq repeat column nq-1; their sums are not stored
+            const int qc = q < nq ? q : nq - 1;
+            const c2<double> cc = cb[(int64_t)qc * ldc + p];
+            ax[q] = fma(v.x, cc.x, fma(-v.y, cc.y, ax[q]));
+            ay[q] = fma(v.x, cc.y, fma(v.y, cc.x, ay[q]));
         }
     }
-    if (e >= n) return;
+    if (!live) return;
 #pragma unroll
     for (int q = 0; q < CQ; ++q) {
-        if (q0 + q >= Q) break;
-        c2<double>* o = out + (int64_t)(q0 + q) * ldo + e;
-        double ox = alpha * ax[q], oy = alpha * ay[q];
-        if (beta != 0.0) {
-            ox += beta * o->x;
-            oy += beta * o->y;
+        if (q < nq) {
+            c2<double>* o = out + (int64_t)(q0 + q) * ldo + e;
+            double ox = alpha * ax[q], oy = alpha * ay[q];
+            if (beta != 0.0) {
+                ox += beta * o->x;
+                oy += beta * o->y;
+            }
+            *o = c2<double>{ox, oy};
         }
-        *o = c2<double>{ox, oy};
     }
 }
 
@@ -296,18 +318,23 @@ int vhw_blocks(int tiles) {
 
 // C (P x Q, ld ldc) = V^H W over n rows
 int vhw(BCtx& c, const c2<double>* V, int P, const c2<double>* W, int Q, c2<double>* C, int64_t ldc) {
-    const int ni = P <= 64 ? 1 : 4;
-    const int vp = 64 * ni;
-    const int tiles = ((P + vp - 1) / vp) * ((Q + VQ - 1) / VQ);
-    const int G = vhw_blocks(tiles);
+    // waves per 16-column subtile: a narrow V (the CholeskyQR Gram, early
+    // block steps) splits the k-steps over the four waves instead
+    const int wpt = P <= 16 ? 4 : (P <= 32 ? 2 : 1);
+    const int tp = 16 * (4 / wpt);
+    const int ptiles = (P + tp - 1) / tp, qtiles = (Q + VQ - 1) / VQ;
+    const int G = vhw_blocks(ptiles * qtiles);
     if ((size_t)G * P * Q > VPART) {
         set_error("block too large for the multi-dot partials");
         return ERR_SIZE;
     }
-    if (ni == 1)
-        hipLaunchKernelGGL(k_vhw<1>, dim3(tiles, G), dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
+    const dim3 grid(ptiles, G, qtiles);
+    if (wpt == 4)
+        hipLaunchKernelGGL(k_vhw<4>, grid, dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
+    else if (wpt == 2)
+        hipLaunchKernelGGL(k_vhw<2>, grid, dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
     else
-        hipLaunchKernelGGL(k_vhw<4>, dim3(tiles, G), dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
+        hipLaunchKernelGGL(k_vhw<1>, grid, dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
     hipLaunchKernelGGL(k_vhw_reduce, dim3((P * Q + BT / 64 - 1) / (BT / 64)), dim3(BT), 0, c.s, c.ws.vpart, P, Q, G,
                        C, ldc);
     MARIE_HIP(hipGetLastError());
