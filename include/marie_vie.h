@@ -93,6 +93,19 @@ int marie_vie_get_fN(marie_vie_op* h, double* fN);
  * machine-precision self-terms. */
 int marie_vie_assemble_N(marie_vie_op* h, double dx, double ko, const double* singular);
 
+/* getOPERATORS' other output types (getOPERATORS.m:88-95 for 'N', :108-115
+ * for 'K'): type MARIE_TYPE_T returns the Toeplitz generator
+ * (assembly_nop.m / assembly_kop.m: L x M x N x 6, or x 3 for K) and
+ * MARIE_TYPE_C its circulant (circulant_nop.m / circulant_kop.m:
+ * Px x Py x Pz x 6 or x 3), assembled on the device as marie_vie_assemble_N /
+ * _K do, into the host buffer `out` (interleaved complex, column-major). The
+ * handle's stored operator is not changed. op = MARIE_OP_N (with `singular`
+ * as marie_vie_assemble_N) or MARIE_OP_K (`singular` ignored). */
+#define MARIE_TYPE_T 0
+#define MARIE_TYPE_C 1
+int marie_vie_assemble_type(marie_vie_op* h, int op, double dx, double ko, const double* singular, int type,
+                            double* out);
+
 /* The five self-term integrals the library uses (10 doubles out). */
 int marie_vie_singular_terms(double dx, double ko, double* out10);
 

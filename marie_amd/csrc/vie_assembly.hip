@@ -716,7 +716,18 @@ __global__ void k_to_f32(const c2<double>* __restrict__ s, c2<float>* __restrict
         d[i] = c2<float>{(float)s[i].x, (float)s[i].y};
 }
 
-int assemble_N(Op* h, double dx, double ko, const double* singular) {
+// copy n complex of a device array to the host (stream-ordered, waited for)
+static int to_host(const c2<double>* d, double* host, size_t n, hipStream_t s) {
+    MARIE_HIP(hipMemcpyAsync(host, d, n * sizeof(c2<double>), hipMemcpyDeviceToHost, s));
+    MARIE_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+
+// stage < 0: assemble and store fN on the handle (getOPERATORS 'FFTofC');
+// stage 0 / 1: copy the Toeplitz generator (L x M x N x 6, 'T') or the
+// circulant (Px x Py x Pz x 6, 'C') to `out` instead, leaving the handle's
+// operator as it was (getOPERATORS.m:88-95)
+int assemble_N(Op* h, double dx, double ko, const double* singular, int stage, double* out) {
     const int L = h->L, M = h->M, N = h->N;
     // any circulant side >= 2L-1 embeds the same Toeplitz operator (circulant_nop.m
     // uses exactly 2L); other sides let the caller pick FFT-friendly factors
@@ -783,9 +794,20 @@ int assemble_N(Op* h, double dx, double ko, const double* singular) {
     hipLaunchKernelGGL(k_pairs, dim3(nmed * 21), dim3(AT), 0, s, Ju, offs, dx, ko, r10.r);
     hipLaunchKernelGGL(k_pairs, dim3(8 * 21), dim3(AT), 0, s, Ju + nmed * 21, offs + 3 * nmed, dx, ko, r20.r);
     hipLaunchKernelGGL(k_fixup, dim3(1), dim3(192), 0, s, G, Ju, offs, nmed, sing, L, M, N, dx);
+    if (stage == 0) {
+        TRYH(hipGetLastError());
+        TRY(to_host(G, out, (size_t)nv * 6, s));
+        cleanup();
+        return OK;
+    }
     hipLaunchKernelGGL(k_circulant, dim3(4096), dim3(256), 0, s, G, (c2<double>*)full, L, M, N, h->Px, h->Py, h->Pz,
                        0);
     TRYH(hipGetLastError());
+    if (stage == 1) {
+        TRY(to_host((const c2<double>*)full, out, nf, s));
+        cleanup();
+        return OK;
+    }
     TRY(fft3_forward_inplace(h, (c2<double>*)full, 6, s));
     TRYH(hipStreamSynchronize(s));
     {
@@ -799,7 +821,8 @@ int assemble_N(Op* h, double dx, double ko, const double* singular) {
     return OK;
 }
 
-int assemble_K(Op* h, double dx, double ko) {
+// stage as assemble_N (getOPERATORS.m:101-115): 'T' is L x M x N x 3, 'C' Px x Py x Pz x 3
+int assemble_K(Op* h, double dx, double ko, int stage, double* out) {
     const int L = h->L, M = h->M, N = h->N;
     if (h->Px < 2 * L - 1 || h->Py < 2 * M - 1 || h->Pz < 2 * N - 1) {
         set_error("assembly needs a circulant of at least (2L-1) x (2M-1) x (2N-1)");
@@ -858,9 +881,20 @@ int assemble_K(Op* h, double dx, double ko) {
     hipLaunchKernelGGL(k_kpairs, dim3(nmed * 12), dim3(AT), 0, s, Jm, offs, dx, ko, r10.r, 0);
     hipLaunchKernelGGL(k_kpairs, dim3(8 * 21), dim3(AT), 0, s, Jn, offs + 3 * nmed, dx, ko, r20.r, 1);
     hipLaunchKernelGGL(k_kfix, dim3(1), dim3(192), 0, s, G, Jm, Jn, offs, nmed, L, M, N, coef);
+    if (stage == 0) {
+        TRYH(hipGetLastError());
+        TRY(to_host(G, out, (size_t)nv * 3, s));
+        cleanup();
+        return OK;
+    }
     hipLaunchKernelGGL(k_circulant, dim3(4096), dim3(256), 0, s, G, (c2<double>*)full, L, M, N, h->Px, h->Py, h->Pz,
                        1);
     TRYH(hipGetLastError());
+    if (stage == 1) {
+        TRY(to_host((const c2<double>*)full, out, (size_t)3 * h->Px * h->Py * h->Pz, s));
+        cleanup();
+        return OK;
+    }
     TRY(fft3_forward_inplace(h, (c2<double>*)full, 3, s));
     TRYH(hipStreamSynchronize(s));
     {
@@ -882,7 +916,20 @@ extern "C" int marie_vie_assemble_N(marie_vie_op* hh, double dx, double ko, cons
     Op* h = reinterpret_cast<Op*>(hh);
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
     MARIE_HIP(hipSetDevice(h->device));
-    return assemble_N(h, dx, ko, singular);
+    return assemble_N(h, dx, ko, singular, -1, nullptr);
+}
+
+extern "C" int marie_vie_assemble_type(marie_vie_op* hh, int op, double dx, double ko, const double* singular,
+                                       int type, double* out) {
+    Op* h = reinterpret_cast<Op*>(hh);
+    if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
+    if ((op != MARIE_OP_N && op != MARIE_OP_K) || (type != MARIE_TYPE_T && type != MARIE_TYPE_C) || !out) {
+        set_error("invalid arguments (op MARIE_OP_N or MARIE_OP_K, type MARIE_TYPE_T or MARIE_TYPE_C, out)");
+        return MARIE_ERR_ARG;
+    }
+    MARIE_HIP(hipSetDevice(h->device));
+    if (op == MARIE_OP_N) return assemble_N(h, dx, ko, singular, type, out);
+    return assemble_K(h, dx, ko, type, out);
 }
 
 extern "C" int marie_vie_singular_terms(double dx, double ko, double* out10) {

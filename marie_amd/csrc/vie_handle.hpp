@@ -114,7 +114,7 @@ int store_fN(Op* h, void* full64);   // takes ownership of a device complex128 f
 int load_fN(Op* h, void* out64);     // full complex128 fN into a device buffer
 int store_circ(Op* h, int kind, void* full64);   // kind 0: fN, 1: fK
 int load_circ(Op* h, int kind, void* out64);
-int assemble_K(Op* h, double dx, double ko);
+int assemble_K(Op* h, double dx, double ko, int stage = -1, double* out = nullptr);
 int autotune(Op* h, int prec, int iters, double* best_ms5);
 int timing_flush(Op* h);   // folds the recorded ring rows into kernel_ms
 

@@ -40,6 +40,7 @@ SIGNATURES = [
     ('marie_vie_get_fN', _I, [_P, _P]),
     ('marie_vie_assemble_N', _I, [_P, _D, _D, _P]),
     ('marie_vie_singular_terms', _I, [_D, _D, _P]),
+    ('marie_vie_assemble_type', _I, [_P, _I, _D, _D, _P, _I, _P]),
     ('marie_vie_singular_terms_directfn', _I, [_D, _D, _I, _P]),
     ('marie_vie_set_fK', _I, [_P, _P]),
     ('marie_vie_get_fK', _I, [_P, _P]),
@@ -290,6 +291,18 @@ class Operator:
             check(load().marie_vie_assemble_N(self._h, float(dx), float(ko), _ptr(s)))
         else:
             check(load().marie_vie_assemble_N(self._h, float(dx), float(ko), None))
+
+    def assemble_type(self, dx, ko, op=OP_N, type_='T', singular=None):
+        """getOPERATORS' 'T' (Toeplitz generator, L x M x N x c) or 'C'
+        (circulant, Px x Py x Pz x c) of the N (c = 6) or K (c = 3) operator,
+        assembled on the device; the handle's operator is not changed."""
+        nc = 6 if op == OP_N else 3
+        shape = (self.L, self.M, self.N, nc) if type_ == 'T' else tuple(self.P) + (nc,)
+        out = np.empty(int(np.prod(shape)), np.complex128)
+        sv = None if singular is None else np.ascontiguousarray(np.asarray(singular, np.complex128))
+        check(load().marie_vie_assemble_type(self._h, op, float(dx), float(ko), None if sv is None else _ptr(sv),
+                                             0 if type_ == 'T' else 1, _ptr(out)))
+        return out.reshape(shape, order='F')
 
     def set_fK(self, fK):
         Px, Py, Pz = self.P

@@ -24,6 +24,7 @@ operators are cached per fN array so a solver loop does not re-upload it
 (the reference keeps fN_gpu in a closure, JVIE_Solver.m:271-276).
 GPU_flag = 0 is rejected: this package has no CPU path.
 """
+import sys
 import time
 
 import numpy as np
@@ -165,17 +166,26 @@ def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operat
     which equals DEMCEM's values to 1e-10. A length-5 complex vector supplies
     the five self-terms verbatim. With return_operator=True the
     device-resident lib.Operator is returned instead of the host array.
+
+    type 'T' returns the Toeplitz generator (L x M x N x 6 for N, x 3 for K)
+    and 'C' its circulant embedding (Px x Py x Pz x c) instead of the FFT of
+    the circulant (getOPERATORS.m:88-95, :108-114); both are assembled on the
+    device by the same kernels and copied back.
     """
     if op not in ('N', 'K'):
         raise NotImplementedError("operator '%s' is not on this package's path" % op)
-    if type != 'FFTofC':
-        raise NotImplementedError("type '%s': only 'FFTofC' is produced on the device" % type)
+    if type not in ('FFTofC', 'T', 'C'):
+        raise ValueError("type must be 'FFTofC', 'T' or 'C' (getOPERATORS.m:88-95)")
+    if type != 'FFTofC' and return_operator:
+        raise ValueError("return_operator needs type 'FFTofC' (the product uses the FFT of the circulant)")
     r = np.asarray(r)
     L, M, N = r.shape[:3]
     dx = r[1, 0, 0, 0] - r[0, 0, 0, 0]
     ko = 2 * np.pi / C0 * f                    # getOPERATORS.m:48
     o = lib.Operator(L, M, N, circulant_shape, precisions=precisions)
     if op == 'K':                              # getOPERATORS.m:101-115
+        if type != 'FFTofC':
+            return o.assemble_type(dx, ko, lib.OP_K, type)
         o.assemble_K(dx, ko)
         return o if return_operator else o.get_fK()
     if isinstance(singular, str):
@@ -187,6 +197,8 @@ def getOPERATORS(r, f, op='N', type='FFTofC', singular='DIRECTFN', return_operat
             raise ValueError("singular must be 'DIRECTFN', 'DEMCEM' or five complex values")
     else:
         sv = np.asarray(singular, np.complex128)
+    if type != 'FFTofC':
+        return o.assemble_type(dx, ko, lib.OP_N, type, sv)
     o.assemble_N(dx, ko, sv)
     if return_operator:
         return o
@@ -316,8 +328,15 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
     restarts keeping `ritz` harmonic Ritz vectors (JVIE_Solver.m:378-450).
     Any other solver letter runs BiCGStab (JVIE_Solver.m:451-492, the default), preconditioned
     when `precond` is 'L' or 'R'; iter is then in half steps / 2 and resvec absolute.
-    Forms I and II (JVIE_Solver.m:176-300).
+    Forms I and II (JVIE_Solver.m:176-300). precision='fp32-refined-c64'
+    stores the refinement cycles' Krylov basis in complex64 as well (every
+    dot product, norm and Givens rotation still in FP64).
+
+    logfile: a path the run summary is appended to (JVIE_Solver.m:115-122,
+    :159-166 and the solver line of :335, :355, :375, ...); None or a path
+    that cannot be opened writes it to standard output, as fid = 1 does.
     """
+    t_start = time.perf_counter()
     _require_gpu(gpu_flag)
     if tol < 1e-15 or tol >= 1:          # JVIE_Solver.m:78-80
         tol = 1e-3
@@ -326,9 +345,20 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
     use_pc = precond in ('L', 'R')       # JVIE_Solver.m:92-98
     if use_pc and precision != 'fp64':
         raise ValueError('the preconditioned solvers run in FP64')
+    if precision not in ('fp64', 'fp32-refined', 'fp32-refined-c64'):
+        raise ValueError("precision must be 'fp64', 'fp32-refined' or 'fp32-refined-c64'")
     Mr, Mc, Gram, idx, Vrhs = jvie_system(Einc, e_r, r, freq, form)
     L, M, N = e_r.shape
-    prec = lib.PREC_F64 | (lib.PREC_F32 if precision == 'fp32-refined' else 0)
+    prec = lib.PREC_F64 | (lib.PREC_F32 if precision != 'fp64' else 0)
+    log = _open_log(logfile)
+    dxv = float(np.asarray(r)[1, 0, 0, 0] - np.asarray(r)[0, 0, 0, 0])
+    log.write('\n ----------------------------------------------------------')
+    log.write('\n Domain:                %dx%dx%d voxels' % (L, M, N))
+    log.write('\n Resolution:            %.2fmm' % (dxv * 1000))
+    log.write('\n # DOFS:                %d' % (3 * L * M * N))
+    log.write('\n # DOFS in Scatterer:   %d' % idx.size)
+    log.write('\n Operating Frequency:   %.2f MHz' % (freq / 1e6))
+    log.write('\n\n ----------------------------------------------------------')
     if isinstance(fN, lib.Operator):
         op = fN
     else:
@@ -347,16 +377,42 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
             x, flag, relres, it, resvec = op.gmres_precond(Vrhs, None, inner_it, tol, outer_it, side=side)
         else:
             x, flag, relres, it, resvec = op.gmres(Vrhs, None, inner_it, tol, outer_it,
-                                                   mode=1 if precision == 'fp32-refined' else 0,
+                                                   mode={'fp64': 0, 'fp32-refined': 1, 'fp32-refined-c64': 2}[precision],
                                                    inner_tol=inner_tol)
     else:
         if precision != 'fp64':
             raise ValueError('BiCGStab runs in FP64')
         x, flag, relres, it, resvec = op.bicgstab(Vrhs, None, tol, max_it, precond=use_pc)
     solvert = time.perf_counter() - t0
+    # the solver line of JVIE_Solver.m:328-495 and the closing line of :517
+    pc = {'R': 'w/ Right preconditioner', 'L': 'w/ Left preconditioner'}.get(precond, 'wo/ preconditioner')
+    if solver == 'G':
+        head = 'GMRES solver (%d restart), %s' % (inner_it, pc)
+    elif solver == 'D':
+        head = 'GMRES solver (%d restart, deflated %d), %s \n' % (inner_it, ritz, pc)
+    else:
+        head = 'BICGSTAB solver, %s' % ('w/ preconditioner' if use_pc else 'wo/ preconditioner')
+    if precision != 'fp64':
+        head = head.replace('),', '), %s,' % precision, 1)
+    log.write('\n GPU ' + head)
+    log.write(' Time  = %.2f [sec] for %d iterations, flag %d, residue %g, relative residue %g \n'
+              % (solvert, len(resvec), flag, resvec[-1] if len(resvec) else 0.0, relres))
+    log.write(' VIE Solve finished. Elapsed Time  = %.2f [sec] \n' % (time.perf_counter() - t_start))
+    if log is not sys.stdout:
+        log.close()
     Je = np.zeros(3 * L * M * N, complex)
     Je[idx - 1] = x
     return Je.reshape(L, M, N, 3, order='F'), flag, relres, it, resvec, solvert
+
+
+def _open_log(logfile):
+    """fid of JVIE_Solver.m:115-122: the file opened for appending, or standard output."""
+    if logfile is None:
+        return sys.stdout
+    try:
+        return open(logfile, 'a')
+    except OSError:
+        return sys.stdout
 
 
 def jvie2solver(Vrhs, op, tol, Vini=None, max_it=1000):

@@ -85,3 +85,25 @@ def test_directfn_operator_at_config2_shape():
           % (np.linalg.norm(got) / scale, np.linalg.norm(got - dfN) / scale), flush=True)
     assert np.linalg.norm(got - dfN) <= 1e-12 * scale
     assert np.linalg.norm(got) > 1e-6 * scale
+
+
+@pytest.mark.parametrize("op", ["N", "K"])
+def test_getoperators_toeplitz_and_circulant_types(op):
+    """getOPERATORS' 'T' and 'C' outputs against the oracle's assembly and
+    circulant embedding (the same Toeplitz generator the 'FFTofC' path
+    transforms), with the reference's default self-terms for N."""
+    from marie_amd import vie
+    shape, dx, f = (7, 6, 5), 6.25e-3, 298.2e6
+    print("case getOPERATORS types T and C,", op, shape, flush=True)
+    L, M, N = shape
+    xs = [(np.arange(s) - (s - 1) / 2) * dx for s in shape]
+    r = np.stack(np.meshgrid(*xs, indexing='ij'), axis=-1)
+    method = G['directfn'][_case(dx, f), 1] if op == 'N' else 'own'
+    for t, want_shape in (('T', (L, M, N)), ('C', (2 * L, 2 * M, 2 * N))):
+        got = vie.getOPERATORS(r, f, op, type=t)
+        ref = oop.get_operators(L, M, N, dx, f, op, type_=t, method=method)
+        assert got.shape == want_shape + (6 if op == 'N' else 3,)
+        assert _rel(got, ref) <= 1e-12, (op, t, _rel(got, ref))
+    # the FFT of the 'C' output is the 'FFTofC' operator
+    fC = np.fft.fftn(vie.getOPERATORS(r, f, op, type='C'), axes=(0, 1, 2))
+    assert _rel(fC, vie.getOPERATORS(r, f, op)) <= 1e-13

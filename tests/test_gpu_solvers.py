@@ -151,9 +151,12 @@ def test_bicgstab_fixed_iterations_and_edges():
     assert fl2 == 0 and it2 == 0 and rv2.size == 1 and np.array_equal(x, xs)
 
 
-def test_jvie_solver_dispatch():
+def test_jvie_solver_dispatch(tmp_path):
     """JVIE_Solver's default (BiCGStab), 'G' and the preconditioned branches
-    reach the same currents on a small sphere in form I."""
+    reach the same currents on a small sphere in form I; each run appends the
+    reference's summary (JVIE_Solver.m:159-166, :335) to one log file, whose
+    iteration counts and residues are read back."""
+    import re
     from marie_amd import vie
     print("case JVIE_Solver dispatch", flush=True)
     n, dx, f = 12, 6.25e-3, 298.2e6
@@ -166,10 +169,26 @@ def test_jvie_solver_dispatch():
     fN = oop.get_operators(n, n, n, dx, f, 'N')
     ref, flag, *_ = vie.JVIE_Solver(Einc, e_r, r, f, fN, 1e-10, 1, 'G')
     assert flag == 0
+    log = tmp_path / 'vie.log'
+    runs = []
     for solver, precond in (('B', 0), ('M', 'L'), ('G', 'R'), ('G', 'L'), ('B', 'R'), ('D', 0), ('D', 'R')):
-        J, flag, relres, it, resvec, t = vie.JVIE_Solver(Einc, e_r, r, f, fN, 1e-8, 1, solver, precond)
+        J, flag, relres, it, resvec, t = vie.JVIE_Solver(Einc, e_r, r, f, fN, 1e-8, 1, solver, precond,
+                                                         logfile=str(log))
         assert flag == 0 and relres <= 1e-8, (solver, precond)
         assert _rel(J, ref) <= 1e-6, (solver, precond)
+        runs.append((solver, precond, len(resvec), relres))
+    text = log.read_text()
+    assert text.count(' Domain:                %dx%dx%d voxels' % (n, n, n)) == len(runs)
+    assert text.count(' # DOFS:                %d' % (3 * n ** 3)) == len(runs)
+    heads = re.findall(r'\n GPU (GMRES|BICGSTAB) solver[^\n]*?, (wo?/ [A-Za-z ]*preconditioner)', text)
+    times = re.findall(r'Time  = [0-9.]+ \[sec\] for (\d+) iterations, flag (\d+), residue \S+, '
+                       r'relative residue (\S+) ', text)
+    assert len(heads) == len(times) == len(runs)
+    for (solver, precond, nres, relres), (kind, pc), (n_it, fl, rr) in zip(runs, heads, times):
+        assert kind == ('GMRES' if solver in 'GD' else 'BICGSTAB')
+        assert pc.startswith('wo/') == (precond not in ('L', 'R'))
+        assert int(n_it) == nres and int(fl) == 0 and abs(float(rr) - relres) <= 1e-5 * relres
+    assert text.count('VIE Solve finished.') == len(runs)
 
 
 def _hetero_problem(n=12, dx=6.25e-3, f=298.2e6, form=2, seed=3):
