@@ -3,6 +3,7 @@
 #include "vie_handle.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -96,6 +97,7 @@ int marie_vie_create(marie_vie_op** out, int L, int M, int N, int Px, int Py, in
     h->L = L; h->M = M; h->N = N; h->Px = Px; h->Py = Py; h->Pz = Pz;
     h->precisions = precisions;
     h->device = device;
+    if (const char* g = std::getenv("MARIE_XY_GROUP")) h->xy_group = std::atoi(g);
     int st = OK;
     for (int a = 0; a < 3 && st == OK; ++a) st = build_plan(a == 0 ? Px : a == 1 ? Py : Pz, &h->plan[a]);
     if (st == OK && hipStreamCreate(&h->stream) != hipSuccess) {

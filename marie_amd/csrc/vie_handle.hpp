@@ -83,6 +83,10 @@ struct Op {
 
     // per-kernel variants (fwd_x, fwd_y, z, inv_y, inv_x): the register engine
     // where the side allows it (apply falls back to the LDS-stage engine)
+    // right-hand sides per x/y launch of a multi-column product (0: as many
+    // as keep their x-pass buffers within kXYGroupBytes; MARIE_XY_GROUP at
+    // creation sets it)
+    int xy_group = 0;
     int emax64[5] = {-4, -16, -8, -8, -4};
     int emax32[5] = {-8, -16, -16, -16, -8};
 
@@ -100,6 +104,10 @@ struct Op {
 
     int64_t nvox() const { return (int64_t)L * M * N; }
 };
+
+// bytes of x-pass output the multi-column product keeps per group of
+// columns: below the 256 MB last-level cache, so the y pass reads it there
+constexpr int64_t kXYGroupBytes = (int64_t)192 << 20;
 
 int ensure_work(Op* h, int nrhs);
 int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s);

@@ -753,32 +753,93 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
         MARIE_HIP(hipEventRecord(tev[0], s));
     }
     const int conj_in = (op == OP_AN && transp) ? 1 : 0;
+    // The x and y passes of several right-hand sides run in groups of columns
+    // whose A slabs fit the last-level cache together (256 MB): the y pass
+    // then reads what the x pass wrote from that cache, and the inverse x pass
+    // what the inverse y pass wrote. The z pass takes every column at once
+    // (each fN entry is read once for all of them).
+    const int64_t slabA = nline * Px, slabB = (int64_t)N * Py * Px;
+    int grp = h->xy_group;
+    if (grp <= 0) grp = (int)std::max<int64_t>(1, kXYGroupBytes / (3 * slabA * (int64_t)sizeof(c2<T>)));
+    grp = std::min(grp, nrhs);
 
-    // forward x
-    if (reg[0]) {
-        reg_fwd_x<T>(Px, rc[0], s, x, A, h->pos, Mc, twx, L, M, N, h->nidx, conj_in, ncomp);
-    } else {
-        const int lg = pick_lg(Px, e[0] * NT, 16);
-        const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
-        const size_t sm = (((size_t)Px << lg) + Px) * sizeof(c2<T>);    // tile + twiddles
-        MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e[0], RS, hipLaunchKernelGGL((k_fwd_x<T, EM, RS>), g, dim3(NT), sm,
-                   s, x, A, h->pos, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, conj_in)));
+    auto fwd_xy = [&](int q0, int gq) -> int {
+        const int nc = 3 * gq;
+        const c2<T>* xq = x + (int64_t)q0 * h->nidx;
+        c2<T>* Aq = A + (int64_t)3 * q0 * slabA;
+        c2<T>* Bq = B + (int64_t)3 * q0 * slabB;
+        // forward x
+        if (reg[0]) {
+            reg_fwd_x<T>(Px, rc[0], s, xq, Aq, h->pos, Mc, twx, L, M, N, h->nidx, conj_in, nc);
+        } else {
+            const int lg = pick_lg(Px, e[0] * NT, 16);
+            const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), nc);
+            const size_t sm = (((size_t)Px << lg) + Px) * sizeof(c2<T>);    // tile + twiddles
+            MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e[0], RS, hipLaunchKernelGGL((k_fwd_x<T, EM, RS>), g, dim3(NT),
+                       sm, s, xq, Aq, h->pos, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, conj_in)));
+        }
+        MARIE_HIP(hipGetLastError());
+        if (tm && grp == nrhs) MARIE_HIP(hipEventRecord(tev[1], s));
+        // forward y: A[c][k][j<M][i] -> B[c][k][j<Py][i]
+        if (reg[1]) {
+            reg_strided<T>(false, Py, rc[1], s, Aq, Bq, twy, Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
+                           (int64_t)Py * Px, M, Py, slabA, slabB, N, nc);
+        } else {
+            const int lg = pick_lg(Py, e[1] * NT, txmax);
+            const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, nc);
+            const size_t sm = (((size_t)Py << lg) + Py) * sizeof(c2<T>);    // tile + twiddles
+            MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e[1], RS, hipLaunchKernelGGL((k_strided<T, EM, false, RS>), g,
+                       dim3(NT), sm, s, Aq, Bq, twy, h->plan[1], Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
+                       (int64_t)Py * Px, M, Py, lg, slabA, slabB)));
+        }
+        MARIE_HIP(hipGetLastError());
+        return OK;
+    };
+    auto inv_yx = [&](int q0, int gq) -> int {
+        const int nc = 3 * gq;
+        const c2<T>* xq = x + (int64_t)q0 * h->nidx;
+        c2<T>* yq = y + (int64_t)q0 * h->nidx;
+        c2<T>* Aq = A + (int64_t)3 * q0 * slabA;
+        const c2<T>* Bq = B + (int64_t)3 * q0 * slabB;
+        // inverse y: B[c][k][j<Py][i] -> A[c][k][j<M][i]
+        if (reg[3]) {
+            reg_strided<T>(true, Py, rc[3], s, Bq, Aq, twy, Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
+                           (int64_t)M * Px, Py, M, slabB, slabA, N, nc);
+        } else {
+            const int lg = pick_lg(Py, e[3] * NT, txmax);
+            const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, nc);
+            const size_t sm = (((size_t)Py << lg) + Py) * sizeof(c2<T>);    // tile + twiddles
+            MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e[3], RS, hipLaunchKernelGGL((k_strided<T, EM, true, RS>), g,
+                       dim3(NT), sm, s, Bq, Aq, twy, h->plan[1], Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
+                       (int64_t)M * Px, Py, M, lg, slabB, slabA)));
+        }
+        MARIE_HIP(hipGetLastError());
+        if (tm && grp == nrhs) MARIE_HIP(hipEventRecord(tev[4], s));
+        // inverse x + crop + epilogue
+        const T scale = T(1.0 / ((double)Px * (double)Py * (double)Pz));
+        if (reg[4]) {
+            reg_inv_x<T>(Px, rc[4], s, Aq, yq, xq, h->pos, Mr, Mc, twx, L, M, N, h->nidx, op, transp, (T)h->Gram,
+                         scale, nc);
+        } else {
+            const int lg = pick_lg(Px, e[4] * NT, 16);
+            const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), nc);
+            const size_t sm = (((size_t)Px << lg) + Px) * sizeof(c2<T>);    // tile + twiddles
+            MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e[4], RS, hipLaunchKernelGGL((k_inv_x<T, EM, RS>), g, dim3(NT),
+                       sm, s, Aq, yq, xq, h->pos, Mr, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, op, transp,
+                       (T)h->Gram, scale)));
+        }
+        MARIE_HIP(hipGetLastError());
+        return OK;
+    };
+
+    // with grouped columns the timing ring cannot split the interleaved
+    // passes: fwd_x reads as both forward passes (fwd_y zero), inv_x as both
+    // inverse passes (inv_y zero)
+    for (int q0 = 0; q0 < nrhs; q0 += grp) {
+        const int st = fwd_xy(q0, std::min(grp, nrhs - q0));
+        if (st != OK) return st;
     }
-    MARIE_HIP(hipGetLastError());
-    if (tm) MARIE_HIP(hipEventRecord(tev[1], s));
-    // forward y: A[c][k][j<M][i] -> B[c][k][j<Py][i]
-    if (reg[1]) {
-        reg_strided<T>(false, Py, rc[1], s, A, B, twy, Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
-                       (int64_t)Py * Px, M, Py, (int64_t)N * M * Px, (int64_t)N * Py * Px, N, ncomp);
-    } else {
-        const int lg = pick_lg(Py, e[1] * NT, txmax);
-        const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, ncomp);
-        const size_t sm = (((size_t)Py << lg) + Py) * sizeof(c2<T>);    // tile + twiddles
-        MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e[1], RS, hipLaunchKernelGGL((k_strided<T, EM, false, RS>), g,
-                   dim3(NT), sm, s, A, B, twy, h->plan[1], Px, (int64_t)Px, (int64_t)M * Px, (int64_t)Px,
-                   (int64_t)Py * Px, M, Py, lg, (int64_t)N * M * Px, (int64_t)N * Py * Px)));
-    }
-    MARIE_HIP(hipGetLastError());
+    if (tm && grp < nrhs) MARIE_HIP(hipEventRecord(tev[1], s));
     if (tm) MARIE_HIP(hipEventRecord(tev[2], s));
     // z: fused forward, tensor product, inverse
     if (reg[2]) {
@@ -802,35 +863,10 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     }
     MARIE_HIP(hipGetLastError());
     if (tm) MARIE_HIP(hipEventRecord(tev[3], s));
-    // inverse y: B[c][k][j<Py][i] -> A[c][k][j<M][i]
-    if (reg[3]) {
-        reg_strided<T>(true, Py, rc[3], s, B, A, twy, Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
-                       (int64_t)M * Px, Py, M, (int64_t)N * Py * Px, (int64_t)N * M * Px, N, ncomp);
-    } else {
-        const int lg = pick_lg(Py, e[3] * NT, txmax);
-        const dim3 g((unsigned)((Px + (1 << lg) - 1) >> lg), N, ncomp);
-        const size_t sm = (((size_t)Py << lg) + Py) * sizeof(c2<T>);    // tile + twiddles
-        MARIE_RSET(h->plan[1], MARIE_EMAX_XY(T, e[3], RS, hipLaunchKernelGGL((k_strided<T, EM, true, RS>), g,
-                   dim3(NT), sm, s, B, A, twy, h->plan[1], Px, (int64_t)Px, (int64_t)Py * Px, (int64_t)Px,
-                   (int64_t)M * Px, Py, M, lg, (int64_t)N * Py * Px, (int64_t)N * M * Px)));
-    }
-    MARIE_HIP(hipGetLastError());
-    if (tm) MARIE_HIP(hipEventRecord(tev[4], s));
-    // inverse x + crop + epilogue
-    {
-        const T scale = T(1.0 / ((double)Px * (double)Py * (double)Pz));
-        if (reg[4]) {
-            reg_inv_x<T>(Px, rc[4], s, A, y, x, h->pos, Mr, Mc, twx, L, M, N, h->nidx, op, transp, (T)h->Gram, scale,
-                         ncomp);
-        } else {
-            const int lg = pick_lg(Px, e[4] * NT, 16);
-            const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), ncomp);
-            const size_t sm = (((size_t)Px << lg) + Px) * sizeof(c2<T>);    // tile + twiddles
-            MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e[4], RS, hipLaunchKernelGGL((k_inv_x<T, EM, RS>), g, dim3(NT),
-                       sm, s, A, y, x, h->pos, Mr, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, op, transp,
-                       (T)h->Gram, scale)));
-        }
-        MARIE_HIP(hipGetLastError());
+    if (tm && grp < nrhs) MARIE_HIP(hipEventRecord(tev[4], s));
+    for (int q0 = 0; q0 < nrhs; q0 += grp) {
+        const int st = inv_yx(q0, std::min(grp, nrhs - q0));
+        if (st != OK) return st;
     }
     if (tm) {
         MARIE_HIP(hipEventRecord(tev[5], s));

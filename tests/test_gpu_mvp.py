@@ -110,6 +110,47 @@ def test_multi_rhs_matches_single():
         assert _rel(Y[:, k], yr) <= TOL64
 
 
+@pytest.mark.parametrize("transp", [False, True])
+def test_multi_rhs_column_groups(transp, monkeypatch):
+    """A multi-column product whose x and y passes run in groups of columns
+    (MARIE_XY_GROUP=2 over 5 columns: groups of 2, 2, 1, the path large
+    grids take to keep each group's buffers in the last-level cache) equals
+    the oracle per column and, bit for bit, the one-group product, in FP64,
+    complex64 and through the allocation-free entry point."""
+    import torch
+    from marie_amd import lib as L_
+    L, M, N, nr = 10, 12, 8, 5
+    print("case nrhs=5 in column groups of 2, transp", transp, flush=True)
+    fN, Mr, Mc, idx, _ = _case(L, M, N, seed=17)
+    rng = np.random.default_rng(18)
+    X = rng.standard_normal((nr, idx.size)) + 1j * rng.standard_normal((nr, idx.size))
+    prec = L_.PREC_F64 | L_.PREC_F32
+    monkeypatch.setenv('MARIE_XY_GROUP', '2')
+    grouped = _op(L, M, N, None, fN, Mr, Mc, 1.5, idx, prec)
+    monkeypatch.setenv('MARIE_XY_GROUP', '0')
+    whole = _op(L, M, N, None, fN, Mr, Mc, 1.5, idx, prec)
+    dev = torch.device('cuda:0')
+    for p, dt in ((L_.PREC_F64, torch.complex128), (L_.PREC_F32, torch.complex64)):
+        Xd = torch.tensor(X, dtype=dt, device=dev)
+        out = {}
+        for name, op in (('grouped', grouped), ('whole', whole)):
+            Y = torch.empty_like(Xd)
+            op.apply_device(Xd.data_ptr(), Y.data_ptr(), transp=transp, prec=p, nrhs=nr)
+            torch.cuda.synchronize()
+            out[name] = Y.cpu().numpy()
+        assert np.array_equal(out['grouped'], out['whole'])
+        Yn = torch.empty_like(Xd)
+        grouped.apply_device_noalloc(Xd.data_ptr(), Yn.data_ptr(), transp=transp, prec=p, nrhs=nr)
+        torch.cuda.synchronize()
+        assert np.array_equal(Yn.cpu().numpy(), out['whole'])
+        tol = TOL64 if p == L_.PREC_F64 else TOL32
+        for k in range(nr):
+            yr = mvp.mv_AN(X[k], fN, Mr, Mc, 1.5, 'transp' if transp else 'notransp', idx)
+            assert _rel(out['grouped'][k], yr) <= tol, (p, k)
+    grouped.close()
+    whole.close()
+
+
 def test_unsorted_one_based_index():
     L, M, N = 6, 7, 5
     print("case unsorted 1-based idx", flush=True)
