@@ -170,7 +170,7 @@ int marie_vie_apply_device_noalloc(marie_vie_op* h, int op, int transp, int prec
                                    void* y_dev, int nrhs);
 size_t marie_vie_noalloc_scratch_bytes(void);
 
-/* Restarted GMRES(restart), 1 <= restart <= 511, on A_N with MARIE's pgmres semantics
+/* Restarted GMRES(restart), 1 <= restart <= 511 (<= 503 in mode 3), on A_N with MARIE's pgmres semantics
  * (src_numeric/src_iterative/pgmres.m:70-181, pgmres_iter.m:22-150):
  * stop when the least-squares residual < tol*||b||; flag/relres from the
  * true residual ||b - A x||/||b||; resvec relative to ||b|| (capacity
@@ -183,7 +183,14 @@ size_t marie_vie_noalloc_scratch_bytes(void);
  * refinement step is one GMRES(restart) cycle on the FP64 residual r, stopped
  * when its least-squares residual reaches max(inner_tol*||r||,
  * 0.9*tol*||b||) (at most 0.5*||r||); x and r = b - A x are then updated in
- * FP64, so the residual is replaced by the true one at every restart. */
+ * FP64, so the residual is replaced by the true one at every restart.
+ * 3 = the refinement of mode 2 with communication-avoiding (s-step) cycles:
+ * up to 8 complex64 products of a scaled power basis back to back, then one
+ * block Gram-Schmidt pass and CholeskyQR2 of the block on the device
+ * (complex64 basis, FP64 accumulation), the Hessenberg columns and Givens
+ * rotations on the host; at most `restart` products per refinement step and
+ * the same stopping rule. No counterpart in the reference (its pgmres is
+ * mode 0); iter[0] counts the last cycle's accepted products. */
 int marie_vie_gmres_device(marie_vie_op* h, const void* b_dev, void* x_dev, int restart, double tol,
                            int maxit, int mode, double inner_tol, int* flag, double* relres,
                            int* iter, double* resvec, int resvec_cap, int* nres);

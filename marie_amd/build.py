@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libmarie_vie.so')
 SOURCES = ['vie_capi.hip', 'vie_mvp.hip', 'vie_mvp_reg_f64.hip', 'vie_mvp_reg_f32.hip', 'vie_krylov.hip',
-           'vie_assembly.hip', 'vie_block.hip', 'vie_adjoint.hip', 'vie_excitation.hip', 'vie_bicgstab.hip', 'vie_directfn.hip']
+           'vie_assembly.hip', 'vie_block.hip', 'vie_adjoint.hip', 'vie_excitation.hip', 'vie_bicgstab.hip', 'vie_directfn.hip',
+           'vie_sstep.hip']
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',

@@ -438,16 +438,20 @@ extern "C" int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* cfg6) {
     }
     // accepted codes: the LDS-stage engine's elements per thread (positive;
     // a value between compiled variants runs the next larger one), the
-    // register engine's points per thread (negative) and, for the z pass, its
-    // one-group-per-component codes -(100 * TX + E)
+    // register engine's points per thread (negative), for the z pass its
+    // one-group-per-component codes -(100 * TX + E) and for the y passes the
+    // complex64 column-pair codes -(200 + E)
     static const int kReg[] = {4, 8, 16, 10, 12, 14, 20, 24, 28};
     static const int kZ3[] = {404, 408, 416, 804, 808, 816, 414};
+    static const int kPair[] = {204, 208, 216};
     for (int k = 0; k < 5; ++k) {
         const int v = cfg6[k];
         bool ok = v == 4 || v == 8 || v == 16 || v == 32;
         for (int r : kReg) ok = ok || v == -r;
         if (k == 2)
             for (int r : kZ3) ok = ok || v == -r;
+        if (k == 1 || k == 3)
+            for (int r : kPair) ok = ok || v == -r;
         if (!ok) {
             set_error("tile code " + std::to_string(v) + " of pass " + std::to_string(k) + " is not a compiled variant");
             return MARIE_ERR_ARG;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "vie_fft.hpp"
 
@@ -111,6 +112,10 @@ constexpr int64_t kXYGroupBytes = (int64_t)192 << 20;
 
 int ensure_work(Op* h, int nrhs);
 int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s);
+// oscale * (the product) for MARIE_OP_AN, N, L, K: the factor enters the
+// epilogue's coefficients (used to keep a Krylov power basis near unit norm)
+int apply_scaled(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s,
+                 double oscale);
 // allocation-free product on the per-device scratch array (vie_mvp.hip)
 int scratch_init(int device);
 size_t scratch_capacity();
@@ -125,5 +130,9 @@ int load_circ(Op* h, int kind, void* out64);
 int assemble_K(Op* h, double dx, double ko, int stage = -1, double* out = nullptr);
 int autotune(Op* h, int prec, int iters, double* best_ms5);
 int timing_flush(Op* h);   // folds the recorded ring rows into kernel_ms
+// refinement with s-step GMRES cycles on complex64 products (vie_sstep.hip)
+int sstep_refine(Op* h, hipStream_t s, int transp, const c2<double>* b, c2<double>* x, c2<double>* r, c2<double>* t,
+                 double bnorm, double rnorm, int m, double tol, int maxit, double inner_tol,
+                 std::vector<double>& resvec, int* p_out, int* outit_out);
 
 }  // namespace marie

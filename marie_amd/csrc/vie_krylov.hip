@@ -488,9 +488,10 @@ static int alloc_ws(Ctx& c, int m, int mode) {
     // complex64 basis columns start on 16-byte boundaries (even stride), so the
     // multi-dot reads two elements per 16-byte load
     c.ldv = mode == 2 ? ((n + 1) & ~(int64_t)1) : n;
-    int st = A(&c.ws.V, (size_t)(m + 1) * c.ldv * ev);
-    if (!st) st = A(&c.ws.w, n * ev);
-    if (!st && mode != 0) st = A(&c.ws.rT, n * ev);
+    // mode 3 keeps its complex64 basis in the s-step workspace (vie_sstep.hip)
+    int st = mode == 3 ? OK : A(&c.ws.V, (size_t)(m + 1) * c.ldv * ev);
+    if (!st && mode != 3) st = A(&c.ws.w, n * ev);
+    if (!st && (mode == 1 || mode == 2)) st = A(&c.ws.rT, n * ev);
     if (!st) st = A((void**)&c.ws.r, n * sizeof(c2<double>));
     if (!st) st = A((void**)&c.ws.t, n * sizeof(c2<double>));
     if (!st && mode == 1) st = A((void**)&c.ws.xs, n * sizeof(c2<float>));
@@ -628,8 +629,12 @@ int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxi
     } else if (mode == 1) {
         st = refine<double>(c, b, x, m, tol, maxit, inner_tol, bnorm, resvec, &p, &outit);
         if (st) return done(st);
-    } else {
+    } else if (mode == 2) {
         st = refine<float>(c, b, x, m, tol, maxit, inner_tol, bnorm, resvec, &p, &outit);
+        if (st) return done(st);
+    } else {
+        st = sstep_refine(c.h, c.s, c.transp, b, x, c.ws.r, c.ws.t, bnorm, rn, m, tol, maxit, inner_tol, resvec, &p,
+                          &outit);
         if (st) return done(st);
     }
     iter[0] = p;
@@ -1118,12 +1123,12 @@ static int gmres_entry(marie_vie_op* hh, const void* b, void* x, int restart, do
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
     MARIE_HIP(hipSetDevice(h->device));
     if (!b || !x || !flag || !relres || !iter || restart < 1 || restart > kMaxCols - 1 || maxit < 1 || !(tol > 0) ||
-        (mode < 0 || mode > 2)) {
-        set_error("invalid gmres arguments (restart must be 1..511, mode 0..2)");
+        (mode < 0 || mode > 3) || (mode == 3 && restart > 503)) {
+        set_error("invalid gmres arguments (restart must be 1..511, 1..503 in mode 3; mode 0..3)");
         return MARIE_ERR_ARG;
     }
     if (!(h->precisions & PREC_F64) || (mode != 0 && !(h->precisions & PREC_F32))) {
-        set_error("gmres needs an FP64 operator (and FP32 for modes 1 and 2)");
+        set_error("gmres needs an FP64 operator (and FP32 for modes 1-3)");
         return MARIE_ERR_STATE;
     }
     if (!h->have_fN || !h->have_mat || !h->have_idx) {

@@ -676,10 +676,11 @@ static int ensure_precond(Op* h, hipStream_t s) {
 // One product on work buffers Abuf ([3 nrhs][N][M][Px]) and Bbuf
 // ([3 nrhs][N][Py][Px]); every element a pass reads was written by an earlier
 // pass of the same product, so the buffers need no clearing. `timed`: record
-// the per-kernel events when the handle's timing is on.
+// the per-kernel events when the handle's timing is on. `oscale` multiplies
+// the result (folded into the epilogue's two coefficients, no extra pass).
 template <typename T>
 static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs, hipStream_t s, void* Abuf,
-                   void* Bbuf, bool timed) {
+                   void* Bbuf, bool timed, double oscale = 1.0) {
     const bool f64 = sizeof(T) == 8;
     const bool kop = (op == OP_K);
     const c2<T>* fN = reinterpret_cast<const c2<T>*>(kop ? (f64 ? h->fK64 : h->fK32) : (f64 ? h->fN64 : h->fN32));
@@ -705,7 +706,12 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
     bool reg[5];
     const int Pp[5] = {Px, Py, Pz, Py, Px};
     for (int k = 0; k < 5; ++k)
-        reg[k] = em[k] < 0 && (k == 2 ? reg_z_supported<T>(Pp[k], -em[k]) : reg_supported<T>(Pp[k], -em[k]));
+        reg[k] = em[k] < 0 && (k == 2   ? reg_z_supported<T>(Pp[k], -em[k])
+                               : (k & 1) ? reg_y_supported<T>(Pp[k], -em[k])
+                                         : reg_supported<T>(Pp[k], -em[k]));
+    // column pairs need an even x side (a pair never straddles two rows)
+    for (int k = 1; k < 5; k += 2)
+        if (reg[k] && -em[k] > 200 && (Px & 1)) reg[k] = false;
     // register-engine points per thread of each pass; a pass whose
     // mixed-radix side (96, 112, 160, 192, 224) has no plan for the requested
     // count takes that side's own (12, 14, 20 points; z: 12 and 10 where its
@@ -816,17 +822,18 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
         MARIE_HIP(hipGetLastError());
         if (tm && grp == nrhs) MARIE_HIP(hipEventRecord(tev[4], s));
         // inverse x + crop + epilogue
-        const T scale = T(1.0 / ((double)Px * (double)Py * (double)Pz));
+        const T scale = T(oscale / ((double)Px * (double)Py * (double)Pz));
+        const T gram = T(h->Gram * oscale);
         if (reg[4]) {
-            reg_inv_x<T>(Px, rc[4], s, Aq, yq, xq, h->pos, Mr, Mc, twx, L, M, N, h->nidx, op, transp, (T)h->Gram,
-                         scale, nc);
+            reg_inv_x<T>(Px, rc[4], s, Aq, yq, xq, h->pos, Mr, Mc, twx, L, M, N, h->nidx, op, transp, gram, scale,
+                         nc);
         } else {
             const int lg = pick_lg(Px, e[4] * NT, 16);
             const dim3 g((unsigned)((nline + (1 << lg) - 1) >> lg), nc);
             const size_t sm = (((size_t)Px << lg) + Px) * sizeof(c2<T>);    // tile + twiddles
             MARIE_RSET(h->plan[0], MARIE_EMAX_XY(T, e[4], RS, hipLaunchKernelGGL((k_inv_x<T, EM, RS>), g, dim3(NT),
                        sm, s, Aq, yq, xq, h->pos, Mr, Mc, twx, h->plan[0], L, M, N, lg, h->nidx, op, transp,
-                       (T)h->Gram, scale)));
+                       gram, scale)));
         }
         MARIE_HIP(hipGetLastError());
         return OK;
@@ -944,6 +951,22 @@ int apply(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs,
     return apply_t<float>(h, op, transp, (const c2<float>*)x, (c2<float>*)y, nrhs, s, h->A, h->B, true);
 }
 
+int apply_scaled(Op* h, int op, int transp, int prec, const void* x, void* y, int nrhs, hipStream_t s,
+                 double oscale) {
+    if (op == OP_P) {
+        set_error("the scaled product covers MARIE_OP_AN, N, L and K");
+        return ERR_ARG;
+    }
+    const int chk = apply_check(h, op, transp, prec, nrhs);
+    if (chk) return chk;
+    int st = ensure_work(h, nrhs);
+    if (st) return st;
+    if (prec == PREC_F64)
+        return apply_t<double>(h, op, transp, (const c2<double>*)x, (c2<double>*)y, nrhs, s, h->A, h->B, true,
+                               oscale);
+    return apply_t<float>(h, op, transp, (const c2<float>*)x, (c2<float>*)y, nrhs, s, h->A, h->B, true, oscale);
+}
+
 // ------------------------------------------------- allocation-free product
 // Work buffers of marie_vie_apply_device_noalloc: one file-scope device array
 // per GPU, held until the process ends. The host name `scratch` is not a
@@ -1056,6 +1079,12 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
             any = any || (f64 ? reg_supported<double>(Pp3[a], mixE[v]) : reg_supported<float>(Pp3[a], mixE[v]));
         if (!any) continue;
         for (int k = 0; k < 5; ++k) cands[nc][k] = -mixE[v];
+        ++nc;
+    }
+    // complex64 y passes on column pairs (code 200 + E), x and z passes at 16 points
+    if (!f64 && reg_y_supported<float>(h->Py, 216) && nc < 16) {
+        for (int k = 0; k < 5; ++k) cands[nc][k] = -16;
+        cands[nc][1] = cands[nc][3] = -216;
         ++nc;
     }
     // z pass with one thread group per component (code 100 * TX + E)

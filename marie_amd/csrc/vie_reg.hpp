@@ -9,6 +9,8 @@ template <typename T> bool reg_supported(int P, int E);
 // z pass codes: E (4, 8, 16) for the all-components kernel, 100 * TX + E for
 // the kernel with one thread group per component (TX = 4 or 8 columns)
 template <typename T> bool reg_z_supported(int P, int code);
+// y passes: also the complex64 column-pair codes 200 + E
+template <typename T> bool reg_y_supported(int P, int code);
 template <typename T>
 void reg_fwd_x(int P, int E, hipStream_t s, const c2<T>* x, c2<T>* A, const int* pos, const c2<T>* Mc,
                const c2<T>* tw, int L, int M, int N, int64_t nidx, int conj_in, int ncomp);

@@ -1,0 +1,799 @@
+// Communication-avoiding (s-step) GMRES cycles on complex64 products inside
+// FP64 iterative refinement: marie_vie_gmres_device mode 3.
+//
+// The refinement is that of modes 1-2 (pgmres.m's flag/relres on the FP64
+// true residual, the FP64 residual replaced at every restart); what differs
+// is the inner cycle. Classical Gram-Schmidt (pgmres_iter.m:106-107) reads
+// the whole basis twice per product; here the basis grows by blocks of up
+// to S columns:
+//
+//   p_0 = q_last,  p_j = (A p_{j-1}) / gamma   j = 1..S   S complex64 products back to back
+//   [C; G0] = [Q P]^H P                                   one pass over Q and P (matrix cores)
+//   R1 = chol(G0 - C^H C)                                 on the device (block Gram-Schmidt with
+//   P  = (P - Q C) R1^{-1}                                the Pythagorean Gram, one pass)
+//   R2 = chol(P^H P),  P = P R2^{-1}                      second CholeskyQR pass
+//
+// so the basis is read twice per block instead of twice per product, and the
+// products of a block run without Gram-Schmidt traffic between them. The
+// power basis p_j is kept near unit norm by a fixed gamma (the norm of the
+// solve's first product), folded into the product's epilogue. With
+// P = Q C + Q_new R2 R1, the Arnoldi relation A Q_k = Q_{k+1} H follows from
+// A p_{j-1} = gamma p_j column by column (host, FP64); the cycle's least
+// squares min || beta e_1 - H y || runs on the host (Householder, vie_dense.hpp).
+// A CholeskyQR pivot below 1e-10 of the column's squared norm (a power-basis
+// column the complex64 data no longer resolves) ends the block at that
+// column; the next block continues from the last accepted one.
+//
+// Every dot product accumulates in FP64 (the complex64 operands convert
+// exactly); the basis is stored in complex64.
+#include "../../include/marie_vie.h"
+#include "vie_handle.hpp"
+#include "vie_reduce.hpp"
+#include "vie_dense.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace marie {
+
+namespace {
+
+using cd = std::complex<double>;
+constexpr int ST = 256;          // threads per block
+constexpr int SG = 1024;         // max blocks of a reduction
+constexpr int SM = 8;            // largest block of products (columns of P)
+constexpr int SKR = 64;          // rows per staged chunk of the dot kernel
+constexpr int SGD = 1024;        // most row blocks of the dot kernel
+constexpr int kMaxBasis = 512;   // most basis columns a combination reads
+
+__device__ __forceinline__ double2 ld2(const c2<float>* p) {
+    const c2<float> v = *p;
+    return double2{(double)v.x, (double)v.y};
+}
+__device__ __forceinline__ double2 ld2(const c2<double>* p) {
+    const c2<double> v = *p;
+    return double2{v.x, v.y};
+}
+
+// part[(p * SM + q) * G + rb] = sum over the row chunks rb, rb + G, ... of
+// conj(V_p) W_q, p < P, q < Qn (<= SM = 8). V, W complex64 columns (stride
+// ldq). A workgroup (4 waves) takes 64 columns of V and the W block; each
+// chunk of SKR rows is staged in LDS, then the 4-row k-steps run on the
+// matrix cores (v_mfma_f64_16x16x4f64) with the W block packed as the 16
+// real columns [Re W | Im W]: D1 = Re(V)^T [Re W | Im W], D2 = Im(V)^T [Re W |
+// Im W], so conj(V)^T W = (D1[:, q] + D2[:, 8+q]) + j (D1[:, 8+q] - D2[:, q])
+// takes two real products per k-step. When the tile holds fewer than 64
+// columns, the waves of a 16-column subtile split its k-steps and add their
+// sums in LDS. Operand layout: lane i + 16 k holds A[i][k] and B[k][i];
+// D[r][c] is element r / 4 of lane c + 16 (r % 4).
+typedef double dx4 __attribute__((ext_vector_type(4)));
+constexpr int STP = 64;
+
+__global__ __launch_bounds__(ST) void k_sdot(const c2<float>* __restrict__ V, int P, const c2<float>* __restrict__ W,
+                                             int Qn, int64_t ldq, int64_t n, c2<double>* __restrict__ part, int G) {
+    static_assert(SM == 8, "the packed W operand holds 8 complex columns");
+    __shared__ __attribute__((aligned(32))) c2<float> Vs[SKR][STP + 1];
+    __shared__ c2<float> Ws[SKR][SM + 1];
+    const int p0 = blockIdx.x * STP;
+    const int rb = blockIdx.y;
+    const int pv = P - p0 < STP ? P - p0 : STP;
+    const int nsub = (pv + 15) / 16;                 // 16-column subtiles with data
+    const int wps = nsub == 1 ? 4 : (nsub == 2 ? 2 : 1);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int sub = wave / wps, kpart = wave % wps;
+    const bool active = sub < nsub;
+    const int li = lane & 15, lk = lane >> 4;
+    const int64_t nchunk = (n + SKR - 1) / SKR;
+    constexpr int LV = SKR * STP / ST;               // V elements staged per thread and chunk
+    constexpr int LW = SKR * SM / ST;                // W elements staged per thread and chunk
+    c2<float> vreg[LV], wreg[LW];
+    auto fetch = [&](int64_t ch) {
+        const int64_t r0 = ch * SKR;
+#pragma unroll
+        for (int k = 0; k < LV; ++k) {
+            const int i = threadIdx.x + ST * k;
+            const int r = i % SKR, pc = i / SKR;
+            const int64_t e = r0 + r;
+            vreg[k] = (pc < pv && e < n) ? V[(int64_t)(p0 + pc) * ldq + e] : c2<float>{0.f, 0.f};
+        }
+#pragma unroll
+        for (int k = 0; k < LW; ++k) {
+            const int i = threadIdx.x + ST * k;
+            const int r = i % SKR, qc = i / SKR;
+            const int64_t e = r0 + r;
+            wreg[k] = (qc < Qn && e < n) ? W[(int64_t)qc * ldq + e] : c2<float>{0.f, 0.f};
+        }
+    };
+    dx4 d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
+    if (rb < nchunk) fetch(rb);
+    for (int64_t ch = rb; ch < nchunk; ch += G) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < LV; ++k) {
+            const int i = threadIdx.x + ST * k;
+            Vs[i % SKR][i / SKR] = vreg[k];
+        }
+#pragma unroll
+        for (int k = 0; k < LW; ++k) {
+            const int i = threadIdx.x + ST * k;
+            Ws[i % SKR][i / SKR] = wreg[k];
+        }
+        __syncthreads();
+        if (ch + G < nchunk) fetch(ch + G);
+        if (active) {
+            for (int ks = kpart; ks < SKR / 4; ks += wps) {
+                const c2<float> a = Vs[4 * ks + lk][16 * sub + li];
+                const c2<float> w = Ws[4 * ks + lk][li & 7];
+                const double bv = li < 8 ? (double)w.x : (double)w.y;
+                d1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a.x, bv, d1, 0, 0, 0);
+                d2 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a.y, bv, d2, 0, 0, 0);
+            }
+        }
+    }
+    if (wps > 1) {
+        // the k-step sums of a subtile's waves, added in a fixed order (the
+        // staging area is free after the barrier)
+        __syncthreads();
+        dx4* red = reinterpret_cast<dx4*>(&Vs[0][0]);     // 4 waves x 64 lanes x 2 dx4 = 16 KB
+        red[(wave * 64 + lane) * 2] = d1;
+        red[(wave * 64 + lane) * 2 + 1] = d2;
+        __syncthreads();
+        if (kpart != 0) return;
+        for (int w2 = 1; w2 < wps; ++w2) {
+            d1 += red[((wave + w2) * 64 + lane) * 2];
+            d2 += red[((wave + w2) * 64 + lane) * 2 + 1];
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        // lanes li < 8 combine with lane li + 8 of the same k group
+        const double d1h = __shfl_down(d1[e], 8, 16), d2h = __shfl_down(d2[e], 8, 16);
+        const int pr = 16 * sub + 4 * e + lk;
+        if (li < 8 && pr < pv && li < Qn)
+            part[((int64_t)(p0 + pr) * SM + li) * G + rb] = c2<double>{d1[e] + d2h, d1h - d2[e]};
+    }
+}
+
+// C[p * SM + q] = sum_rb part[(p * SM + q) G + rb] (q < Qn; columns past Qn
+// are set to zero): one wave per output, fixed order
+__global__ __launch_bounds__(ST) void k_sdot_reduce(const c2<double>* __restrict__ part, int P, int Qn, int G,
+                                                    c2<double>* __restrict__ C) {
+    const int o = blockIdx.x * (ST / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (o >= P * SM) return;
+    const int q = o % SM;
+    double x = 0.0, y = 0.0;
+    if (q < Qn) {
+        const c2<double>* src = part + (int64_t)o * G;
+        for (int rb = lane; rb < G; rb += 64) {
+            x += src[rb].x;
+            y += src[rb].y;
+        }
+        for (int s = 32; s > 0; s >>= 1) {
+            x += __shfl_down(x, s, 64);
+            y += __shfl_down(y, s, 64);
+        }
+    }
+    if (lane == 0) C[o] = c2<double>{x, y};
+}
+
+// CholeskyQR factor of the block's Gram: G[c][d] = C[(k + c) SM + d] minus
+// (proj) sum_{i<k} conj(C[i SM + c]) C[i SM + d] (the Pythagorean form of
+// (P - Q C)^H (P - Q C) with C = Q^H P). R (upper, column-major SM x SM) with
+// G = R^H R, Rinv = R^{-1}; the factor stops at the first column whose pivot
+// is not above thresh * C[(k + j) SM + j] (the column's squared norm before
+// the projection): nok[0] = accepted columns; R and Rinv are zero past them.
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+    return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ double2 zcmul(double2 a, double2 b) {      // conj(a) b
+    return double2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+
+__global__ __launch_bounds__(ST) void k_schol(const c2<double>* __restrict__ C, int k, int sb, int proj, double thresh,
+                                              c2<double>* __restrict__ R, c2<double>* __restrict__ Rinv,
+                                              int* __restrict__ nok) {
+    __shared__ double2 Gs[SM][SM];
+    __shared__ double2 Rs[SM][SM];
+    __shared__ double2 Is[SM][SM];
+    __shared__ double d0[SM];
+    const int t = threadIdx.x;
+    if (t < SM * SM) {
+        const int c = t / SM, d = t % SM;
+        double2 g = {0.0, 0.0};
+        if (c < sb && d < sb) {
+            const c2<double> v = C[(int64_t)(k + c) * SM + d];
+            g = double2{v.x, v.y};
+            if (c == d) d0[c] = v.x;
+            if (proj) {
+                for (int i = 0; i < k; ++i) {
+                    const c2<double> a = C[(int64_t)i * SM + c], b = C[(int64_t)i * SM + d];
+                    const double2 p = zcmul(double2{a.x, a.y}, double2{b.x, b.y});
+                    g.x -= p.x;
+                    g.y -= p.y;
+                }
+            }
+        }
+        Gs[c][d] = g;
+        Rs[c][d] = double2{0.0, 0.0};
+        Is[c][d] = double2{0.0, 0.0};
+    }
+    __syncthreads();
+    if (t == 0) {
+        // R's diagonal is real and positive: every division is by a real pivot
+        double rinv[SM];
+        int ok = 0;
+        for (int j = 0; j < sb; ++j) {
+            double piv = Gs[j][j].x;
+            for (int i = 0; i < j; ++i) piv -= Rs[i][j].x * Rs[i][j].x + Rs[i][j].y * Rs[i][j].y;
+            if (!(piv > thresh * d0[j])) break;
+            const double rjj = sqrt(piv);
+            rinv[j] = 1.0 / rjj;
+            Rs[j][j] = double2{rjj, 0.0};
+            for (int l = j + 1; l < sb; ++l) {
+                double2 v = Gs[j][l];
+                for (int i = 0; i < j; ++i) {
+                    const double2 p = zcmul(Rs[i][j], Rs[i][l]);
+                    v.x -= p.x;
+                    v.y -= p.y;
+                }
+                Rs[j][l] = double2{v.x * rinv[j], v.y * rinv[j]};
+            }
+            ok = j + 1;
+        }
+        // the factor's entries past the accepted columns are zero
+        for (int j = 0; j < SM; ++j)
+            for (int l = 0; l < SM; ++l)
+                if (j >= ok || l >= ok) Rs[j][l] = double2{0.0, 0.0};
+        // inverse of the leading ok x ok upper triangle, column by column
+        for (int l = 0; l < ok; ++l) {
+            Is[l][l] = double2{rinv[l], 0.0};
+            for (int i = l - 1; i >= 0; --i) {
+                double2 v = {0.0, 0.0};
+                for (int j = i + 1; j <= l; ++j) {
+                    const double2 p = zmul(Rs[i][j], Is[j][l]);
+                    v.x += p.x;
+                    v.y += p.y;
+                }
+                Is[i][l] = double2{-v.x * rinv[i], -v.y * rinv[i]};
+            }
+        }
+        nok[0] = ok;
+    }
+    __syncthreads();
+    if (t < SM * SM) {
+        const int c = t % SM, d = t / SM;     // column-major: entry (c, d) at d * SM + c
+        R[d * SM + c] = c2<double>{Rs[c][d].x, Rs[c][d].y};
+        Rinv[d * SM + c] = c2<double>{Is[c][d].x, Is[c][d].y};
+    }
+}
+
+// P = (P - Q C) Rinv on the block's columns P_c = Q[k + c], c < sb (one thread
+// per row; Q columns i < k with coefficients C[i SM + c] read as uniform
+// operands; Rinv upper triangular, column-major). FP64 arithmetic on the
+// complex64 data, complex64 result in place.
+__global__ __launch_bounds__(ST) void k_supdate(c2<float>* __restrict__ Q, int64_t ldq, int k, int sb,
+                                                const c2<double>* __restrict__ C, const c2<double>* __restrict__ Rinv,
+                                                int proj, int64_t n) {
+    const int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x;
+    if (e >= n) return;
+    double ax[SM], ay[SM];
+#pragma unroll
+    for (int c = 0; c < SM; ++c) {
+        if (c < sb) {
+            const double2 v = ld2(Q + (int64_t)(k + c) * ldq + e);
+            ax[c] = v.x;
+            ay[c] = v.y;
+        } else {
+            ax[c] = 0.0;
+            ay[c] = 0.0;
+        }
+    }
+    if (proj) {
+        for (int i = 0; i < k; ++i) {
+            const double2 q = ld2(Q + (int64_t)i * ldq + e);
+            const c2<double>* ci = C + (int64_t)i * SM;
+#pragma unroll
+            for (int c = 0; c < SM; ++c) {
+                const c2<double> cc = ci[c];
+                ax[c] -= q.x * cc.x - q.y * cc.y;
+                ay[c] -= q.x * cc.y + q.y * cc.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int d = SM - 1; d >= 0; --d) {
+        if (d < sb) {
+            double ox = 0.0, oy = 0.0;
+#pragma unroll
+            for (int c = 0; c <= d; ++c) {
+                const c2<double> r = Rinv[d * SM + c];
+                ox += ax[c] * r.x - ay[c] * r.y;
+                oy += ax[c] * r.y + ay[c] * r.x;
+            }
+            Q[(int64_t)(k + d) * ldq + e] = c2<float>{(float)ox, (float)oy};
+        }
+    }
+}
+
+// x += sum_{i<k} Q_i y_i (FP64 accumulation, k <= kMaxBasis)
+__global__ __launch_bounds__(ST) void k_scomb(c2<double>* __restrict__ x, const c2<float>* __restrict__ Q, int64_t ldq,
+                                              int k, const c2<double>* __restrict__ y, int64_t n) {
+    __shared__ c2<double> cs[kMaxBasis];
+    for (int i = threadIdx.x; i < k; i += ST) cs[i] = y[i];
+    __syncthreads();
+    for (int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x; e < n; e += (int64_t)gridDim.x * ST) {
+        double2 acc = ld2(x + e);
+        for (int i = 0; i < k; ++i) {
+            const double2 a = ld2(Q + (int64_t)i * ldq + e);
+            const c2<double> c = cs[i];
+            acc.x += a.x * c.x - a.y * c.y;
+            acc.y += a.x * c.y + a.y * c.x;
+        }
+        x[e] = c2<double>{acc.x, acc.y};
+    }
+}
+
+template <typename TV>
+__global__ __launch_bounds__(ST) void k_snorm2(const c2<TV>* __restrict__ v, int64_t n, double* __restrict__ npart,
+                                               int G) {
+    __shared__ double sh[ST / 64];
+    double acc[1] = {0.0};
+    for (int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x; e < n; e += (int64_t)G * ST) {
+        const double2 a = ld2(v + e);
+        acc[0] += a.x * a.x + a.y * a.y;
+    }
+    block_reduce<1>(acc, sh);
+    if (threadIdx.x == 0) npart[blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(ST) void k_ssum(const double* __restrict__ p, int G, double* __restrict__ out) {
+    __shared__ double sh[ST / 64];
+    double acc[1] = {0.0};
+    for (int b = threadIdx.x; b < G; b += ST) acc[0] += p[b];
+    block_reduce<1>(acc, sh);
+    if (threadIdx.x == 0) out[0] = acc[0];
+}
+
+// q = a * r (complex64 from FP64), or from complex64 in place
+template <typename TI>
+__global__ void k_sscale(c2<float>* __restrict__ q, double a, const c2<TI>* __restrict__ r, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const double2 v = ld2(r + e);
+        q[e] = c2<float>{(float)(a * v.x), (float)(a * v.y)};
+    }
+}
+
+// r = b - t
+__global__ void k_ssub(c2<double>* __restrict__ r, const c2<double>* __restrict__ b, const c2<double>* __restrict__ t,
+                       int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        r[e] = c2<double>{b[e].x - t[e].x, b[e].y - t[e].y};
+}
+
+struct SWs {
+    c2<float>* Q = nullptr;      // (m + SM + 1) columns of ldq complex64
+    c2<double>* part = nullptr;  // k_sdot partials
+    c2<double>* C = nullptr;     // (m + SM + 1) x SM block coefficients (row-major), then SM x SM Gram
+    c2<double>* G2 = nullptr;    // SM x SM
+    c2<double>* R = nullptr;     // 4 SM x SM: R1, R1inv, R2, R2inv
+    c2<double>* y = nullptr;     // kMaxBasis solution coefficients
+    int* nok = nullptr;          // 2
+    double* npart = nullptr;
+    double* nout = nullptr;
+    void* hp = nullptr;          // pinned host mirror
+    size_t hp_bytes = 0;
+    hipEvent_t ev = nullptr;     // behind each block's copies to the mirror
+    void free_all() {
+        void* p[] = {Q, part, C, G2, R, y, nok, npart, nout};
+        for (void* q : p)
+            if (q) (void)hipFree(q);
+        if (hp) (void)hipHostFree(hp);
+        if (ev) (void)hipEventDestroy(ev);
+    }
+};
+
+struct SCtx {
+    Op* h;
+    hipStream_t s;
+    int64_t n;
+    int64_t ldq;
+    int transp;
+    int G;               // reduction blocks
+    int m;               // most columns multiplied per cycle
+    SWs w;
+};
+
+int s_grid(int64_t n) {
+    int64_t g = (n + ST * 4 - 1) / (ST * 4);
+    return (int)(g < 1 ? 1 : (g > SG ? SG : g));
+}
+
+template <typename TV> int s_norm(SCtx& c, const c2<TV>* v, double* out) {
+    hipLaunchKernelGGL(k_snorm2<TV>, dim3(c.G), dim3(ST), 0, c.s, v, c.n, c.w.npart, c.G);
+    hipLaunchKernelGGL(k_ssum, dim3(1), dim3(ST), 0, c.s, c.w.npart, c.G, c.w.nout);
+    MARIE_HIP(hipGetLastError());
+    double v2 = 0.0;
+    MARIE_HIP(hipMemcpyAsync(&v2, c.w.nout, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    MARIE_HIP(hipStreamSynchronize(c.s));
+    *out = std::sqrt(v2);
+    return OK;
+}
+
+// C (P x SM, row-major, columns past Qn zero) = Q[v0 : v0 + P)^H Q[w0 : w0 + Qn)
+int s_dot(SCtx& c, int v0, int P, int w0, int Qn, c2<double>* C) {
+    const int ptiles = (P + STP - 1) / STP;
+    // about 1024 workgroups over the column tiles
+    const int G = std::max(64, std::min(SGD, SGD / ptiles));
+    const dim3 grid(ptiles, G);
+    hipLaunchKernelGGL(k_sdot, grid, dim3(ST), 0, c.s, c.w.Q + (int64_t)v0 * c.ldq, P, c.w.Q + (int64_t)w0 * c.ldq,
+                       Qn, c.ldq, c.n, c.w.part, G);
+    hipLaunchKernelGGL(k_sdot_reduce, dim3((P * SM + ST / 64 - 1) / (ST / 64)), dim3(ST), 0, c.s, c.w.part, P, Qn,
+                       G, C);
+    MARIE_HIP(hipGetLastError());
+    return OK;
+}
+
+// H columns of one accepted block (see the file header): k basis columns
+// before the block, nk accepted products; C (k x SM host, row-major) the
+// projection coefficients, Rc = R2 R1 (column-major SM x SM)
+void s_hessenberg(dense::Mat& H, int ldh, int k, int nk, double gamma, const std::vector<cd>& C,
+                  const std::vector<cd>& Rc) {
+    const int kk = k + nk;
+    // coordinates of p_{j+1} in [Q Q_new]: rows < k from C, rows k.. from Rc
+    auto rfull = [&](int row, int j) -> cd { return row < k ? C[(size_t)row * SM + j] : Rc[(size_t)j * SM + (row - k)]; };
+    // A q_{k-1} = A p_0 = gamma p_1
+    for (int r = 0; r < kk; ++r) H[(size_t)(k - 1) * ldh + r] = gamma * rfull(r, 0);
+    for (int r = kk; r < ldh; ++r) H[(size_t)(k - 1) * ldh + r] = cd(0, 0);
+    // q_new_j = (p_{j+1} - Q C[:, j] - sum_{i<j} q_new_i Rc[i, j]) / Rc[j, j]
+    // A q_new_j = (gamma p_{j+2} - A Q C[:, j] - sum_{i<j} A q_new_i Rc[i, j]) / Rc[j, j]
+    for (int j = 0; j + 1 < nk; ++j) {
+        std::vector<cd> a(kk);
+        for (int r = 0; r < kk; ++r) a[r] = gamma * rfull(r, j + 1);
+        for (int i = 0; i < k; ++i) {
+            const cd cij = C[(size_t)i * SM + j];
+            if (cij == cd(0, 0)) continue;
+            for (int r = 0; r < kk; ++r) a[r] -= cij * H[(size_t)i * ldh + r];
+        }
+        for (int i = 0; i < j; ++i) {
+            const cd rij = Rc[(size_t)j * SM + i];
+            for (int r = 0; r < kk; ++r) a[r] -= rij * H[(size_t)(k + i) * ldh + r];
+        }
+        const cd rjj = Rc[(size_t)j * SM + j];
+        for (int r = 0; r < kk; ++r) H[(size_t)(k + j) * ldh + r] = a[r] / rjj;
+        for (int r = kk; r < ldh; ++r) H[(size_t)(k + j) * ldh + r] = cd(0, 0);
+    }
+}
+
+// The cycle's least squares min || beta e_1 - H y || by Givens rotations,
+// one Hessenberg column at a time (H is upper Hessenberg: the coordinates of
+// p_{j+1} reach row k + j only, R being upper triangular).
+struct SGivens {
+    int ld;
+    std::vector<double> cs;
+    std::vector<cd> sn, g, R;    // R: rotated columns, column-major ld x m
+    SGivens(int ld_, int m, double beta) : ld(ld_), cs(m, 1.0), sn(m, cd(0, 0)), g(m + 1, cd(0, 0)),
+                                           R((size_t)ld_ * m, cd(0, 0)) {
+        g[0] = cd(beta, 0);
+    }
+    // rotates column `col` (rows 0..col+1 of H) in; returns |residual| after it
+    double add(const dense::Mat& H, int ldh, int col) {
+        std::vector<cd> h(col + 2);
+        for (int i = 0; i <= col + 1; ++i) h[i] = H[(size_t)col * ldh + i];
+        for (int i = 0; i < col; ++i) {
+            const cd a = h[i], b = h[i + 1];
+            h[i] = cs[i] * a + sn[i] * b;
+            h[i + 1] = -std::conj(sn[i]) * a + cs[i] * b;
+        }
+        const cd a = h[col], b = h[col + 1];
+        const double na = std::abs(a), nb = std::abs(b);
+        if (nb == 0.0) {
+            cs[col] = 1.0;
+            sn[col] = cd(0, 0);
+        } else if (na == 0.0) {
+            cs[col] = 0.0;
+            sn[col] = std::conj(b) / nb;
+        } else {
+            const double tt = std::hypot(na, nb);
+            cs[col] = na / tt;
+            sn[col] = (a / na) * std::conj(b) / tt;
+        }
+        h[col] = cs[col] * a + sn[col] * b;
+        const cd gk = g[col];
+        g[col] = cs[col] * gk;
+        g[col + 1] = -std::conj(sn[col]) * gk;
+        for (int i = 0; i <= col; ++i) R[(size_t)col * ld + i] = h[i];
+        return std::abs(g[col + 1]);
+    }
+    // y = R^{-1} g over the first ncol columns; false on a zero pivot
+    bool solve(int ncol, std::vector<cd>& y) const {
+        y.assign(ncol, cd(0, 0));
+        for (int i = ncol - 1; i >= 0; --i) {
+            cd v = g[i];
+            for (int j = i + 1; j < ncol; ++j) v -= R[(size_t)j * ld + i] * y[j];
+            const cd d = R[(size_t)i * ld + i];
+            if (d == cd(0, 0)) return false;
+            y[i] = v / d;
+        }
+        return true;
+    }
+};
+
+// Products p_1..p_sb of the block that starts at basis column k (p_0 = q_{k-1}).
+int s_products(SCtx& c, int k, int sb, double& gamma) {
+    const size_t ldq = (size_t)c.ldq;
+    for (int j = 0; j < sb; ++j) {
+        const c2<float>* in = c.w.Q + (size_t)(k - 1 + j) * ldq;
+        c2<float>* out = c.w.Q + (size_t)(k + j) * ldq;
+        if (gamma <= 0.0) {
+            // the solve's first product sets the power basis' scale
+            int st = apply(c.h, OP_AN, c.transp, PREC_F32, in, out, 1, c.s);
+            if (st) return st;
+            double g = 0.0;
+            st = s_norm<float>(c, out, &g);
+            if (st) return st;
+            if (!(g > 0.0) || !std::isfinite(g)) {
+                set_error("the complex64 product of the residual direction is zero or not finite");
+                return ERR_ARG;
+            }
+            gamma = g;
+            hipLaunchKernelGGL(k_sscale<float>, dim3(c.G), dim3(ST), 0, c.s, out, 1.0 / gamma, out, c.n);
+            MARIE_HIP(hipGetLastError());
+        } else {
+            const int st = apply_scaled(c.h, OP_AN, c.transp, PREC_F32, in, out, 1, c.s, 1.0 / gamma);
+            if (st) return st;
+        }
+    }
+    return OK;
+}
+
+// Block Gram-Schmidt and CholeskyQR2 of the block at column k, then its
+// small matrices (C rows 0..k-1, R1, R1inv, R2, R2inv, accepted counts) to
+// the pinned mirror and an event behind them.
+int s_passes(SCtx& c, int k, int sb, hipEvent_t ev) {
+    c2<double>* R1 = c.w.R;
+    c2<double>* R1i = c.w.R + SM * SM;
+    c2<double>* R2 = c.w.R + 2 * SM * SM;
+    c2<double>* R2i = c.w.R + 3 * SM * SM;
+    // [C; G0] = [Q P]^H P, R1 = chol(G0 - C^H C), P = (P - Q C) R1^{-1}
+    int st = s_dot(c, 0, k + sb, k, sb, c.w.C);
+    if (st) return st;
+    hipLaunchKernelGGL(k_schol, dim3(1), dim3(ST), 0, c.s, c.w.C, k, sb, 1, 1e-10, R1, R1i, c.w.nok);
+    const unsigned gu = (unsigned)((c.n + ST - 1) / ST);
+    hipLaunchKernelGGL(k_supdate, dim3(gu), dim3(ST), 0, c.s, c.w.Q, c.ldq, k, sb, c.w.C, R1i, 1, c.n);
+    MARIE_HIP(hipGetLastError());
+    // R2 = chol(P^H P), P = P R2^{-1} (the Gram's rows start at row 0 of G2)
+    st = s_dot(c, k, sb, k, sb, c.w.G2);
+    if (st) return st;
+    hipLaunchKernelGGL(k_schol, dim3(1), dim3(ST), 0, c.s, c.w.G2, 0, sb, 0, 1e-10, R2, R2i, c.w.nok + 1);
+    hipLaunchKernelGGL(k_supdate, dim3(gu), dim3(ST), 0, c.s, c.w.Q, c.ldq, k, sb, c.w.C, R2i, 0, c.n);
+    MARIE_HIP(hipGetLastError());
+    unsigned char* hp = reinterpret_cast<unsigned char*>(c.w.hp);
+    const size_t cbytes = (size_t)k * SM * sizeof(c2<double>);
+    const size_t rbytes = (size_t)4 * SM * SM * sizeof(c2<double>);
+    MARIE_HIP(hipMemcpyAsync(hp, c.w.C, cbytes, hipMemcpyDeviceToHost, c.s));
+    MARIE_HIP(hipMemcpyAsync(hp + cbytes, c.w.R, rbytes, hipMemcpyDeviceToHost, c.s));
+    MARIE_HIP(hipMemcpyAsync(hp + cbytes + rbytes, c.w.nok, 2 * sizeof(int), hipMemcpyDeviceToHost, c.s));
+    MARIE_HIP(hipEventRecord(ev, c.s));
+    return OK;
+}
+
+// products the recent convergence rate predicts are still needed after a
+// residual `res` (0 when the rate is not known yet)
+int s_need(const std::vector<double>& hist, double res, double abstol) {
+    if ((int)hist.size() <= SM) return 0;
+    const double rho = std::pow(hist.back() / hist[hist.size() - 1 - SM], 1.0 / SM);
+    if (!(rho > 0.0 && rho < 1.0)) return 0;
+    if (res < abstol) return -1;
+    return (int)std::ceil(std::log(abstol / res) / std::log(rho));
+}
+
+// One cycle on the FP64 residual r (norm beta): the correction is added to x.
+// Stops when the least-squares residual is below abstol or m products are
+// done. Appends the least-squares residual after every accepted product.
+//
+// While the host turns block j's small matrices into Hessenberg columns and
+// Givens rotations, the device already runs block j+1's products (queued
+// behind block j's passes unless the convergence rate predicts that block j
+// ends the cycle); they are redone when block j stops early in its CholeskyQR.
+int s_cycle(SCtx& c, c2<double>* x, const c2<double>* r, double beta, double abstol, double& gamma, int* kout,
+            std::vector<double>& resv) {
+    const int m = c.m;
+    const int ldh = m + SM + 1;
+    hipLaunchKernelGGL(k_sscale<double>, dim3(c.G), dim3(ST), 0, c.s, c.w.Q, 1.0 / beta, r, c.n);
+    MARIE_HIP(hipGetLastError());
+    dense::Mat H((size_t)ldh * (m + SM), cd(0, 0));
+    SGivens gv(ldh, m + SM, beta);
+    std::vector<cd> y, Ch, R4((size_t)4 * SM * SM);
+    std::vector<double> hist;
+    hist.push_back(beta);
+    int k = 1;          // basis columns; q_{k-1} not yet multiplied
+    int sb = std::min(SM, m);
+    bool queued = false;   // the current block's products are already queued
+    int ncol = 0;
+    double res = beta;
+    while (k - 1 < m) {
+        int st = OK;
+        if (!queued) st = s_products(c, k, sb, gamma);
+        if (st) return st;
+        st = s_passes(c, k, sb, c.w.ev);
+        if (st) return st;
+        // queue the next block unless this one is predicted to end the cycle
+        int kn = k + sb, sbn = 0;
+        if (kn - 1 < m) {
+            const int need_now = s_need(hist, hist.back(), abstol);
+            if (need_now == 0 || need_now > sb) {
+                const int need_after = need_now == 0 ? SM : need_now - sb + 1;
+                sbn = std::max(1, std::min(std::min(SM, need_after), m - (kn - 1)));
+                st = s_products(c, kn, sbn, gamma);
+                if (st) return st;
+            }
+        }
+        MARIE_HIP(hipEventSynchronize(c.w.ev));
+        unsigned char* hp = reinterpret_cast<unsigned char*>(c.w.hp);
+        const size_t cbytes = (size_t)k * SM * sizeof(c2<double>);
+        const size_t rbytes = (size_t)4 * SM * SM * sizeof(c2<double>);
+        const c2<double>* hc = reinterpret_cast<const c2<double>*>(hp);
+        const c2<double>* hr = reinterpret_cast<const c2<double>*>(hp + cbytes);
+        const int* hn = reinterpret_cast<const int*>(hp + cbytes + rbytes);
+        const int nk = std::min(hn[0], hn[1]);
+        if (nk == 0) break;      // the next direction lies in the basis: the space is exhausted
+        Ch.resize((size_t)k * SM);
+        for (size_t i = 0; i < (size_t)k * SM; ++i) Ch[i] = cd(hc[i].x, hc[i].y);
+        for (size_t i = 0; i < R4.size(); ++i) R4[i] = cd(hr[i].x, hr[i].y);
+        // Rc = R2 R1 (upper triangular, leading nk x nk)
+        std::vector<cd> Rc((size_t)SM * SM, cd(0, 0));
+        for (int d = 0; d < nk; ++d)
+            for (int cc = 0; cc <= d; ++cc) {
+                cd v(0, 0);
+                for (int i = cc; i <= d; ++i) v += R4[2 * SM * SM + (size_t)i * SM + cc] * R4[(size_t)d * SM + i];
+                Rc[(size_t)d * SM + cc] = v;
+            }
+        s_hessenberg(H, ldh, k, nk, gamma, Ch, Rc);
+        const int k0 = k;
+        // least-squares residual after each accepted product (Hessenberg
+        // columns k0-1 .. k0+nk-2); a block with a non-finite column is dropped
+        bool finite = true;
+        for (int col = k0 - 1; col < k0 - 1 + nk && finite; ++col)
+            for (int rr = 0; rr <= col + 1; ++rr)
+                finite = finite && std::isfinite(H[(size_t)col * ldh + rr].real()) &&
+                         std::isfinite(H[(size_t)col * ldh + rr].imag());
+        if (!finite) break;
+        for (int col = k0 - 1; col < k0 - 1 + nk; ++col) {
+            const double rr = gv.add(H, ldh, col);
+            hist.push_back(rr);
+            resv.push_back(rr);
+        }
+        k += nk;
+        res = hist.back();
+        ncol = k - 1;
+        if (res < abstol) break;
+        if (nk == sb && sbn > 0) {
+            // the queued block starts where this one ended
+            sb = sbn;
+            queued = true;
+        } else {
+            // this block stopped early (or nothing was queued): size the next
+            // one by the rate and queue it from the last accepted column
+            const int need = s_need(hist, res, abstol);
+            const int stopped = nk < sb ? nk : SM;
+            sb = need > 0 ? std::min(SM, need + 1) : SM;
+            sb = std::max(1, std::min(std::min(sb, stopped), m - (k - 1)));
+            queued = false;
+        }
+    }
+    // products queued past the cycle's end only wrote basis columns past ncol
+    if (ncol == 0) {
+        MARIE_HIP(hipStreamSynchronize(c.s));
+        *kout = 0;
+        return OK;
+    }
+    if (!gv.solve(ncol, y)) {
+        set_error("the s-step cycle's least-squares problem is singular");
+        return ERR_NONFINITE;
+    }
+    std::vector<c2<double>> yh(ncol);
+    for (int i = 0; i < ncol; ++i) yh[i] = c2<double>{y[i].real(), y[i].imag()};
+    MARIE_HIP(hipMemcpyAsync(c.w.y, yh.data(), ncol * sizeof(c2<double>), hipMemcpyHostToDevice, c.s));
+    hipLaunchKernelGGL(k_scomb, dim3(c.G), dim3(ST), 0, c.s, x, c.w.Q, c.ldq, ncol, c.w.y, c.n);
+    MARIE_HIP(hipGetLastError());
+    MARIE_HIP(hipStreamSynchronize(c.s));
+    *kout = ncol;
+    return OK;
+}
+
+}  // namespace
+
+// Mode 3 of marie_vie_gmres_device: refinement cycles of s-step GMRES with
+// complex64 products. r holds b - A x on entry (FP64, norm rnorm), t is FP64
+// scratch; the inner-target rule is that of the other refined modes.
+int sstep_refine(Op* h, hipStream_t s, int transp, const c2<double>* b, c2<double>* x, c2<double>* r, c2<double>* t,
+                 double bnorm, double rnorm, int m, double tol, int maxit, double inner_tol,
+                 std::vector<double>& resvec, int* p_out, int* outit_out) {
+    SCtx c;
+    c.h = h;
+    c.s = s;
+    c.n = h->nidx;
+    c.ldq = (c.n + 1) & ~(int64_t)1;
+    c.transp = transp;
+    c.G = s_grid(c.n);
+    c.m = m;
+    if (m + SM + 1 > kMaxBasis) {
+        set_error("the s-step cycle takes at most " + std::to_string(kMaxBasis - SM - 1) + " products");
+        return ERR_ARG;
+    }
+    auto A = [&](void** p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            set_error("out of device memory for the s-step workspace");
+            return ERR_ALLOC;
+        }
+        return OK;
+    };
+    const int ncolmax = m + SM + 1;
+    int st = A((void**)&c.w.Q, (size_t)ncolmax * c.ldq * sizeof(c2<float>));
+    if (!st) st = A((void**)&c.w.part, (size_t)ncolmax * SM * SGD * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.w.C, (size_t)ncolmax * SM * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.w.G2, (size_t)SM * SM * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.w.R, (size_t)4 * SM * SM * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.w.y, (size_t)kMaxBasis * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.w.nok, 2 * sizeof(int));
+    if (!st) st = A((void**)&c.w.npart, SG * sizeof(double));
+    if (!st) st = A((void**)&c.w.nout, sizeof(double));
+    c.w.hp_bytes = ((size_t)ncolmax * SM + 4 * SM * SM) * sizeof(c2<double>) + 2 * sizeof(int);
+    if (!st && hipHostMalloc(&c.w.hp, c.w.hp_bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c.w.hp = nullptr;
+        set_error("pinned host allocation failed");
+        st = ERR_ALLOC;
+    }
+    if (!st && hipEventCreateWithFlags(&c.w.ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        c.w.ev = nullptr;
+        set_error("hipEventCreate failed");
+        st = ERR_HIP;
+    }
+    if (st) {
+        c.w.free_all();
+        return st;
+    }
+    double gamma = 0.0;
+    int outit = 0, p = 0;
+    double rcur = rnorm;
+    while (outit < maxit && rcur > tol * bnorm) {
+        ++outit;
+        double rel = 0.9 * tol * bnorm / rcur;
+        if (inner_tol > rel) rel = inner_tol;
+        if (rel > 0.5) rel = 0.5;
+        std::vector<double> rv;
+        st = s_cycle(c, x, r, rcur, rel * rcur, gamma, &p, rv);
+        if (st) break;
+        // r = b - A64 x
+        st = apply(h, OP_AN, transp, PREC_F64, x, t, 1, s);
+        if (st) break;
+        hipLaunchKernelGGL(k_ssub, dim3(c.G), dim3(ST), 0, s, r, b, t, c.n);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("kernel launch failed");
+            st = ERR_HIP;
+            break;
+        }
+        st = s_norm<double>(c, r, &rcur);
+        if (st) break;
+        for (double v : rv) resvec.push_back(v / bnorm);
+        resvec.push_back(rcur / bnorm);
+        if (p == 0) break;      // no progress possible in complex64
+    }
+    (void)hipStreamSynchronize(s);
+    c.w.free_all();
+    *p_out = p;
+    *outit_out = outit;
+    return st;
+}
+
+}  // namespace marie

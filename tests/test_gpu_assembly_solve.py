@@ -156,12 +156,12 @@ def test_gmres_zero_rhs_and_maxit_flag():
     assert fl == 1 and it[1] == 1
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_gmres_refined_modes_reach_tolerance(mode):
-    """complex64 matvecs (mode 1: FP64 basis, mode 2: complex64 basis) with FP64
-    refinement: true residual <= tol by the oracle's FP64 product, and the
-    solution within the FP64 solve's own error of the oracle's pgmres solution
-    converged to 1e-11."""
+    """complex64 matvecs (mode 1: FP64 basis, mode 2: complex64 basis, mode 3:
+    s-step cycles with a complex64 basis) with FP64 refinement: true residual
+    <= tol by the oracle's FP64 product, and the solution within the FP64
+    solve's own error of the oracle's pgmres solution converged to 1e-11."""
     from marie_amd import lib
     n, dx, f = 16, 6.25e-3, 298.2e6
     print("case gmres refined mode", mode, flush=True)
@@ -184,6 +184,46 @@ def test_gmres_refined_modes_reach_tolerance(mode):
     assert em_ <= max(5 * e64, 1e-6), (em_, e64)
     # resvec ends with the refined FP64 true residual
     assert abs(rvm[-1] - relm) <= 1e-9
+
+
+@pytest.mark.parametrize("restart", [3, 8, 13, 40])
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_gmres_sstep_restarts_and_adjoint(restart, adjoint):
+    """Mode 3 (blocks of up to 8 complex64 products, block Gram-Schmidt and
+    CholeskyQR on the device) with restarts below, at and between the block
+    size, on A and on A^H: the true residual by the oracle's FP64 product
+    (mv_AN 'notransp' / 'transp') is <= tol, the solution is within the FP64
+    solve's own error of the oracle's pgmres solution, and resvec ends with the
+    refined true residual."""
+    import torch
+    from marie_amd import lib
+    n, dx, f = 12, 6.25e-3, 298.2e6
+    print("case gmres s-step restart", restart, "adjoint", adjoint, flush=True)
+    r, e_r, Einc, em = _sphere_problem(n, dx, f)
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(Einc, e_r, dx, f, form=2)
+    op = lib.Operator(n, n, n, precisions=lib.PREC_F64 | lib.PREC_F32)
+    op.set_fN(fN)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx)
+    way = 'transp' if adjoint else 'notransp'
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, way, idx)
+    xref, fl, rel, _, _ = krylov.pgmres(A, b, 50, 1e-11, 200)
+    dev = torch.device('cuda:0')
+    bd = torch.from_numpy(b).to(dev)
+    out = {}
+    for mode in (0, 3):
+        xd = torch.zeros_like(bd)
+        call = op.gmres_adjoint_device if adjoint else op.gmres_device
+        out[mode] = call(bd.data_ptr(), xd.data_ptr(), restart=restart if mode else 50, tol=1e-6, maxit=400,
+                         mode=mode, inner_tol=1e-7) + (xd.cpu().numpy(),)
+    fl0, rel0, _, _, x0 = out[0]
+    fl3, rel3, it3, rv3, x3 = out[3]
+    assert fl == 0 and fl0 == 0 and fl3 == 0, (fl0, fl3, rel3)
+    assert rel3 <= 1e-6 and _rel(A(x3), b) <= 1e-6
+    assert _rel(x3, xref) <= max(5 * _rel(x0, xref), 1e-6)
+    assert abs(rv3[-1] - rel3) <= 1e-9
+    assert it3[0] <= restart
 
 
 def test_default_circulant_for_large_prime_factor():

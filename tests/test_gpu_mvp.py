@@ -314,6 +314,41 @@ def test_z_split_variants(prec, shape):
         assert _rel(op.apply(x, L_.OP_K), yk) <= tol, ('K', code)
 
 
+@pytest.mark.parametrize("shape", [(16, 16, 32), (20, 64, 12), (24, 128, 8)])
+def test_column_pair_y_pass(shape):
+    """complex64 y passes on column pairs (codes 200 + E: 16-byte accesses,
+    two lines per thread) give the oracle's products: both directions, two
+    right-hand sides, both fN storage forms; power-of-two y sides with an even
+    x side that is not always a power of two (40, 48)."""
+    from marie_amd import lib as L_
+    L, M, N = shape
+    print("case column pairs", shape, flush=True)
+    rng = np.random.default_rng(41)
+    G = rng.standard_normal((L, M, N, 6)) + 1j * rng.standard_normal((L, M, N, 6))
+    for c in range(6):
+        if oop.GP_COEFF['L'][c] < 0:
+            G[0, :, :, c] = 0
+        if oop.GP_COEFF['M'][c] < 0:
+            G[:, 0, :, c] = 0
+        if oop.GP_COEFF['N'][c] < 0:
+            G[:, :, 0, c] = 0
+    fN_sym = oop.fft_operator(oop.circulant_nop(G))
+    _, Mr, Mc, idx, x = _case(L, M, N, seed=42)
+    x2 = np.stack([x, rng.standard_normal(x.size) + 1j * rng.standard_normal(x.size)], axis=1)
+    fN_rand = _case(L, M, N, seed=43)[0]
+    for fN in (fN_sym, fN_rand):
+        op = _op(L, M, N, None, fN, Mr, Mc, 0.8, idx, prec=L_.PREC_F32)
+        yr = [mvp.mv_AN(x2[:, q], fN, Mr, Mc, 0.8, 'notransp', idx) for q in range(2)]
+        yt = mvp.mv_AN(x, fN, Mr, Mc, 0.8, 'transp', idx)
+        for code in (204, 208, 216):
+            for ex in (-8, 16):
+                op.set_tiles([ex, -code, 8, -code, ex, 0], L_.PREC_F32)
+                y = op.apply(x2)
+                for q in range(2):
+                    assert _rel(y[:, q], yr[q]) <= TOL32, (code, ex, q)
+                assert _rel(op.apply(x, L_.OP_AN, True), yt) <= TOL32, (code, ex)
+
+
 MIXED = [
     ((112, 96, 10), ([-14, -12, 8, -12, -14], [-28, -24, 8, -24, -28], [-28, -12, 8, -24, -14])),  # 224, 192
     ((48, 56, 10), ([-12, -14, 8, -14, -12], [-24, -28, 8, -28, -24], [-24, -14, 8, -28, -12])),    # 96, 112
