@@ -639,6 +639,13 @@ int gmres(Op* h, const c2<double>* b, c2<double>* x, int m, double tol, int maxi
     }
     iter[0] = p;
     iter[1] = outit;
+    if (mode != 0) {
+        // the refinement ends on the FP64 true residual of the returned x
+        // (r = b - A x, its norm the last resvec entry): pgmres.m:175's relres
+        *relres = resvec.back();
+        *flag = (*relres <= tol) ? 0 : 1;
+        return done(OK);
+    }
     // true residual (pgmres.m:175)
     st = matvec<double>(c, x, c.ws.t, false);
     if (st) return done(st);

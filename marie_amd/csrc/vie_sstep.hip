@@ -1,28 +1,29 @@
-// Communication-avoiding (s-step) GMRES cycles on complex64 products inside
-// FP64 iterative refinement: marie_vie_gmres_device mode 3.
+// Block-orthogonalised GMRES cycles on complex64 products inside FP64
+// iterative refinement: marie_vie_gmres_device mode 3.
 //
 // The refinement is that of modes 1-2 (pgmres.m's flag/relres on the FP64
 // true residual, the FP64 residual replaced at every restart); what differs
 // is the inner cycle. Classical Gram-Schmidt (pgmres_iter.m:106-107) reads
 // the whole basis twice per product; here the basis grows by blocks of up
-// to S columns:
+// to SM = 8 columns:
 //
-//   p_0 = q_last,  p_j = (A p_{j-1}) / gamma   j = 1..S   S complex64 products back to back
-//   [C; G0] = [Q P]^H P                                   one pass over Q and P (matrix cores)
-//   R1 = chol(G0 - C^H C)                                 on the device (block Gram-Schmidt with
-//   P  = (P - Q C) R1^{-1}                                the Pythagorean Gram, one pass)
-//   R2 = chol(P^H P),  P = P R2^{-1}                      second CholeskyQR pass
+//   b_0 = q_last; for j < S: w = A b_j, b_{j+1} = w minus its projection on
+//        b_0..b_j, normalised                  local Arnoldi (the block only)
+//   P = [b_1 .. b_S]
+//   [C; G0] = [Q P]^H P                        one pass over Q and P (matrix cores)
+//   R1 = chol(G0 - C^H C)                      on the device (block Gram-Schmidt with
+//   P  = (P - Q C) R1^{-1}                     the Pythagorean Gram, one pass)
+//   R2 = chol(P^H P),  P = P R2^{-1}           second CholeskyQR pass
 //
-// so the basis is read twice per block instead of twice per product, and the
-// products of a block run without Gram-Schmidt traffic between them. The
-// power basis p_j is kept near unit norm by a fixed gamma (the norm of the
-// solve's first product), folded into the product's epilogue. With
-// P = Q C + Q_new R2 R1, the Arnoldi relation A Q_k = Q_{k+1} H follows from
-// A p_{j-1} = gamma p_j column by column (host, FP64); the cycle's least
-// squares min || beta e_1 - H y || runs on the host (Householder, vie_dense.hpp).
-// A CholeskyQR pivot below 1e-10 of the column's squared norm (a power-basis
-// column the complex64 data no longer resolves) ends the block at that
-// column; the next block continues from the last accepted one.
+// so the whole basis is read twice per block instead of twice per product,
+// while every product still takes a unit vector orthogonal to its own block
+// (a well-conditioned block, unlike a power basis). With the local relation
+// A b_j = [b_0 .. b_{j+1}] h_j and P = Q C + Q_new R2 R1, the Arnoldi
+// relation A Q_k = Q_{k+1} H follows column by column (host, FP64); the
+// cycle's least squares min || beta e_1 - H y || is updated with Givens
+// rotations on the host. A CholeskyQR pivot below 1e-10 of the column's
+// squared norm ends the block at that column; the next block continues from
+// the last accepted one.
 //
 // Every dot product accumulates in FP64 (the complex64 operands convert
 // exactly); the basis is stored in complex64.
@@ -57,6 +58,14 @@ __device__ __forceinline__ double2 ld2(const c2<float>* p) {
 __device__ __forceinline__ double2 ld2(const c2<double>* p) {
     const c2<double> v = *p;
     return double2{v.x, v.y};
+}
+// one complex64 basis element, non-temporal: the block passes stream the
+// whole basis (far larger than the last-level cache) once each, and the
+// operator's octant and work buffers keep their place in the cache
+__device__ __forceinline__ c2<float> ldf_nt(const c2<float>* p) {
+    typedef float v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return c2<float>{v.x, v.y};
 }
 
 // part[(p * SM + q) * G + rb] = sum over the row chunks rb, rb + G, ... of
@@ -98,7 +107,7 @@ __global__ __launch_bounds__(ST) void k_sdot(const c2<float>* __restrict__ V, in
             const int i = threadIdx.x + ST * k;
             const int r = i % SKR, pc = i / SKR;
             const int64_t e = r0 + r;
-            vreg[k] = (pc < pv && e < n) ? V[(int64_t)(p0 + pc) * ldq + e] : c2<float>{0.f, 0.f};
+            vreg[k] = (pc < pv && e < n) ? ldf_nt(V + (int64_t)(p0 + pc) * ldq + e) : c2<float>{0.f, 0.f};
         }
 #pragma unroll
         for (int k = 0; k < LW; ++k) {
@@ -198,26 +207,37 @@ __device__ __forceinline__ double2 zcmul(double2 a, double2 b) {      // conj(a)
 __global__ __launch_bounds__(ST) void k_schol(const c2<double>* __restrict__ C, int k, int sb, int proj, double thresh,
                                               c2<double>* __restrict__ R, c2<double>* __restrict__ Rinv,
                                               int* __restrict__ nok) {
+    static_assert(ST == 4 * SM * SM, "four threads per Gram entry");
     __shared__ double2 Gs[SM][SM];
     __shared__ double2 Rs[SM][SM];
     __shared__ double2 Is[SM][SM];
+    __shared__ double2 red[4][SM * SM];
     __shared__ double d0[SM];
     const int t = threadIdx.x;
+    {
+        // entry (c, d) of C^H C: four threads over interleaved rows i
+        const int e = t % (SM * SM), q = t / (SM * SM);
+        const int c = e / SM, d = e % SM;
+        double2 acc = {0.0, 0.0};
+        if (proj && c < sb && d < sb) {
+            for (int i = q; i < k; i += 4) {
+                const c2<double> a = C[(int64_t)i * SM + c], b = C[(int64_t)i * SM + d];
+                const double2 p = zcmul(double2{a.x, a.y}, double2{b.x, b.y});
+                acc.x += p.x;
+                acc.y += p.y;
+            }
+        }
+        red[q][e] = acc;
+    }
+    __syncthreads();
     if (t < SM * SM) {
         const int c = t / SM, d = t % SM;
         double2 g = {0.0, 0.0};
         if (c < sb && d < sb) {
             const c2<double> v = C[(int64_t)(k + c) * SM + d];
-            g = double2{v.x, v.y};
+            g = double2{v.x - (red[0][t].x + red[1][t].x + red[2][t].x + red[3][t].x),
+                        v.y - (red[0][t].y + red[1][t].y + red[2][t].y + red[3][t].y)};
             if (c == d) d0[c] = v.x;
-            if (proj) {
-                for (int i = 0; i < k; ++i) {
-                    const c2<double> a = C[(int64_t)i * SM + c], b = C[(int64_t)i * SM + d];
-                    const double2 p = zcmul(double2{a.x, a.y}, double2{b.x, b.y});
-                    g.x -= p.x;
-                    g.y -= p.y;
-                }
-            }
         }
         Gs[c][d] = g;
         Rs[c][d] = double2{0.0, 0.0};
@@ -296,7 +316,8 @@ __global__ __launch_bounds__(ST) void k_supdate(c2<float>* __restrict__ Q, int64
     }
     if (proj) {
         for (int i = 0; i < k; ++i) {
-            const double2 q = ld2(Q + (int64_t)i * ldq + e);
+            const c2<float> qf = ldf_nt(Q + (int64_t)i * ldq + e);
+            const double2 q = double2{(double)qf.x, (double)qf.y};
             const c2<double>* ci = C + (int64_t)i * SM;
 #pragma unroll
             for (int c = 0; c < SM; ++c) {
@@ -318,6 +339,156 @@ __global__ __launch_bounds__(ST) void k_supdate(c2<float>* __restrict__ Q, int64
             }
             Q[(int64_t)(k + d) * ldq + e] = c2<float>{(float)ox, (float)oy};
         }
+    }
+}
+
+// ---- a block's local Arnoldi: each product's output orthogonalised against
+// the block's own columns Q[c0 .. c0 + nl) (the cycle's last multiplied
+// column and the block's earlier ones), so every product takes a unit vector
+// orthogonal to its block; the block's columns meet the rest of the basis in
+// the block Gram-Schmidt pass.
+constexpr int SL = SM + 1;       // most columns of a block's local orthogonalisation
+
+// The block's own columns were written moments before and are read through
+// the caches (the block, 8 columns, fits the last-level cache).
+__device__ __forceinline__ float4 ld4(const c2<float>* p) { return *reinterpret_cast<const float4*>(p); }
+
+// part[i * G + blk] = block sums of conj(Q_{c0+i}) w over its rows, i < nl,
+// and part[nl * G + blk] = block sum of |w|^2.
+// Columns start on 16-byte boundaries (even ldq): each thread reads two
+// adjacent elements per 16-byte load; the last element of an odd n is added
+// by thread 0 of block 0.
+__global__ __launch_bounds__(ST) void k_ldot(const c2<float>* __restrict__ Q, int64_t ldq, int c0, int nl,
+                                             const c2<float>* __restrict__ w, int64_t n, c2<double>* __restrict__ part,
+                                             int G) {
+    static_assert(ST == 256, "the block sums read four LDS entries per lane");
+    __shared__ double sh[(2 * SL + 1) * ST];
+    double acc[2 * SL + 1];
+#pragma unroll
+    for (int q = 0; q < 2 * SL + 1; ++q) acc[q] = 0.0;
+    const int64_t n2 = n >> 1;
+    const c2<float>* qb = Q + (int64_t)c0 * ldq;
+    for (int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x; e < n2; e += (int64_t)G * ST) {
+        // every column's load is issued before the first product uses one
+        float4 a[SL];
+#pragma unroll
+        for (int q = 0; q < SL; ++q)
+            if (q < nl) a[q] = ld4(qb + (int64_t)q * ldq + 2 * e);
+        const float4 wv = ld4(w + 2 * e);
+        acc[2 * SL] += (double)wv.x * wv.x + (double)wv.y * wv.y + (double)wv.z * wv.z + (double)wv.w * wv.w;
+#pragma unroll
+        for (int q = 0; q < SL; ++q) {
+            if (q < nl) {
+                acc[2 * q] += (double)a[q].x * wv.x + (double)a[q].y * wv.y + (double)a[q].z * wv.z +
+                              (double)a[q].w * wv.w;
+                acc[2 * q + 1] += (double)a[q].x * wv.y - (double)a[q].y * wv.x + (double)a[q].z * wv.w -
+                                  (double)a[q].w * wv.z;
+            }
+        }
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const double2 wv = ld2(w + n - 1);
+        acc[2 * SL] += wv.x * wv.x + wv.y * wv.y;
+        for (int q = 0; q < nl; ++q) {
+            const double2 a = ld2(Q + (int64_t)(c0 + q) * ldq + n - 1);
+            acc[2 * q] += a.x * wv.x + a.y * wv.y;
+            acc[2 * q + 1] += a.x * wv.y - a.y * wv.x;
+        }
+    }
+    // block sums through LDS: row q of sh holds every thread's acc[q]; wave w
+    // adds rows w, w + 4, ... (four LDS reads per lane, then a shuffle tree)
+    // and its lane 0 writes the row's real or imaginary part
+#pragma unroll
+    for (int q = 0; q < 2 * SL + 1; ++q) sh[q * ST + threadIdx.x] = acc[q];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double* pd = reinterpret_cast<double*>(part);
+    for (int q = wave; q < 2 * SL + 1; q += ST / 64) {
+        const bool used = q == 2 * SL || (q >> 1) < nl;
+        if (!used) continue;
+        double v = sh[q * ST + lane] + sh[q * ST + lane + 64] + sh[q * ST + lane + 128] + sh[q * ST + lane + 192];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) {
+            const int64_t slot = q == 2 * SL ? (int64_t)nl * G + blockIdx.x : (int64_t)(q >> 1) * G + blockIdx.x;
+            pd[2 * slot + (q == 2 * SL ? 0 : (q & 1))] = v;
+            if (q == 2 * SL) pd[2 * slot + 1] = 0.0;
+        }
+    }
+}
+
+// h[i] = sum_b part[i * G + b], fixed order; one block per i < nl, and block
+// nl writes the sum of row nl (|w|^2) to *w2
+__global__ __launch_bounds__(ST) void k_lsum(const c2<double>* __restrict__ part, int G, int nl,
+                                             c2<double>* __restrict__ h, c2<double>* __restrict__ w2) {
+    __shared__ double sh[ST / 64 * 2];
+    double acc[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < G; b += ST) {
+        const c2<double> p = part[(int64_t)blockIdx.x * G + b];
+        acc[0] += p.x;
+        acc[1] += p.y;
+    }
+    block_reduce<2>(acc, sh);
+    if (threadIdx.x == 0) (blockIdx.x < (unsigned)nl ? h[blockIdx.x] : *w2) = c2<double>{acc[0], acc[1]};
+}
+
+// out = (w - sum_{i<nl} Q_{c0+i} h[i]) / beta (FP64 arithmetic, complex64
+// result), with beta^2 = |w|^2 - sum |h_i|^2 (the columns are orthonormal:
+// the Pythagorean norm of the projected vector, so no second pass over it);
+// block 0 stores beta in h[nl], the local Hessenberg column's subdiagonal.
+// beta^2 below 1e-12 |w|^2 (w in the block's span to FP64 accuracy) gives a
+// zero column, where the block's CholeskyQR then stops. Two elements per
+// thread per 16-byte access (even ldq), the last element of an odd n by
+// thread 0 of block 0.
+__global__ __launch_bounds__(ST) void k_lcomb(c2<float>* __restrict__ out, const c2<float>* __restrict__ w,
+                                              const c2<float>* __restrict__ Q, int64_t ldq, int c0, int nl,
+                                              c2<double>* __restrict__ h, const c2<double>* __restrict__ w2,
+                                              int64_t n) {
+    __shared__ c2<double> cs[SL];
+    __shared__ double scale;
+    if (threadIdx.x < nl) cs[threadIdx.x] = h[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double ww = w2->x;
+        double b2 = ww;
+        for (int i = 0; i < nl; ++i) b2 -= cs[i].x * cs[i].x + cs[i].y * cs[i].y;
+        const double beta = b2 > 1e-12 * ww ? sqrt(b2) : 0.0;
+        scale = beta > 0.0 ? 1.0 / beta : 0.0;
+        if (blockIdx.x == 0) h[nl] = c2<double>{beta, 0.0};
+    }
+    __syncthreads();
+    const double a = scale;
+    const int64_t n2 = n >> 1;
+    const c2<float>* qb = Q + (int64_t)c0 * ldq;
+    for (int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x; e < n2; e += (int64_t)gridDim.x * ST) {
+        float4 qv[SL];
+#pragma unroll
+        for (int i = 0; i < SL; ++i)
+            if (i < nl) qv[i] = ld4(qb + (int64_t)i * ldq + 2 * e);
+        const float4 wv = ld4(w + 2 * e);
+        double ax = wv.x, ay = wv.y, bx = wv.z, by = wv.w;
+#pragma unroll
+        for (int i = 0; i < SL; ++i) {
+            if (i < nl) {
+                const float4 q = qv[i];
+                const c2<double> c = cs[i];
+                ax -= (double)q.x * c.x - (double)q.y * c.y;
+                ay -= (double)q.x * c.y + (double)q.y * c.x;
+                bx -= (double)q.z * c.x - (double)q.w * c.y;
+                by -= (double)q.z * c.y + (double)q.w * c.x;
+            }
+        }
+        const float4 o = {(float)(a * ax), (float)(a * ay), (float)(a * bx), (float)(a * by)};
+        *reinterpret_cast<float4*>(out + 2 * e) = o;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        double2 acc = ld2(w + n - 1);
+        for (int i = 0; i < nl; ++i) {
+            const double2 q = ld2(Q + (int64_t)(c0 + i) * ldq + n - 1);
+            const c2<double> c = cs[i];
+            acc.x -= q.x * c.x - q.y * c.y;
+            acc.y -= q.x * c.y + q.y * c.x;
+        }
+        out[n - 1] = c2<float>{(float)(a * acc.x), (float)(a * acc.y)};
     }
 }
 
@@ -386,11 +557,15 @@ struct SWs {
     int* nok = nullptr;          // 2
     double* npart = nullptr;
     double* nout = nullptr;
+    c2<float>* wt = nullptr;     // n: a product's output before its local orthogonalisation
+    c2<double>* lpart = nullptr; // SL x SG local dot partials
+    c2<double>* hloc = nullptr;  // SM x (SL + 1) local Hessenberg columns of a block
+    c2<double>* w2 = nullptr;    // |w|^2 of the product being orthogonalised
     void* hp = nullptr;          // pinned host mirror
     size_t hp_bytes = 0;
     hipEvent_t ev = nullptr;     // behind each block's copies to the mirror
     void free_all() {
-        void* p[] = {Q, part, C, G2, R, y, nok, npart, nout};
+        void* p[] = {Q, part, C, G2, R, y, nok, npart, nout, wt, lpart, hloc, w2};
         for (void* q : p)
             if (q) (void)hipFree(q);
         if (hp) (void)hipHostFree(hp);
@@ -439,22 +614,34 @@ int s_dot(SCtx& c, int v0, int P, int w0, int Qn, c2<double>* C) {
     return OK;
 }
 
-// H columns of one accepted block (see the file header): k basis columns
-// before the block, nk accepted products; C (k x SM host, row-major) the
-// projection coefficients, Rc = R2 R1 (column-major SM x SM)
-void s_hessenberg(dense::Mat& H, int ldh, int k, int nk, double gamma, const std::vector<cd>& C,
-                  const std::vector<cd>& Rc) {
+// H columns of one accepted block: k basis columns before the block, nk
+// accepted products. The block's local Arnoldi gives A b_j = sum_{l<=j+1}
+// Hl[j][l] b_l (b_0 = q_{k-1}, column-major ldl per j); the block pass gives
+// b_{j+1} = Q C[:, j] + Q_new Rc[:, j] (C: k x SM host, row-major; Rc = R2 R1
+// column-major SM x SM), so in [Q Q_new] coordinates
+//   A q_{k-1} = sum_l Hl[0][l] T_l,   T_0 = e_{k-1},  T_l = [C[:, l-1]; Rc[:, l-1]]
+//   A q_new_j = (A b_{j+1} - sum_{i<k} C[i][j] A q_i - sum_{i<j} Rc[i][j] A q_new_i) / Rc[j][j].
+void s_hessenberg(dense::Mat& H, int ldh, int k, int nk, const std::vector<cd>& Hl, int ldl,
+                  const std::vector<cd>& C, const std::vector<cd>& Rc) {
     const int kk = k + nk;
-    // coordinates of p_{j+1} in [Q Q_new]: rows < k from C, rows k.. from Rc
-    auto rfull = [&](int row, int j) -> cd { return row < k ? C[(size_t)row * SM + j] : Rc[(size_t)j * SM + (row - k)]; };
-    // A q_{k-1} = A p_0 = gamma p_1
-    for (int r = 0; r < kk; ++r) H[(size_t)(k - 1) * ldh + r] = gamma * rfull(r, 0);
-    for (int r = kk; r < ldh; ++r) H[(size_t)(k - 1) * ldh + r] = cd(0, 0);
-    // q_new_j = (p_{j+1} - Q C[:, j] - sum_{i<j} q_new_i Rc[i, j]) / Rc[j, j]
-    // A q_new_j = (gamma p_{j+2} - A Q C[:, j] - sum_{i<j} A q_new_i Rc[i, j]) / Rc[j, j]
+    auto tcoord = [&](int row, int l) -> cd {
+        if (l == 0) return row == k - 1 ? cd(1, 0) : cd(0, 0);
+        return row < k ? C[(size_t)row * SM + (l - 1)] : Rc[(size_t)(l - 1) * SM + (row - k)];
+    };
+    // A b_j in [Q Q_new] coordinates (j <= nk - 1 uses T_0 .. T_{j+1}, T_nk at most)
+    auto ab = [&](int j, std::vector<cd>& a) {
+        a.assign(kk, cd(0, 0));
+        for (int l = 0; l <= j + 1; ++l) {
+            const cd hl = Hl[(size_t)j * ldl + l];
+            if (hl == cd(0, 0)) continue;
+            for (int r = 0; r < kk; ++r) a[r] += hl * tcoord(r, l);
+        }
+    };
+    std::vector<cd> a;
+    ab(0, a);
+    for (int r = 0; r < ldh; ++r) H[(size_t)(k - 1) * ldh + r] = r < kk ? a[r] : cd(0, 0);
     for (int j = 0; j + 1 < nk; ++j) {
-        std::vector<cd> a(kk);
-        for (int r = 0; r < kk; ++r) a[r] = gamma * rfull(r, j + 1);
+        ab(j + 1, a);
         for (int i = 0; i < k; ++i) {
             const cd cij = C[(size_t)i * SM + j];
             if (cij == cd(0, 0)) continue;
@@ -465,8 +652,7 @@ void s_hessenberg(dense::Mat& H, int ldh, int k, int nk, double gamma, const std
             for (int r = 0; r < kk; ++r) a[r] -= rij * H[(size_t)(k + i) * ldh + r];
         }
         const cd rjj = Rc[(size_t)j * SM + j];
-        for (int r = 0; r < kk; ++r) H[(size_t)(k + j) * ldh + r] = a[r] / rjj;
-        for (int r = kk; r < ldh; ++r) H[(size_t)(k + j) * ldh + r] = cd(0, 0);
+        for (int r = 0; r < ldh; ++r) H[(size_t)(k + j) * ldh + r] = r < kk ? a[r] / rjj : cd(0, 0);
     }
 }
 
@@ -524,30 +710,23 @@ struct SGivens {
     }
 };
 
-// Products p_1..p_sb of the block that starts at basis column k (p_0 = q_{k-1}).
-int s_products(SCtx& c, int k, int sb, double& gamma) {
+// The block's sb products and their local Arnoldi: b_0 = q_{k-1} = Q[k-1],
+// w = A b_j, h = [b_0 .. b_j]^H w, Q[k + j] = b_{j+1} = (w - [b_0 .. b_j] h) / ||.||
+// (local Hessenberg column j of the block in hloc, on the device).
+int s_products(SCtx& c, int k, int sb) {
     const size_t ldq = (size_t)c.ldq;
     for (int j = 0; j < sb; ++j) {
         const c2<float>* in = c.w.Q + (size_t)(k - 1 + j) * ldq;
         c2<float>* out = c.w.Q + (size_t)(k + j) * ldq;
-        if (gamma <= 0.0) {
-            // the solve's first product sets the power basis' scale
-            int st = apply(c.h, OP_AN, c.transp, PREC_F32, in, out, 1, c.s);
-            if (st) return st;
-            double g = 0.0;
-            st = s_norm<float>(c, out, &g);
-            if (st) return st;
-            if (!(g > 0.0) || !std::isfinite(g)) {
-                set_error("the complex64 product of the residual direction is zero or not finite");
-                return ERR_ARG;
-            }
-            gamma = g;
-            hipLaunchKernelGGL(k_sscale<float>, dim3(c.G), dim3(ST), 0, c.s, out, 1.0 / gamma, out, c.n);
-            MARIE_HIP(hipGetLastError());
-        } else {
-            const int st = apply_scaled(c.h, OP_AN, c.transp, PREC_F32, in, out, 1, c.s, 1.0 / gamma);
-            if (st) return st;
-        }
+        const int st = apply(c.h, OP_AN, c.transp, PREC_F32, in, c.w.wt, 1, c.s);
+        if (st) return st;
+        c2<double>* hj = c.w.hloc + (size_t)j * (SL + 1);
+        hipLaunchKernelGGL(k_ldot, dim3(c.G), dim3(ST), 0, c.s, c.w.Q, c.ldq, k - 1, j + 1, c.w.wt, c.n, c.w.lpart,
+                           c.G);
+        hipLaunchKernelGGL(k_lsum, dim3(j + 2), dim3(ST), 0, c.s, c.w.lpart, c.G, j + 1, hj, c.w.w2);
+        hipLaunchKernelGGL(k_lcomb, dim3(c.G), dim3(ST), 0, c.s, out, c.w.wt, c.w.Q, c.ldq, k - 1, j + 1, hj,
+                           c.w.w2, c.n);
+        MARIE_HIP(hipGetLastError());
     }
     return OK;
 }
@@ -576,9 +755,11 @@ int s_passes(SCtx& c, int k, int sb, hipEvent_t ev) {
     unsigned char* hp = reinterpret_cast<unsigned char*>(c.w.hp);
     const size_t cbytes = (size_t)k * SM * sizeof(c2<double>);
     const size_t rbytes = (size_t)4 * SM * SM * sizeof(c2<double>);
+    const size_t lbytes = (size_t)SM * (SL + 1) * sizeof(c2<double>);
     MARIE_HIP(hipMemcpyAsync(hp, c.w.C, cbytes, hipMemcpyDeviceToHost, c.s));
     MARIE_HIP(hipMemcpyAsync(hp + cbytes, c.w.R, rbytes, hipMemcpyDeviceToHost, c.s));
-    MARIE_HIP(hipMemcpyAsync(hp + cbytes + rbytes, c.w.nok, 2 * sizeof(int), hipMemcpyDeviceToHost, c.s));
+    MARIE_HIP(hipMemcpyAsync(hp + cbytes + rbytes, c.w.hloc, lbytes, hipMemcpyDeviceToHost, c.s));
+    MARIE_HIP(hipMemcpyAsync(hp + cbytes + rbytes + lbytes, c.w.nok, 2 * sizeof(int), hipMemcpyDeviceToHost, c.s));
     MARIE_HIP(hipEventRecord(ev, c.s));
     return OK;
 }
@@ -601,7 +782,7 @@ int s_need(const std::vector<double>& hist, double res, double abstol) {
 // Givens rotations, the device already runs block j+1's products (queued
 // behind block j's passes unless the convergence rate predicts that block j
 // ends the cycle); they are redone when block j stops early in its CholeskyQR.
-int s_cycle(SCtx& c, c2<double>* x, const c2<double>* r, double beta, double abstol, double& gamma, int* kout,
+int s_cycle(SCtx& c, c2<double>* x, const c2<double>* r, double beta, double abstol, int* kout,
             std::vector<double>& resv) {
     const int m = c.m;
     const int ldh = m + SM + 1;
@@ -609,7 +790,7 @@ int s_cycle(SCtx& c, c2<double>* x, const c2<double>* r, double beta, double abs
     MARIE_HIP(hipGetLastError());
     dense::Mat H((size_t)ldh * (m + SM), cd(0, 0));
     SGivens gv(ldh, m + SM, beta);
-    std::vector<cd> y, Ch, R4((size_t)4 * SM * SM);
+    std::vector<cd> y, Ch, R4((size_t)4 * SM * SM), Hl((size_t)SM * (SL + 1));
     std::vector<double> hist;
     hist.push_back(beta);
     int k = 1;          // basis columns; q_{k-1} not yet multiplied
@@ -619,7 +800,7 @@ int s_cycle(SCtx& c, c2<double>* x, const c2<double>* r, double beta, double abs
     double res = beta;
     while (k - 1 < m) {
         int st = OK;
-        if (!queued) st = s_products(c, k, sb, gamma);
+        if (!queued) st = s_products(c, k, sb);
         if (st) return st;
         st = s_passes(c, k, sb, c.w.ev);
         if (st) return st;
@@ -630,7 +811,7 @@ int s_cycle(SCtx& c, c2<double>* x, const c2<double>* r, double beta, double abs
             if (need_now == 0 || need_now > sb) {
                 const int need_after = need_now == 0 ? SM : need_now - sb + 1;
                 sbn = std::max(1, std::min(std::min(SM, need_after), m - (kn - 1)));
-                st = s_products(c, kn, sbn, gamma);
+                st = s_products(c, kn, sbn);
                 if (st) return st;
             }
         }
@@ -638,9 +819,11 @@ int s_cycle(SCtx& c, c2<double>* x, const c2<double>* r, double beta, double abs
         unsigned char* hp = reinterpret_cast<unsigned char*>(c.w.hp);
         const size_t cbytes = (size_t)k * SM * sizeof(c2<double>);
         const size_t rbytes = (size_t)4 * SM * SM * sizeof(c2<double>);
+        const size_t lbytes = (size_t)SM * (SL + 1) * sizeof(c2<double>);
         const c2<double>* hc = reinterpret_cast<const c2<double>*>(hp);
         const c2<double>* hr = reinterpret_cast<const c2<double>*>(hp + cbytes);
-        const int* hn = reinterpret_cast<const int*>(hp + cbytes + rbytes);
+        const c2<double>* hl = reinterpret_cast<const c2<double>*>(hp + cbytes + rbytes);
+        const int* hn = reinterpret_cast<const int*>(hp + cbytes + rbytes + lbytes);
         const int nk = std::min(hn[0], hn[1]);
         if (nk == 0) break;      // the next direction lies in the basis: the space is exhausted
         Ch.resize((size_t)k * SM);
@@ -654,7 +837,8 @@ int s_cycle(SCtx& c, c2<double>* x, const c2<double>* r, double beta, double abs
                 for (int i = cc; i <= d; ++i) v += R4[2 * SM * SM + (size_t)i * SM + cc] * R4[(size_t)d * SM + i];
                 Rc[(size_t)d * SM + cc] = v;
             }
-        s_hessenberg(H, ldh, k, nk, gamma, Ch, Rc);
+        for (size_t i = 0; i < Hl.size(); ++i) Hl[i] = cd(hl[i].x, hl[i].y);
+        s_hessenberg(H, ldh, k, nk, Hl, SL + 1, Ch, Rc);
         const int k0 = k;
         // least-squares residual after each accepted product (Hessenberg
         // columns k0-1 .. k0+nk-2); a block with a non-finite column is dropped
@@ -746,7 +930,11 @@ int sstep_refine(Op* h, hipStream_t s, int transp, const c2<double>* b, c2<doubl
     if (!st) st = A((void**)&c.w.nok, 2 * sizeof(int));
     if (!st) st = A((void**)&c.w.npart, SG * sizeof(double));
     if (!st) st = A((void**)&c.w.nout, sizeof(double));
-    c.w.hp_bytes = ((size_t)ncolmax * SM + 4 * SM * SM) * sizeof(c2<double>) + 2 * sizeof(int);
+    if (!st) st = A((void**)&c.w.wt, (size_t)c.ldq * sizeof(c2<float>));
+    if (!st) st = A((void**)&c.w.lpart, (size_t)(SL + 1) * SG * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.w.w2, sizeof(c2<double>));
+    if (!st) st = A((void**)&c.w.hloc, (size_t)SM * (SL + 1) * sizeof(c2<double>));
+    c.w.hp_bytes = ((size_t)ncolmax * SM + 4 * SM * SM + SM * (SL + 1)) * sizeof(c2<double>) + 2 * sizeof(int);
     if (!st && hipHostMalloc(&c.w.hp, c.w.hp_bytes, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         c.w.hp = nullptr;
@@ -763,7 +951,6 @@ int sstep_refine(Op* h, hipStream_t s, int transp, const c2<double>* b, c2<doubl
         c.w.free_all();
         return st;
     }
-    double gamma = 0.0;
     int outit = 0, p = 0;
     double rcur = rnorm;
     while (outit < maxit && rcur > tol * bnorm) {
@@ -772,7 +959,7 @@ int sstep_refine(Op* h, hipStream_t s, int transp, const c2<double>* b, c2<doubl
         if (inner_tol > rel) rel = inner_tol;
         if (rel > 0.5) rel = 0.5;
         std::vector<double> rv;
-        st = s_cycle(c, x, r, rcur, rel * rcur, gamma, &p, rv);
+        st = s_cycle(c, x, r, rcur, rel * rcur, &p, rv);
         if (st) break;
         // r = b - A64 x
         st = apply(h, OP_AN, transp, PREC_F64, x, t, 1, s);
