@@ -8,9 +8,10 @@ form II, complex128, 256^3 x 6 circulant assembled on the device.
 
 A step is one matvec y = A_N x of the JVIE operator over the compact
 non-air DOFs (mv_AN 'notransp'), x and y resident in HBM. `value` is FP64
-matvecs/s; the same line carries FP32 matvecs/s, GMRES(50) time-to-1e-6
-(true residual) for plain FP64 and for complex64 matvecs with FP64 iterative
-refinement, the refined solution's error against the FP64 solution, the
+matvecs/s; the same line carries FP32 matvecs/s, time-to-1e-6 (true
+residual) for plain FP64 GMRES(50) and for complex64 matvecs in
+block-orthogonalised cycles with FP64 iterative refinement (GMRES mode 3),
+the refined solution's error against the FP64 solution, the
 per-kernel HIP-event timings with their HBM rates, the roofline of the
 dominant kernel and a CPU baseline (the oracle's numpy/scipy restatement of
 mv_AN on a bounded sample).
@@ -76,6 +77,29 @@ def solve_bytes(cycle_lengths, nidx, bmv, s):
     for m in cycle_lengths:
         for k in range(1, m + 1):
             tot += bmv + s * (2 * k + 3) * nidx
+    return tot
+
+
+def block_solve_bytes(cycle_lengths, nidx, bmv, s, sblk=8):
+    """Algorithmic HBM bytes of a refined solve with block-orthogonalised
+    cycles (GMRES mode 3): per product bmv plus its local Gram-Schmidt
+    against the j + 1 columns of its block, s (2 (j + 1) + 3) nidx (dots read
+    the columns and w, the combination reads them and w again and writes the
+    new column); per block of sb products with k basis columns before it,
+    s (2 k + 6 sb) nidx (the block's dots read k + sb columns, its Gram sb,
+    the projection k + sb and writes sb, the second CholeskyQR reads and
+    writes sb)."""
+    tot = 0
+    for m in cycle_lengths:
+        k = 1
+        done = 0
+        while done < m:
+            sb = min(sblk, m - done)
+            for j in range(sb):
+                tot += bmv + s * (2 * (j + 1) + 3) * nidx
+            tot += s * (2 * k + 6 * sb) * nidx
+            k += sb
+            done += sb
     return tot
 
 
@@ -272,11 +296,11 @@ def main():
 
     if args.profile_solve:
         b = torch.from_numpy(np.ascontiguousarray(Vrhs)).to(dev)
-        for mode in (0, 2):
+        for mode, rs in ((0, 50), (3, 128)):
             xs = torch.zeros_like(b)
             torch.cuda.synchronize()
             t = time.perf_counter()
-            flag, relres, it, rv = op.gmres_device(b.data_ptr(), xs.data_ptr(), restart=50, tol=1e-6, maxit=200,
+            flag, relres, it, rv = op.gmres_device(b.data_ptr(), xs.data_ptr(), restart=rs, tol=1e-6, maxit=200,
                                                    mode=mode, inner_tol=1e-7)
             torch.cuda.synchronize()
             log('mode %d solve %.3f s, %d resvec entries, iter %s, relres %.2e' % (
@@ -346,29 +370,36 @@ def main():
     if not args.no_solve:
         b = torch.from_numpy(np.ascontiguousarray(Vrhs)).to(dev)
 
-        def solve(mode, tol, inner_tol=1e-7):
+        def solve(mode, tol, inner_tol=1e-7, restart=50):
             xs = torch.zeros_like(b)
             torch.cuda.synchronize()
             t = time.perf_counter()
-            flag, relres, it, rv = op.gmres_device(b.data_ptr(), xs.data_ptr(), restart=50, tol=tol, maxit=200,
+            flag, relres, it, rv = op.gmres_device(b.data_ptr(), xs.data_ptr(), restart=restart, tol=tol, maxit=200,
                                                    mode=mode, inner_tol=inner_tol)
             torch.cuda.synchronize()
-            # resvec holds one entry per inner iteration plus, in the refined
-            # modes, the true residual after each of the it[1] cycles
+            dt = time.perf_counter() - t
+            # resvec holds one entry per inner iteration (product) plus, in the
+            # refined modes, the true residual after each of the it[1] cycles
             nit = len(rv) - 1 - (it[1] if mode else 0)
-            return xs, flag, relres, nit, time.perf_counter() - t, None, it
+            return xs, flag, relres, nit, dt, None, it
 
         # tightly converged FP64 solution: the reference for the error of each 1e-6 solve
         xref, _, rel_ref, it_ref, t_ref, _, _ = solve(0, 1e-10)
         log('fp64 reference solve to 1e-10: %.3f s, %d iterations, relres %.2e' % (t_ref, it_ref, rel_ref))
         nref = torch.linalg.norm(xref)
         res = {}
-        for mode, name in ((0, 'fp64'), (2, 'fp32_refined'), (1, 'fp32_refined_fp64basis')):
-            xs, flag, relres, nit, dt, _, itc = solve(mode, 1e-6)
+        # the refined solve: complex64 products in block-orthogonalised cycles
+        # of up to 128 (mode 3); the classical Gram-Schmidt refinements (modes
+        # 2 and 1, GMRES(50)) beside it
+        refined_restart = 128
+        for mode, name, rs in ((0, 'fp64', 50), (3, 'fp32_refined', refined_restart),
+                               (2, 'fp32_refined_cgs', 50), (1, 'fp32_refined_fp64basis', 50)):
+            solve(mode, 1e-6, restart=rs)        # warm: the workspace allocation is not part of the timing
+            xs, flag, relres, nit, dt, _, itc = solve(mode, 1e-6, restart=rs)
             err = (torch.linalg.norm(xs - xref) / nref).item()
-            res[name] = (xs, flag, relres, nit, dt, err, itc)
-            log('%s solve %.3f s flag %d relres %.3g iters %d error vs 1e-10 FP64 %.2e' % (name, dt, flag, relres,
-                                                                                          nit, err))
+            res[name] = (xs, flag, relres, nit, dt, err, itc, rs)
+            log('%s solve %.3f s flag %d relres %.3g products %d cycles %d error vs 1e-10 FP64 %.2e'
+                % (name, dt, flag, relres, nit, itc[1], err))
         x64s = res['fp64'][0]
         # per-solve algorithmic bytes: products and Gram-Schmidt of every
         # Arnoldi step (cycles of `restart` steps, the last one shorter), plus
@@ -376,10 +407,13 @@ def main():
         nS = nidx // 3
         bmv64, bmv32 = 16 * (8 * nS + 48 * L * M * N), 8 * (8 * nS + 48 * L * M * N)
         per_solve = {}
-        for name, mode, s_ in (('fp64', 0, 16), ('fp32_refined', 2, 8)):
-            nit, cycles = res[name][3], res[name][6]
-            lens = [50] * (cycles[1] - 1) + [nit - 50 * (cycles[1] - 1)]
-            by = solve_bytes(lens, nidx, bmv32 if mode else bmv64, s_) + (cycles[1] + 1 if mode else 1) * bmv64
+        for name, mode, s_ in (('fp64', 0, 16), ('fp32_refined', 3, 8), ('fp32_refined_cgs', 2, 8)):
+            nit, cycles, rs = res[name][3], res[name][6], res[name][7]
+            lens = [rs] * (cycles[1] - 1) + [nit - rs * (cycles[1] - 1)]
+            if mode == 3:
+                by = block_solve_bytes(lens, nidx, bmv32, s_) + cycles[1] * bmv64
+            else:
+                by = solve_bytes(lens, nidx, bmv32 if mode else bmv64, s_) + (cycles[1] if mode else 1) * bmv64
             per_solve[name] = {'algorithmic_bytes': by, 'cycle_lengths': lens,
                                'achieved_GBps': by / res[name][4] / 1e9,
                                'frac_of_hbm_peak': by / res[name][4] / 1e9 / HBM_PEAK_GBS}
@@ -387,6 +421,12 @@ def main():
             'time_to_1e-6_fp64_s': res['fp64'][4],
             'time_to_1e-6_fp32_refined_s': res['fp32_refined'][4],
             'speedup_fp32_refined': res['fp64'][4] / res['fp32_refined'][4],
+            'fp32_refined_method': 'GMRES mode 3: complex64 products in block-orthogonalised cycles of up to %d '
+                                   '(blocks of 8: local Gram-Schmidt per product, block Gram-Schmidt + CholeskyQR2 '
+                                   'per block), FP64 iterative refinement on the true residual' % refined_restart,
+            'products_fp32_refined': res['fp32_refined'][3], 'cycles_fp32_refined': res['fp32_refined'][6][1],
+            'time_to_1e-6_fp32_refined_cgs_s': res['fp32_refined_cgs'][4],
+            'gmres_iters_fp32_refined_cgs': res['fp32_refined_cgs'][3],
             'time_to_1e-6_fp32_refined_fp64basis_s': res['fp32_refined_fp64basis'][4],
             'fp32_refined_rel_error_vs_fp64_1e-10': res['fp32_refined'][5],
             'fp64_rel_error_vs_fp64_1e-10': res['fp64'][5],
