@@ -13,7 +13,9 @@
 //   [C; G0] = [Q P]^H P                        one pass over Q and P (matrix cores)
 //   R1 = chol(G0 - C^H C)                      on the device (block Gram-Schmidt with
 //   P  = (P - Q C) R1^{-1}                     the Pythagorean Gram, one pass)
-//   R2 = chol(P^H P),  P = P R2^{-1}           second CholeskyQR pass
+//   R2 = chol(P^H P),  P = P R2^{-1}           second CholeskyQR pass, skipped on the
+//                                              device when every column kept 1e-2 of its
+//                                              squared norm in the first (then R2 = I)
 //
 // so the whole basis is read twice per block instead of twice per product,
 // while every product still takes a unit vector orthogonal to its own block
@@ -83,7 +85,9 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 constexpr int STP = 64;
 
 __global__ __launch_bounds__(ST) void k_sdot(const c2<float>* __restrict__ V, int P, const c2<float>* __restrict__ W,
-                                             int Qn, int64_t ldq, int64_t n, c2<double>* __restrict__ part, int G) {
+                                             int Qn, int64_t ldq, int64_t n, c2<double>* __restrict__ part, int G,
+                                             const int* __restrict__ skip) {
+    if (skip && *skip) return;
     static_assert(SM == 8, "the packed W operand holds 8 complex columns");
     __shared__ __attribute__((aligned(32))) c2<float> Vs[SKR][STP + 1];
     __shared__ c2<float> Ws[SKR][SM + 1];
@@ -171,7 +175,8 @@ __global__ __launch_bounds__(ST) void k_sdot(const c2<float>* __restrict__ V, in
 // C[p * SM + q] = sum_rb part[(p * SM + q) G + rb] (q < Qn; columns past Qn
 // are set to zero): one wave per output, fixed order
 __global__ __launch_bounds__(ST) void k_sdot_reduce(const c2<double>* __restrict__ part, int P, int Qn, int G,
-                                                    c2<double>* __restrict__ C) {
+                                                    c2<double>* __restrict__ C, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int o = blockIdx.x * (ST / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (o >= P * SM) return;
@@ -204,9 +209,25 @@ __device__ __forceinline__ double2 zcmul(double2 a, double2 b) {      // conj(a)
     return double2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
 }
 
+// squared-norm fraction every column of a block must keep after its
+// projection for the second CholeskyQR pass to be skipped
+constexpr double kWell = 1e-2;
+
 __global__ __launch_bounds__(ST) void k_schol(const c2<double>* __restrict__ C, int k, int sb, int proj, double thresh,
                                               c2<double>* __restrict__ R, c2<double>* __restrict__ Rinv,
-                                              int* __restrict__ nok) {
+                                              int* __restrict__ nok, const int* __restrict__ skip,
+                                              int* __restrict__ well) {
+    if (skip && *skip) {
+        // the second factor of a block whose first one kept its columns well
+        // apart from the basis: the identity
+        if (threadIdx.x < SM * SM) {
+            const c2<double> v = {threadIdx.x % SM == threadIdx.x / SM ? 1.0 : 0.0, 0.0};
+            R[threadIdx.x] = v;
+            Rinv[threadIdx.x] = v;
+        }
+        if (threadIdx.x == 0) nok[0] = sb;
+        return;
+    }
     static_assert(ST == 4 * SM * SM, "four threads per Gram entry");
     __shared__ double2 Gs[SM][SM];
     __shared__ double2 Rs[SM][SM];
@@ -284,6 +305,15 @@ __global__ __launch_bounds__(ST) void k_schol(const c2<double>* __restrict__ C, 
             }
         }
         nok[0] = ok;
+        if (well) {
+            // every pivot kept at least kWell of its column's squared norm:
+            // the Pythagorean Gram lost at most a factor 1/kWell to
+            // cancellation and one CholeskyQR pass leaves the block orthonormal
+            // to the basis' own accuracy
+            double rmin = 1.0;
+            for (int j = 0; j < ok; ++j) rmin = fmin(rmin, Rs[j][j].x * Rs[j][j].x / d0[j]);
+            *well = (ok == sb && rmin >= kWell) ? 1 : 0;
+        }
     }
     __syncthreads();
     if (t < SM * SM) {
@@ -299,7 +329,8 @@ __global__ __launch_bounds__(ST) void k_schol(const c2<double>* __restrict__ C, 
 // complex64 data, complex64 result in place.
 __global__ __launch_bounds__(ST) void k_supdate(c2<float>* __restrict__ Q, int64_t ldq, int k, int sb,
                                                 const c2<double>* __restrict__ C, const c2<double>* __restrict__ Rinv,
-                                                int proj, int64_t n) {
+                                                int proj, int64_t n, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x;
     if (e >= n) return;
     double ax[SM], ay[SM];
@@ -554,7 +585,7 @@ struct SWs {
     c2<double>* G2 = nullptr;    // SM x SM
     c2<double>* R = nullptr;     // 4 SM x SM: R1, R1inv, R2, R2inv
     c2<double>* y = nullptr;     // kMaxBasis solution coefficients
-    int* nok = nullptr;          // 2
+    int* nok = nullptr;          // 3: accepted columns of the two factors, second-pass skip flag
     double* npart = nullptr;
     double* nout = nullptr;
     c2<float>* wt = nullptr;     // n: a product's output before its local orthogonalisation
@@ -601,15 +632,15 @@ template <typename TV> int s_norm(SCtx& c, const c2<TV>* v, double* out) {
 }
 
 // C (P x SM, row-major, columns past Qn zero) = Q[v0 : v0 + P)^H Q[w0 : w0 + Qn)
-int s_dot(SCtx& c, int v0, int P, int w0, int Qn, c2<double>* C) {
+int s_dot(SCtx& c, int v0, int P, int w0, int Qn, c2<double>* C, const int* skip = nullptr) {
     const int ptiles = (P + STP - 1) / STP;
     // about 1024 workgroups over the column tiles
     const int G = std::max(64, std::min(SGD, SGD / ptiles));
     const dim3 grid(ptiles, G);
     hipLaunchKernelGGL(k_sdot, grid, dim3(ST), 0, c.s, c.w.Q + (int64_t)v0 * c.ldq, P, c.w.Q + (int64_t)w0 * c.ldq,
-                       Qn, c.ldq, c.n, c.w.part, G);
+                       Qn, c.ldq, c.n, c.w.part, G, skip);
     hipLaunchKernelGGL(k_sdot_reduce, dim3((P * SM + ST / 64 - 1) / (ST / 64)), dim3(ST), 0, c.s, c.w.part, P, Qn,
-                       G, C);
+                       G, C, skip);
     MARIE_HIP(hipGetLastError());
     return OK;
 }
@@ -721,9 +752,12 @@ int s_products(SCtx& c, int k, int sb) {
         const int st = apply(c.h, OP_AN, c.transp, PREC_F32, in, c.w.wt, 1, c.s);
         if (st) return st;
         c2<double>* hj = c.w.hloc + (size_t)j * (SL + 1);
-        hipLaunchKernelGGL(k_ldot, dim3(c.G), dim3(ST), 0, c.s, c.w.Q, c.ldq, k - 1, j + 1, c.w.wt, c.n, c.w.lpart,
-                           c.G);
-        hipLaunchKernelGGL(k_lsum, dim3(j + 2), dim3(ST), 0, c.s, c.w.lpart, c.G, j + 1, hj, c.w.w2);
+        // a quarter of the combination's blocks: each block's 19-row LDS sum
+        // then covers four times the rows
+        const int gd = std::max(1, c.G / 4);
+        hipLaunchKernelGGL(k_ldot, dim3(gd), dim3(ST), 0, c.s, c.w.Q, c.ldq, k - 1, j + 1, c.w.wt, c.n, c.w.lpart,
+                           gd);
+        hipLaunchKernelGGL(k_lsum, dim3(j + 2), dim3(ST), 0, c.s, c.w.lpart, gd, j + 1, hj, c.w.w2);
         hipLaunchKernelGGL(k_lcomb, dim3(c.G), dim3(ST), 0, c.s, out, c.w.wt, c.w.Q, c.ldq, k - 1, j + 1, hj,
                            c.w.w2, c.n);
         MARIE_HIP(hipGetLastError());
@@ -742,15 +776,22 @@ int s_passes(SCtx& c, int k, int sb, hipEvent_t ev) {
     // [C; G0] = [Q P]^H P, R1 = chol(G0 - C^H C), P = (P - Q C) R1^{-1}
     int st = s_dot(c, 0, k + sb, k, sb, c.w.C);
     if (st) return st;
-    hipLaunchKernelGGL(k_schol, dim3(1), dim3(ST), 0, c.s, c.w.C, k, sb, 1, 1e-10, R1, R1i, c.w.nok);
+    int* well = c.w.nok + 2;
+    hipLaunchKernelGGL(k_schol, dim3(1), dim3(ST), 0, c.s, c.w.C, k, sb, 1, 1e-10, R1, R1i, c.w.nok,
+                       (const int*)nullptr, well);
     const unsigned gu = (unsigned)((c.n + ST - 1) / ST);
-    hipLaunchKernelGGL(k_supdate, dim3(gu), dim3(ST), 0, c.s, c.w.Q, c.ldq, k, sb, c.w.C, R1i, 1, c.n);
+    hipLaunchKernelGGL(k_supdate, dim3(gu), dim3(ST), 0, c.s, c.w.Q, c.ldq, k, sb, c.w.C, R1i, 1, c.n,
+                       (const int*)nullptr);
     MARIE_HIP(hipGetLastError());
-    // R2 = chol(P^H P), P = P R2^{-1} (the Gram's rows start at row 0 of G2)
-    st = s_dot(c, k, sb, k, sb, c.w.G2);
+    // R2 = chol(P^H P), P = P R2^{-1} (the Gram's rows start at row 0 of G2);
+    // its kernels return at once (R2 = I) when the first factor kept the
+    // block well apart from the basis
+    st = s_dot(c, k, sb, k, sb, c.w.G2, well);
     if (st) return st;
-    hipLaunchKernelGGL(k_schol, dim3(1), dim3(ST), 0, c.s, c.w.G2, 0, sb, 0, 1e-10, R2, R2i, c.w.nok + 1);
-    hipLaunchKernelGGL(k_supdate, dim3(gu), dim3(ST), 0, c.s, c.w.Q, c.ldq, k, sb, c.w.C, R2i, 0, c.n);
+    hipLaunchKernelGGL(k_schol, dim3(1), dim3(ST), 0, c.s, c.w.G2, 0, sb, 0, 1e-10, R2, R2i, c.w.nok + 1,
+                       (const int*)well, (int*)nullptr);
+    hipLaunchKernelGGL(k_supdate, dim3(gu), dim3(ST), 0, c.s, c.w.Q, c.ldq, k, sb, c.w.C, R2i, 0, c.n,
+                       (const int*)well);
     MARIE_HIP(hipGetLastError());
     unsigned char* hp = reinterpret_cast<unsigned char*>(c.w.hp);
     const size_t cbytes = (size_t)k * SM * sizeof(c2<double>);
@@ -927,7 +968,7 @@ int sstep_refine(Op* h, hipStream_t s, int transp, const c2<double>* b, c2<doubl
     if (!st) st = A((void**)&c.w.G2, (size_t)SM * SM * sizeof(c2<double>));
     if (!st) st = A((void**)&c.w.R, (size_t)4 * SM * SM * sizeof(c2<double>));
     if (!st) st = A((void**)&c.w.y, (size_t)kMaxBasis * sizeof(c2<double>));
-    if (!st) st = A((void**)&c.w.nok, 2 * sizeof(int));
+    if (!st) st = A((void**)&c.w.nok, 3 * sizeof(int));
     if (!st) st = A((void**)&c.w.npart, SG * sizeof(double));
     if (!st) st = A((void**)&c.w.nout, sizeof(double));
     if (!st) st = A((void**)&c.w.wt, (size_t)c.ldq * sizeof(c2<float>));
