@@ -330,7 +330,11 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
     when `precond` is 'L' or 'R'; iter is then in half steps / 2 and resvec absolute.
     Forms I and II (JVIE_Solver.m:176-300). precision='fp32-refined-c64'
     stores the refinement cycles' Krylov basis in complex64 as well (every
-    dot product, norm and Givens rotation still in FP64).
+    dot product, norm and Givens rotation still in FP64);
+    precision='fp32-refined-blocks' grows that complex64 basis by locally
+    orthogonalised blocks of 8 products with one block Gram-Schmidt pass per
+    block (GMRES mode 3; inner_it is the cycle length, up to 503; 128 is the
+    benchmark's choice).
 
     logfile: a path the run summary is appended to (JVIE_Solver.m:115-122,
     :159-166 and the solver line of :335, :355, :375, ...); None or a path
@@ -345,8 +349,9 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
     use_pc = precond in ('L', 'R')       # JVIE_Solver.m:92-98
     if use_pc and precision != 'fp64':
         raise ValueError('the preconditioned solvers run in FP64')
-    if precision not in ('fp64', 'fp32-refined', 'fp32-refined-c64'):
-        raise ValueError("precision must be 'fp64', 'fp32-refined' or 'fp32-refined-c64'")
+    modes = {'fp64': 0, 'fp32-refined': 1, 'fp32-refined-c64': 2, 'fp32-refined-blocks': 3}
+    if precision not in modes:
+        raise ValueError("precision must be 'fp64', 'fp32-refined', 'fp32-refined-c64' or 'fp32-refined-blocks'")
     Mr, Mc, Gram, idx, Vrhs = jvie_system(Einc, e_r, r, freq, form)
     L, M, N = e_r.shape
     prec = lib.PREC_F64 | (lib.PREC_F32 if precision != 'fp64' else 0)
@@ -377,7 +382,7 @@ def JVIE_Solver(Einc, e_r, r, freq, fN, tol=1e-3, form=2, solver='B', precond=0,
             x, flag, relres, it, resvec = op.gmres_precond(Vrhs, None, inner_it, tol, outer_it, side=side)
         else:
             x, flag, relres, it, resvec = op.gmres(Vrhs, None, inner_it, tol, outer_it,
-                                                   mode={'fp64': 0, 'fp32-refined': 1, 'fp32-refined-c64': 2}[precision],
+                                                   mode=modes[precision],
                                                    inner_tol=inner_tol)
     else:
         if precision != 'fp64':
