@@ -480,6 +480,24 @@ def main():
                         'flags_max': int(fl.max()), 'max_rel_diff_vs_single_solves': diff}
             log('block gmres mode %d: %.3f s for %d excitations (sequential FP64 %.3f s), cycles %d, max relres %.2e'
                 % (mode, tb, s_blk, t_seq, it[1], rr.max()))
+        # the same 16 columns one at a time with the refined solver (GMRES mode
+        # 3, cycles of 128): fewer bytes per product than any block variant
+        X = torch.zeros_like(Bd)
+        op.block_gmres_device(Bd[:1].data_ptr(), X[:1].data_ptr(), 1, restart=128, tol=1e-6, maxit=200, mode=3,
+                              inner_tol=1e-7, columns=True)       # warm: workspace allocation
+        X = torch.zeros_like(Bd)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fl, rr, it, rv = op.block_gmres_device(Bd.data_ptr(), X.data_ptr(), s_blk, restart=128, tol=1e-6, maxit=200,
+                                               mode=3, inner_tol=1e-7, columns=True)
+        torch.cuda.synchronize()
+        tc = time.perf_counter() - t
+        diff = max((torch.linalg.norm(X[q] - Xseq[q]) / torch.linalg.norm(Xseq[q])).item() for q in range(s_blk))
+        blk['columns_fp32_refined_mode3'] = {'s': tc, 'cycles_all_columns': it[1], 'max_relres': float(rr.max()),
+                                             'flags_max': int(fl.max()), 'max_rel_diff_vs_single_solves': diff,
+                                             'speedup_vs_sequential_fp64': t_seq / tc}
+        log('16 columns one at a time, refined mode 3: %.3f s (%.2fx the sequential FP64 solves), max relres %.2e'
+            % (tc, t_seq / tc, rr.max()))
         out['batching_16_excitations'] = blk
 
     if not args.no_solve and not args.no_batch and n1 == 128:
@@ -506,11 +524,12 @@ def main():
                 fac = np.tile((Gb * 1j * emb['omega'] * emb['eo'] * (e_rb - 1) / e_rb).ravel(order='F'), 3)[idxb - 1]
                 cols = [fac * Ep.ravel(order='F')[idxb - 1] for Ep in Es]
                 assert np.allclose(cols[0], v0, rtol=1e-12, atol=0)
-                opb = lib.Operator(nb, nb, nb)
+                opb = lib.Operator(nb, nb, nb, precisions=lib.PREC_F64 | lib.PREC_F32)
                 opb.assemble_N(dxb, emb['ko'])
                 opb.set_material(Mrb, Mcb, Gb)
                 opb.set_index(idxb, one_based=True)
                 opb.autotune(lib.PREC_F64, iters=3)
+                opb.autotune(lib.PREC_F32, iters=3)
                 Bb = torch.from_numpy(np.ascontiguousarray(np.stack(cols))).to(dev)
                 probs.append((nb, fb, opb, Bb))
         torch.cuda.synchronize()
@@ -534,12 +553,47 @@ def main():
                                               [x.data_ptr() for x in Xs], [16] * len(probs), **kw)
         torch.cuda.synchronize()
         t_bat = time.perf_counter() - t
+        # the reference's loop: every port of every problem one FP64 GMRES(50) after another
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for nb, fb, opb, Bb in probs:
+            Xb = torch.zeros_like(Bb)
+            for q in range(Bb.shape[0]):
+                opb.gmres_device(Bb[q].data_ptr(), Xb[q].data_ptr(), restart=50, tol=1e-6, maxit=200)
+        torch.cuda.synchronize()
+        t_ports = time.perf_counter() - t
+        # the columns one at a time with the refined solver (mode 3), the
+        # problems one after the other and then all at once
+        kwc = dict(restart=128, tol=1e-6, maxit=200, mode=3, inner_tol=1e-7, columns=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for nb, fb, opb, Bb in probs:
+            Xb = torch.zeros_like(Bb)
+            opb.block_gmres_device(Bb.data_ptr(), Xb.data_ptr(), 16, **kwc)
+        torch.cuda.synchronize()
+        t_seq3 = time.perf_counter() - t
+        Xs3 = [torch.zeros_like(p[3]) for p in probs]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fl3, rr3, it3 = lib.block_gmres_batch([p[2] for p in probs], [p[3].data_ptr() for p in probs],
+                                              [x.data_ptr() for x in Xs3], [16] * len(probs), **kwc)
+        torch.cuda.synchronize()
+        t_bat3 = time.perf_counter() - t
         out['batching_bodies_frequencies'] = {
             'problems': seq, 'ports_per_problem': 16, 'setup_s': t_setup, 'sequential_block_s': t_seq,
             'concurrent_batch_s': t_bat, 'speedup': t_seq / t_bat,
-            'batch_max_relres': float(max(r.max() for r in rrs)), 'batch_flags_max': int(max(f.max() for f in fls))}
-        log('batch of %d problems x 16 ports: one after the other %.3f s, concurrent %.3f s (max relres %.2e; '
-            'setup %.1f s)' % (len(probs), t_seq, t_bat, max(r.max() for r in rrs), t_setup))
+            'batch_max_relres': float(max(r.max() for r in rrs)), 'batch_flags_max': int(max(f.max() for f in fls)),
+            'per_port_fp64_sequential_s': t_ports,
+            'columns_fp32_refined_mode3': {'sequential_s': t_seq3, 'concurrent_batch_s': t_bat3,
+                                           'speedup_concurrent_vs_sequential': t_seq3 / t_bat3,
+                                           'speedup_vs_per_port_fp64': t_ports / t_bat3,
+                                           'max_relres': float(max(r.max() for r in rr3)),
+                                           'flags_max': int(max(f.max() for f in fl3))},
+            'speedup_block_batch_vs_per_port_fp64': t_ports / t_bat}
+        log('batch of %d problems x 16 ports: block GMRES one after the other %.3f s, concurrent %.3f s (max relres '
+            '%.2e; setup %.1f s); per-port FP64 GMRES %.3f s; refined mode-3 columns one after the other %.3f s, '
+            'concurrent %.3f s (max relres %.2e)' % (len(probs), t_seq, t_bat, max(r.max() for r in rrs), t_setup,
+                                                     t_ports, t_seq3, t_bat3, max(r.max() for r in rr3)))
         for p in probs:
             p[2].close()
         del probs, Xs

@@ -302,9 +302,15 @@ int marie_vie_material_gradient_device(marie_vie_op* h, const void* lambda_dev, 
  * {block steps of the
  * last cycle, cycles}; resvec gets the worst column's relative residual per
  * step and, after each cycle, its true value. Limits: nrhs <= 64,
- * (restart+1)*nrhs <= 2048 basis vectors. */
+ * (restart+1)*nrhs <= 2048 basis vectors.
+ * MARIE_BLOCK_COLUMNS solves the columns one at a time instead, each as
+ * marie_vie_gmres_device in `mode` (0-3; 3: block-orthogonalised complex64
+ * cycles, the fastest per column at 128^3) with `restart` its cycle length
+ * (then no nrhs limit); iter = {largest last-cycle count, cycles of all
+ * columns}, resvec each column's final relres. */
 #define MARIE_BLOCK_GRAPH 1
 #define MARIE_BLOCK_CGS2 2
+#define MARIE_BLOCK_COLUMNS 4
 int marie_vie_block_gmres_device(marie_vie_op* h, const void* B_dev, void* X_dev, int nrhs, int restart,
                                  double tol, int maxit, int mode, double inner_tol, int options,
                                  int* flag, double* relres, int* iter, double* resvec, int resvec_cap,

@@ -763,6 +763,29 @@ extern "C" int marie_vie_block_gmres_device(marie_vie_op* hh, const void* B, voi
     Op* h = reinterpret_cast<Op*>(hh);
     if (!h) { set_error("null handle"); return MARIE_ERR_ARG; }
     MARIE_HIP(hipSetDevice(h->device));
+    if (options & MARIE_BLOCK_COLUMNS) {
+        // the columns one at a time, each a marie_vie_gmres_device solve in
+        // `mode` (restart = its cycle length); iter = {the largest last-cycle
+        // count, the cycles of all columns}; resvec: each column's final relres
+        if (!B || !X || !flag || !relres || !iter || nrhs < 1) {
+            set_error("invalid block gmres arguments");
+            return MARIE_ERR_ARG;
+        }
+        const int64_t n = h->nidx;
+        iter[0] = iter[1] = 0;
+        for (int q = 0; q < nrhs; ++q) {
+            int it[2] = {0, 0};
+            const int st = marie_vie_gmres_device(hh, (const c2<double>*)B + q * n, (c2<double>*)X + q * n, restart,
+                                                  tol, maxit, mode, inner_tol, flag + q, relres + q, it, nullptr, 0,
+                                                  nullptr);
+            if (st) return st;
+            iter[0] = std::max(iter[0], it[0]);
+            iter[1] += it[1];
+            if (resvec && q < resvec_cap) resvec[q] = relres[q];
+        }
+        if (nres) *nres = std::min(nrhs, resvec ? resvec_cap : 0);
+        return MARIE_OK;
+    }
     if (!B || !X || !flag || !relres || !iter || nrhs < 1 || nrhs > 64 || restart < 1 ||
         (int64_t)(restart + 1) * nrhs > 2048 || maxit < 1 || !(tol > 0) || (mode != 0 && mode != 1)) {
         set_error("invalid block gmres arguments (nrhs 1..64, (restart+1)*nrhs <= 2048, mode 0 or 1)");

@@ -180,3 +180,37 @@ def test_block_batch_of_uneven_problems():
             assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[:, q])
     with pytest.raises(lib.MarieError):
         lib.block_gmres_batch([ops[0], ops[0]], [Bs[0].data_ptr()] * 2, [Xs[0].data_ptr()] * 2, [nrhs[0]] * 2)
+
+
+@pytest.mark.parametrize("mode,restart", [(0, 50), (3, 40)])
+def test_block_columns_option(problem, reference, mode, restart):
+    """MARIE_BLOCK_COLUMNS: each column one GMRES solve in `mode` (0: pgmres,
+    3: block-orthogonalised complex64 cycles with FP64 refinement), on one
+    handle and through the batch entry over two handles; every column's true
+    residual by the oracle's product <= tol and its solution within the tolerance
+    of the oracle's per-column pgmres."""
+    import torch
+    from marie_amd import lib
+    print("case block columns mode", mode, flush=True)
+    fN, Mr, Mc, Gram, idx, B = problem
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx)
+    dev = torch.device('cuda:0')
+    ops = [_op(problem, lib.PREC_F64 | lib.PREC_F32) for _ in range(2)]
+    Bd = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev)
+    Xd = torch.zeros_like(Bd)
+    fl, rr, it, rv = ops[0].block_gmres_device(Bd.data_ptr(), Xd.data_ptr(), B.shape[1], restart=restart, tol=1e-6,
+                                               maxit=200, mode=mode, inner_tol=1e-7, columns=True)
+    X = Xd.cpu().numpy().T
+    assert np.all(fl == 0) and np.all(rr <= 1e-6) and it[1] >= B.shape[1]
+    for q in range(B.shape[1]):
+        assert _rel(A(X[:, q]), B[:, q]) <= 1e-6
+        assert _rel(X[:, q], reference[:, q]) <= 1e-4
+    Xs = [torch.zeros_like(Bd), torch.zeros_like(Bd)]
+    fls, rrs, its = lib.block_gmres_batch(ops, [Bd.data_ptr(), Bd.data_ptr()], [x.data_ptr() for x in Xs],
+                                          [B.shape[1]] * 2, restart=restart, tol=1e-6, maxit=200, mode=mode,
+                                          inner_tol=1e-7, columns=True)
+    for i in range(2):
+        assert np.all(fls[i] == 0) and np.all(rrs[i] <= 1e-6)
+        assert torch.equal(Xs[i], Xd)      # same kernels, same order, same stream semantics: identical
+    for o in ops:
+        o.close()
