@@ -80,6 +80,7 @@ inline Identity identity(const mxArray* a) {
 
 // The cached device operator of one gateway.
 struct Cache {
+    int precisions = MARIE_PREC_F64;     // MARIE_PREC_* the cached operator is created with
     marie_vie_op* op = nullptr;
     int dims[6] = {0, 0, 0, 0, 0, 0};
     Identity fN, Mr, Mc, idx;
@@ -106,7 +107,7 @@ struct Cache {
         const Identity idf = identity(fNa);
         if (op && idf == fN && std::memcmp(d, dims, sizeof d) == 0) return;
         clear();
-        fail(marie_vie_create(&op, d[0], d[1], d[2], d[3], d[4], d[5], MARIE_PREC_F64, 0));
+        fail(marie_vie_create(&op, d[0], d[1], d[2], d[3], d[4], d[5], precisions, 0));
         std::vector<double> buf;
         interleave(fNa, buf);
         fail(marie_vie_set_fN(op, buf.data()));

@@ -80,4 +80,13 @@ def test_jvie_gmres_gateway():
     # a start vector
     x2, flag2 = g(b, fN, Mr, Mc, Gram, (idx + 1).astype(float), 20, 1e-8, 20, xr, nargout=2)
     assert flag2[0, 0] == 0 and _rel(x2[:, 0], xr) <= 1e-7
+    # the refined precision (complex64 products in block-orthogonalised
+    # cycles, FP64 refinement): pgmres' flag/relres on the true residual
+    x3, flag3, relres3, it3, rv3 = g(b, fN, Mr, Mc, Gram, (idx + 1).astype(float), 40, 1e-6, 20, np.zeros(0),
+                                     'fp32-refined', nargout=5)
+    assert flag3[0, 0] == 0 and relres3[0, 0] <= 1e-6
+    assert _rel(A(x3[:, 0]), b) <= 1e-6 and abs(rv3[-1, 0] - relres3[0, 0]) <= 1e-9
+    assert _rel(x3[:, 0], xr) <= 1e-4
+    with pytest.raises(emul.MexError, match='MARIE:hip:precision'):
+        g(b, fN, Mr, Mc, Gram, (idx + 1).astype(float), 40, 1e-6, 20, np.zeros(0), 'fp16')
     g('clear', nargout=0)
