@@ -43,7 +43,8 @@ class JVIEProblem:
     freq in Hz, dx the voxel size.
     """
 
-    def __init__(self, op, body, Einc, freq, dx, tol=1e-10, restart=50, maxit=200, device='cuda:0', Hinc=None):
+    def __init__(self, op, body, Einc, freq, dx, tol=1e-10, restart=50, maxit=200, device='cuda:0', Hinc=None,
+                 mode=0, inner_tol=1e-7):
         self.op = op
         self.shape = (op.L, op.M, op.N)
         body = np.asarray(body, bool)
@@ -62,7 +63,11 @@ class JVIEProblem:
                                                                         .ravel(order='F')[self.idx]))
                                  for e in E]).to(device)
         self.vox = torch.from_numpy(self.idx % n).to(device)
+        # GMRES mode of the forward and adjoint solves (0: pgmres; 1-3: complex64
+        # products with FP64 refinement, 3 the block-orthogonalised cycles, which
+        # needs the operator created with FP32 as well)
         self.tol, self.restart, self.maxit = tol, restart, maxit
+        self.mode, self.inner_tol = mode, inner_tol
         self.device = device
         self.hinc = None
         if Hinc is not None:
@@ -100,7 +105,7 @@ class _Solve(torch.autograd.Function):
         torch.cuda.synchronize()
         for q in range(prob.nrhs):
             flag, relres, it, rv = prob.op.gmres_device(B[q].data_ptr(), X[q].data_ptr(), prob.restart, prob.tol,
-                                                        prob.maxit)
+                                                        prob.maxit, mode=prob.mode, inner_tol=prob.inner_tol)
             if flag != 0:
                 raise lib.MarieError('forward solve did not converge (relres %.3g)' % relres)
         ctx.prob = prob
@@ -122,7 +127,8 @@ class _Solve(torch.autograd.Function):
             lam = torch.zeros_like(G[q])
             torch.cuda.synchronize()
             flag, relres, it, rv = prob.op.gmres_adjoint_device(G[q].data_ptr(), lam.data_ptr(), prob.restart,
-                                                                prob.tol, prob.maxit)
+                                                                prob.tol, prob.maxit, mode=prob.mode,
+                                                                inner_tol=prob.inner_tol)
             if flag != 0:
                 raise lib.MarieError('adjoint solve did not converge (relres %.3g)' % relres)
             einc = _dense(prob.einc[q])
