@@ -226,6 +226,32 @@ def test_gmres_sstep_restarts_and_adjoint(restart, adjoint):
     assert it3[0] <= restart
 
 
+def test_gmres_sstep_odd_dof_count():
+    """Mode 3 on a body with an odd number of DOFs (the two-element loads of
+    the local Gram-Schmidt and the 16-byte column boundaries then leave one
+    trailing element per column): true residual <= tol by the oracle's
+    product, solution within the FP64 solve's error of the oracle's pgmres."""
+    import torch
+    from marie_amd import lib
+    n, dx, f = 11, 6.25e-3, 298.2e6
+    print("case gmres s-step odd DOF count", flush=True)
+    r, e_r, Einc, em = _sphere_problem(n, dx, f)      # 485 voxels: 1455 DOFs
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    Mr, Mc, Gram, idx, b = fields.jvie_setup(Einc, e_r, dx, f, form=2)
+    assert idx.size % 2 == 1
+    op = lib.Operator(n, n, n, precisions=lib.PREC_F64 | lib.PREC_F32)
+    op.set_fN(fN)
+    op.set_material(Mr, Mc, Gram)
+    op.set_index(idx)
+    A = lambda v: mvp.mv_AN(v, fN, Mr, Mc, Gram, 'notransp', idx)
+    xref, fl, rel, _, _ = krylov.pgmres(A, b, 50, 1e-11, 200)
+    x0, fl0, rel0, _, _ = op.gmres(b, restart=50, tol=1e-6, maxit=200, mode=0)
+    for restart in (9, 40):
+        x3, fl3, rel3, it3, rv3 = op.gmres(b, restart=restart, tol=1e-6, maxit=200, mode=3, inner_tol=1e-7)
+        assert fl3 == 0 and rel3 <= 1e-6 and _rel(A(x3), b) <= 1e-6, restart
+        assert _rel(x3, xref) <= max(5 * _rel(x0, xref), 1e-6), restart
+
+
 def test_default_circulant_for_large_prime_factor():
     """A side whose 2L has a prime factor above 13 (L = 17: 34 = 2 * 17) gets
     the next 7-smooth circulant side (36) by default; the product equals the
