@@ -8,7 +8,7 @@
 // to SM = 8 columns:
 //
 //   b_0 = q_last; for j < S: w = A b_j, b_{j+1} = w minus its projection on
-//        b_0..b_j, normalised                  local Arnoldi (the block only)
+//        b_{j-1}, b_j, normalised              banded local Arnoldi (the block only)
 //   P = [b_1 .. b_S]
 //   [C; G0] = [Q P]^H P                        one pass over Q and P (matrix cores)
 //   R1 = chol(G0 - C^H C)                      on the device (block Gram-Schmidt with
@@ -18,9 +18,10 @@
 //                                              squared norm in the first (then R2 = I)
 //
 // so the whole basis is read twice per block instead of twice per product,
-// while every product still takes a unit vector orthogonal to its own block
-// (a well-conditioned block, unlike a power basis). With the local relation
-// A b_j = [b_0 .. b_{j+1}] h_j and P = Q C + Q_new R2 R1, the Arnoldi
+// while every product still takes a unit vector orthogonal to the two
+// columns before it (a well-conditioned block, unlike a power basis). With
+// the local relation A b_j = [b_0 .. b_{j+1}] h_j (h_j nonzero in rows j-1 ..
+// j+1 only) and P = Q C + Q_new R2 R1, the Arnoldi
 // relation A Q_k = Q_{k+1} H follows column by column (host, FP64); the
 // cycle's least squares min || beta e_1 - H y || is updated with Givens
 // rotations on the host. A CholeskyQR pivot below 1e-10 of the column's
@@ -52,6 +53,7 @@ constexpr int SM = 8;            // largest block of products (columns of P)
 constexpr int SKR = 64;          // rows per staged chunk of the dot kernel
 constexpr int SGD = 1024;        // most row blocks of the dot kernel
 constexpr int kMaxBasis = 512;   // most basis columns a combination reads
+constexpr int kSchRows = 64;     // basis rows of C per LDS stage of the Cholesky kernel
 
 __device__ __forceinline__ double2 ld2(const c2<float>* p) {
     const c2<float> v = *p;
@@ -71,13 +73,13 @@ __device__ __forceinline__ c2<float> ldf_nt(const c2<float>* p) {
 }
 
 // part[(p * SM + q) * G + rb] = sum over the row chunks rb, rb + G, ... of
-// conj(V_p) W_q, p < P, q < Qn (<= SM = 8). V, W complex64 columns (stride
+// conj(V_p) W_q, p < P, q < Qn (<= SM). V, W complex64 columns (stride
 // ldq). A workgroup (4 waves) takes 64 columns of V and the W block; each
 // chunk of SKR rows is staged in LDS, then the 4-row k-steps run on the
-// matrix cores (v_mfma_f64_16x16x4f64) with the W block packed as the 16
-// real columns [Re W | Im W]: D1 = Re(V)^T [Re W | Im W], D2 = Im(V)^T [Re W |
-// Im W], so conj(V)^T W = (D1[:, q] + D2[:, 8+q]) + j (D1[:, 8+q] - D2[:, q])
-// takes two real products per k-step. When the tile holds fewer than 64
+// matrix cores (v_mfma_f64_16x16x4f64) with each tile of 8 W columns packed
+// as the 16 real columns [Re W | Im W]: D1 = Re(V)^T [Re W | Im W], D2 =
+// Im(V)^T [Re W | Im W], so conj(V)^T W = (D1[:, q] + D2[:, 8+q]) + j (D1[:,
+// 8+q] - D2[:, q]) takes two real products per k-step and tile. When the tile holds fewer than 64
 // columns, the waves of a 16-column subtile split its k-steps and add their
 // sums in LDS. Operand layout: lane i + 16 k holds A[i][k] and B[k][i];
 // D[r][c] is element r / 4 of lane c + 16 (r % 4).
@@ -88,7 +90,8 @@ __global__ __launch_bounds__(ST) void k_sdot(const c2<float>* __restrict__ V, in
                                              int Qn, int64_t ldq, int64_t n, c2<double>* __restrict__ part, int G,
                                              const int* __restrict__ skip) {
     if (skip && *skip) return;
-    static_assert(SM == 8, "the packed W operand holds 8 complex columns");
+    static_assert(SM % 8 == 0, "the packed W operand holds 8 complex columns per tile");
+    constexpr int NWT = SM / 8;                      // W tiles of 8 complex columns
     __shared__ __attribute__((aligned(32))) c2<float> Vs[SKR][STP + 1];
     __shared__ c2<float> Ws[SKR][SM + 1];
     const int p0 = blockIdx.x * STP;
@@ -121,7 +124,12 @@ __global__ __launch_bounds__(ST) void k_sdot(const c2<float>* __restrict__ V, in
             wreg[k] = (qc < Qn && e < n) ? W[(int64_t)qc * ldq + e] : c2<float>{0.f, 0.f};
         }
     };
-    dx4 d1 = {0, 0, 0, 0}, d2 = {0, 0, 0, 0};
+    dx4 d1[NWT], d2[NWT];
+#pragma unroll
+    for (int t = 0; t < NWT; ++t) {
+        d1[t] = dx4{0, 0, 0, 0};
+        d2[t] = dx4{0, 0, 0, 0};
+    }
     if (rb < nchunk) fetch(rb);
     for (int64_t ch = rb; ch < nchunk; ch += G) {
         __syncthreads();
@@ -140,36 +148,47 @@ __global__ __launch_bounds__(ST) void k_sdot(const c2<float>* __restrict__ V, in
         if (active) {
             for (int ks = kpart; ks < SKR / 4; ks += wps) {
                 const c2<float> a = Vs[4 * ks + lk][16 * sub + li];
-                const c2<float> w = Ws[4 * ks + lk][li & 7];
-                const double bv = li < 8 ? (double)w.x : (double)w.y;
-                d1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a.x, bv, d1, 0, 0, 0);
-                d2 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a.y, bv, d2, 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < NWT; ++t) {
+                    const c2<float> w = Ws[4 * ks + lk][8 * t + (li & 7)];
+                    const double bv = li < 8 ? (double)w.x : (double)w.y;
+                    d1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a.x, bv, d1[t], 0, 0, 0);
+                    d2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a.y, bv, d2[t], 0, 0, 0);
+                }
             }
         }
     }
     if (wps > 1) {
         // the k-step sums of a subtile's waves, added in a fixed order (the
         // staging area is free after the barrier)
+        static_assert(4 * 64 * 2 * NWT * sizeof(dx4) <= sizeof(Vs), "reduction area");
         __syncthreads();
-        dx4* red = reinterpret_cast<dx4*>(&Vs[0][0]);     // 4 waves x 64 lanes x 2 dx4 = 16 KB
-        red[(wave * 64 + lane) * 2] = d1;
-        red[(wave * 64 + lane) * 2 + 1] = d2;
+        dx4* red = reinterpret_cast<dx4*>(&Vs[0][0]);     // 4 waves x 64 lanes x 2 NWT dx4
+#pragma unroll
+        for (int t = 0; t < NWT; ++t) {
+            red[((wave * 64 + lane) * NWT + t) * 2] = d1[t];
+            red[((wave * 64 + lane) * NWT + t) * 2 + 1] = d2[t];
+        }
         __syncthreads();
         if (kpart != 0) return;
-        for (int w2 = 1; w2 < wps; ++w2) {
-            d1 += red[((wave + w2) * 64 + lane) * 2];
-            d2 += red[((wave + w2) * 64 + lane) * 2 + 1];
-        }
+        for (int w2 = 1; w2 < wps; ++w2)
+#pragma unroll
+            for (int t = 0; t < NWT; ++t) {
+                d1[t] += red[(((wave + w2) * 64 + lane) * NWT + t) * 2];
+                d2[t] += red[(((wave + w2) * 64 + lane) * NWT + t) * 2 + 1];
+            }
     }
     if (!active) return;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        // lanes li < 8 combine with lane li + 8 of the same k group
-        const double d1h = __shfl_down(d1[e], 8, 16), d2h = __shfl_down(d2[e], 8, 16);
-        const int pr = 16 * sub + 4 * e + lk;
-        if (li < 8 && pr < pv && li < Qn)
-            part[((int64_t)(p0 + pr) * SM + li) * G + rb] = c2<double>{d1[e] + d2h, d1h - d2[e]};
-    }
+    for (int t = 0; t < NWT; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            // lanes li < 8 combine with lane li + 8 of the same k group
+            const double d1h = __shfl_down(d1[t][e], 8, 16), d2h = __shfl_down(d2[t][e], 8, 16);
+            const int pr = 16 * sub + 4 * e + lk, q = 8 * t + li;
+            if (li < 8 && pr < pv && q < Qn)
+                part[((int64_t)(p0 + pr) * SM + q) * G + rb] = c2<double>{d1[t][e] + d2h, d1h - d2[t][e]};
+        }
 }
 
 // C[p * SM + q] = sum_rb part[(p * SM + q) G + rb] (q < Qn; columns past Qn
@@ -217,109 +236,121 @@ __global__ __launch_bounds__(ST) void k_schol(const c2<double>* __restrict__ C, 
                                               c2<double>* __restrict__ R, c2<double>* __restrict__ Rinv,
                                               int* __restrict__ nok, const int* __restrict__ skip,
                                               int* __restrict__ well) {
+    static_assert(SM * SM <= ST, "one thread per entry of the block's Gram");
+    const int t = threadIdx.x;
     if (skip && *skip) {
         // the second factor of a block whose first one kept its columns well
         // apart from the basis: the identity
-        if (threadIdx.x < SM * SM) {
-            const c2<double> v = {threadIdx.x % SM == threadIdx.x / SM ? 1.0 : 0.0, 0.0};
-            R[threadIdx.x] = v;
-            Rinv[threadIdx.x] = v;
+        if (t < SM * SM) {
+            const c2<double> v = {t % SM == t / SM ? 1.0 : 0.0, 0.0};
+            R[t] = v;
+            Rinv[t] = v;
         }
-        if (threadIdx.x == 0) nok[0] = sb;
+        if (t == 0) nok[0] = sb;
         return;
     }
-    static_assert(ST == 4 * SM * SM, "four threads per Gram entry");
     __shared__ double2 Gs[SM][SM];
     __shared__ double2 Rs[SM][SM];
     __shared__ double2 Is[SM][SM];
-    __shared__ double2 red[4][SM * SM];
     __shared__ double d0[SM];
-    const int t = threadIdx.x;
-    {
-        // entry (c, d) of C^H C: four threads over interleaved rows i
-        const int e = t % (SM * SM), q = t / (SM * SM);
-        const int c = e / SM, d = e % SM;
-        double2 acc = {0.0, 0.0};
-        if (proj && c < sb && d < sb) {
-            for (int i = q; i < k; i += 4) {
-                const c2<double> a = C[(int64_t)i * SM + c], b = C[(int64_t)i * SM + d];
-                const double2 p = zcmul(double2{a.x, a.y}, double2{b.x, b.y});
-                acc.x += p.x;
-                acc.y += p.y;
+    __shared__ double rinv[SM];
+    __shared__ double2 Cs[kSchRows][SM];
+    __shared__ int ok;
+    // entry (c, d) of C^H C over the basis rows i < k: the rows staged through
+    // LDS kSchRows at a time (coalesced loads), one thread per entry
+    const int c = t / SM, d = t % SM;
+    double2 acc = {0.0, 0.0};
+    if (proj) {
+        for (int i0 = 0; i0 < k; i0 += kSchRows) {
+            const int nr = min(kSchRows, k - i0);
+            for (int e = t; e < nr * SM; e += ST) {
+                const c2<double> v = C[(int64_t)i0 * SM + e];
+                Cs[e / SM][e % SM] = double2{v.x, v.y};
             }
+            __syncthreads();
+            if (t < SM * SM && c < sb && d < sb) {
+                for (int i = 0; i < nr; ++i) {
+                    const double2 p = zcmul(Cs[i][c], Cs[i][d]);
+                    acc.x += p.x;
+                    acc.y += p.y;
+                }
+            }
+            __syncthreads();
         }
-        red[q][e] = acc;
     }
-    __syncthreads();
     if (t < SM * SM) {
-        const int c = t / SM, d = t % SM;
         double2 g = {0.0, 0.0};
         if (c < sb && d < sb) {
             const c2<double> v = C[(int64_t)(k + c) * SM + d];
-            g = double2{v.x - (red[0][t].x + red[1][t].x + red[2][t].x + red[3][t].x),
-                        v.y - (red[0][t].y + red[1][t].y + red[2][t].y + red[3][t].y)};
+            g = double2{v.x - acc.x, v.y - acc.y};
             if (c == d) d0[c] = v.x;
         }
         Gs[c][d] = g;
         Rs[c][d] = double2{0.0, 0.0};
         Is[c][d] = double2{0.0, 0.0};
     }
+    if (t == 0) ok = 0;
     __syncthreads();
+    // right-looking Cholesky G = R^H R (R's diagonal real and positive); the
+    // factor stops at the first pivot not above thresh * the column's squared
+    // norm before the projection
+    for (int j = 0; j < sb; ++j) {
+        if (t == 0) {
+            const double piv = Gs[j][j].x;
+            if (piv > thresh * d0[j]) {
+                const double rjj = sqrt(piv);
+                Rs[j][j] = double2{rjj, 0.0};
+                rinv[j] = 1.0 / rjj;
+                ok = j + 1;
+            }
+        }
+        __syncthreads();
+        if (ok != j + 1) break;
+        if (t > j && t < sb) Rs[j][t] = double2{Gs[j][t].x * rinv[j], Gs[j][t].y * rinv[j]};
+        __syncthreads();
+        if (t < SM * SM) {
+            const int a = t / SM, b = t % SM;
+            if (a > j && b >= a && b < sb) {
+                const double2 p = zcmul(Rs[j][a], Rs[j][b]);
+                Gs[a][b].x -= p.x;
+                Gs[a][b].y -= p.y;
+            }
+        }
+        __syncthreads();
+    }
+    const int nk = ok;
+    // column l of R^{-1} (leading nk x nk upper triangle) by back substitution
+    if (t < nk) {
+        const int l = t;
+        Is[l][l] = double2{rinv[l], 0.0};
+        for (int i = l - 1; i >= 0; --i) {
+            double2 v = {0.0, 0.0};
+            for (int jj = i + 1; jj <= l; ++jj) {
+                const double2 p = zmul(Rs[i][jj], Is[jj][l]);
+                v.x += p.x;
+                v.y += p.y;
+            }
+            Is[i][l] = double2{-v.x * rinv[i], -v.y * rinv[i]};
+        }
+    }
+    __syncthreads();
+    if (t < SM * SM) {
+        const int c = t % SM, d = t / SM;     // column-major: entry (c, d) at d * SM + c
+        const bool in = c < nk && d < nk;
+        R[d * SM + c] = in ? c2<double>{Rs[c][d].x, Rs[c][d].y} : c2<double>{0.0, 0.0};
+        Rinv[d * SM + c] = in ? c2<double>{Is[c][d].x, Is[c][d].y} : c2<double>{0.0, 0.0};
+    }
     if (t == 0) {
-        // R's diagonal is real and positive: every division is by a real pivot
-        double rinv[SM];
-        int ok = 0;
-        for (int j = 0; j < sb; ++j) {
-            double piv = Gs[j][j].x;
-            for (int i = 0; i < j; ++i) piv -= Rs[i][j].x * Rs[i][j].x + Rs[i][j].y * Rs[i][j].y;
-            if (!(piv > thresh * d0[j])) break;
-            const double rjj = sqrt(piv);
-            rinv[j] = 1.0 / rjj;
-            Rs[j][j] = double2{rjj, 0.0};
-            for (int l = j + 1; l < sb; ++l) {
-                double2 v = Gs[j][l];
-                for (int i = 0; i < j; ++i) {
-                    const double2 p = zcmul(Rs[i][j], Rs[i][l]);
-                    v.x -= p.x;
-                    v.y -= p.y;
-                }
-                Rs[j][l] = double2{v.x * rinv[j], v.y * rinv[j]};
-            }
-            ok = j + 1;
-        }
-        // the factor's entries past the accepted columns are zero
-        for (int j = 0; j < SM; ++j)
-            for (int l = 0; l < SM; ++l)
-                if (j >= ok || l >= ok) Rs[j][l] = double2{0.0, 0.0};
-        // inverse of the leading ok x ok upper triangle, column by column
-        for (int l = 0; l < ok; ++l) {
-            Is[l][l] = double2{rinv[l], 0.0};
-            for (int i = l - 1; i >= 0; --i) {
-                double2 v = {0.0, 0.0};
-                for (int j = i + 1; j <= l; ++j) {
-                    const double2 p = zmul(Rs[i][j], Is[j][l]);
-                    v.x += p.x;
-                    v.y += p.y;
-                }
-                Is[i][l] = double2{-v.x * rinv[i], -v.y * rinv[i]};
-            }
-        }
-        nok[0] = ok;
+        nok[0] = nk;
         if (well) {
             // every pivot kept at least kWell of its column's squared norm:
             // the Pythagorean Gram lost at most a factor 1/kWell to
             // cancellation and one CholeskyQR pass leaves the block orthonormal
             // to the basis' own accuracy
             double rmin = 1.0;
-            for (int j = 0; j < ok; ++j) rmin = fmin(rmin, Rs[j][j].x * Rs[j][j].x / d0[j]);
-            *well = (ok == sb && rmin >= kWell) ? 1 : 0;
+            for (int jj = 0; jj < nk; ++jj) rmin = fmin(rmin, Rs[jj][jj].x * Rs[jj][jj].x / d0[jj]);
+            *well = (nk == sb && rmin >= kWell) ? 1 : 0;
         }
-    }
-    __syncthreads();
-    if (t < SM * SM) {
-        const int c = t % SM, d = t / SM;     // column-major: entry (c, d) at d * SM + c
-        R[d * SM + c] = c2<double>{Rs[c][d].x, Rs[c][d].y};
-        Rinv[d * SM + c] = c2<double>{Is[c][d].x, Is[c][d].y};
     }
 }
 
@@ -374,11 +405,15 @@ __global__ __launch_bounds__(ST) void k_supdate(c2<float>* __restrict__ Q, int64
 }
 
 // ---- a block's local Arnoldi: each product's output orthogonalised against
-// the block's own columns Q[c0 .. c0 + nl) (the cycle's last multiplied
-// column and the block's earlier ones), so every product takes a unit vector
-// orthogonal to its block; the block's columns meet the rest of the basis in
-// the block Gram-Schmidt pass.
-constexpr int SL = SM + 1;       // most columns of a block's local orthogonalisation
+// the block's columns Q[c0 .. c0 + nl) (the column it multiplied and the one
+// before it, the cycle's last multiplied column opening the block), so every
+// product takes a unit vector; the block's columns meet each other and the
+// rest of the basis in the block Gram-Schmidt pass.
+constexpr int SL = SM + 1;       // columns b_0 .. b_SM of a block's local relation
+// columns each product is orthogonalised against inside its block: the last
+// two, b_{j-1} and b_j (the block's CholeskyQR makes the rest orthogonal;
+// against all of b_0 .. b_j the 128^3 solve needed the same 166 products)
+constexpr int kBand = 2;
 
 __device__ __forceinline__ float4 ld4(const c2<float>* p) { return *reinterpret_cast<const float4*>(p); }
 // The block's columns are read non-temporally as well: through the caches
@@ -399,22 +434,23 @@ __global__ __launch_bounds__(ST) void k_ldot(const c2<float>* __restrict__ Q, in
                                              const c2<float>* __restrict__ w, int64_t n, c2<double>* __restrict__ part,
                                              int G) {
     static_assert(ST == 256, "the block sums read four LDS entries per lane");
-    __shared__ double sh[(2 * SL + 1) * ST];
-    double acc[2 * SL + 1];
+    static_assert(kBand <= SL, "the band lies inside the block");
+    __shared__ double sh[(2 * kBand + 1) * ST];
+    double acc[2 * kBand + 1];
 #pragma unroll
-    for (int q = 0; q < 2 * SL + 1; ++q) acc[q] = 0.0;
+    for (int q = 0; q < 2 * kBand + 1; ++q) acc[q] = 0.0;
     const int64_t n2 = n >> 1;
     const c2<float>* qb = Q + (int64_t)c0 * ldq;
     for (int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x; e < n2; e += (int64_t)G * ST) {
         // every column's load is issued before the first product uses one
-        float4 a[SL];
+        float4 a[kBand];
 #pragma unroll
-        for (int q = 0; q < SL; ++q)
+        for (int q = 0; q < kBand; ++q)
             if (q < nl) a[q] = ld4nt(qb + (int64_t)q * ldq + 2 * e);
         const float4 wv = ld4(w + 2 * e);
-        acc[2 * SL] += (double)wv.x * wv.x + (double)wv.y * wv.y + (double)wv.z * wv.z + (double)wv.w * wv.w;
+        acc[2 * kBand] += (double)wv.x * wv.x + (double)wv.y * wv.y + (double)wv.z * wv.z + (double)wv.w * wv.w;
 #pragma unroll
-        for (int q = 0; q < SL; ++q) {
+        for (int q = 0; q < kBand; ++q) {
             if (q < nl) {
                 acc[2 * q] += (double)a[q].x * wv.x + (double)a[q].y * wv.y + (double)a[q].z * wv.z +
                               (double)a[q].w * wv.w;
@@ -425,7 +461,7 @@ __global__ __launch_bounds__(ST) void k_ldot(const c2<float>* __restrict__ Q, in
     }
     if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
         const double2 wv = ld2(w + n - 1);
-        acc[2 * SL] += wv.x * wv.x + wv.y * wv.y;
+        acc[2 * kBand] += wv.x * wv.x + wv.y * wv.y;
         for (int q = 0; q < nl; ++q) {
             const double2 a = ld2(Q + (int64_t)(c0 + q) * ldq + n - 1);
             acc[2 * q] += a.x * wv.x + a.y * wv.y;
@@ -436,19 +472,19 @@ __global__ __launch_bounds__(ST) void k_ldot(const c2<float>* __restrict__ Q, in
     // adds rows w, w + 4, ... (four LDS reads per lane, then a shuffle tree)
     // and its lane 0 writes the row's real or imaginary part
 #pragma unroll
-    for (int q = 0; q < 2 * SL + 1; ++q) sh[q * ST + threadIdx.x] = acc[q];
+    for (int q = 0; q < 2 * kBand + 1; ++q) sh[q * ST + threadIdx.x] = acc[q];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double* pd = reinterpret_cast<double*>(part);
-    for (int q = wave; q < 2 * SL + 1; q += ST / 64) {
-        const bool used = q == 2 * SL || (q >> 1) < nl;
+    for (int q = wave; q < 2 * kBand + 1; q += ST / 64) {
+        const bool used = q == 2 * kBand || (q >> 1) < nl;
         if (!used) continue;
         double v = sh[q * ST + lane] + sh[q * ST + lane + 64] + sh[q * ST + lane + 128] + sh[q * ST + lane + 192];
         for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
         if (lane == 0) {
-            const int64_t slot = q == 2 * SL ? (int64_t)nl * G + blockIdx.x : (int64_t)(q >> 1) * G + blockIdx.x;
-            pd[2 * slot + (q == 2 * SL ? 0 : (q & 1))] = v;
-            if (q == 2 * SL) pd[2 * slot + 1] = 0.0;
+            const int64_t slot = q == 2 * kBand ? (int64_t)nl * G + blockIdx.x : (int64_t)(q >> 1) * G + blockIdx.x;
+            pd[2 * slot + (q == 2 * kBand ? 0 : (q & 1))] = v;
+            if (q == 2 * kBand) pd[2 * slot + 1] = 0.0;
         }
     }
 }
@@ -480,7 +516,7 @@ __global__ __launch_bounds__(ST) void k_lcomb(c2<float>* __restrict__ out, const
                                               const c2<float>* __restrict__ Q, int64_t ldq, int c0, int nl,
                                               c2<double>* __restrict__ h, const c2<double>* __restrict__ w2,
                                               int64_t n) {
-    __shared__ c2<double> cs[SL];
+    __shared__ c2<double> cs[kBand];
     __shared__ double scale;
     if (threadIdx.x < nl) cs[threadIdx.x] = h[threadIdx.x];
     __syncthreads();
@@ -497,14 +533,14 @@ __global__ __launch_bounds__(ST) void k_lcomb(c2<float>* __restrict__ out, const
     const int64_t n2 = n >> 1;
     const c2<float>* qb = Q + (int64_t)c0 * ldq;
     for (int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x; e < n2; e += (int64_t)gridDim.x * ST) {
-        float4 qv[SL];
+        float4 qv[kBand];
 #pragma unroll
-        for (int i = 0; i < SL; ++i)
+        for (int i = 0; i < kBand; ++i)
             if (i < nl) qv[i] = ld4nt(qb + (int64_t)i * ldq + 2 * e);
         const float4 wv = ld4(w + 2 * e);
         double ax = wv.x, ay = wv.y, bx = wv.z, by = wv.w;
 #pragma unroll
-        for (int i = 0; i < SL; ++i) {
+        for (int i = 0; i < kBand; ++i) {
             if (i < nl) {
                 const float4 q = qv[i];
                 const c2<double> c = cs[i];
@@ -595,7 +631,7 @@ struct SWs {
     double* npart = nullptr;
     double* nout = nullptr;
     c2<float>* wt = nullptr;     // n: a product's output before its local orthogonalisation
-    c2<double>* lpart = nullptr; // SL x SG local dot partials
+    c2<double>* lpart = nullptr; // (kBand + 1) x SG local dot partials
     c2<double>* hloc = nullptr;  // SM x (SL + 1) local Hessenberg columns of a block
     c2<double>* w2 = nullptr;    // |w|^2 of the product being orthogonalised
     void* hp = nullptr;          // pinned host mirror
@@ -748,23 +784,29 @@ struct SGivens {
 };
 
 // The block's sb products and their local Arnoldi: b_0 = q_{k-1} = Q[k-1],
-// w = A b_j, h = [b_0 .. b_j]^H w, Q[k + j] = b_{j+1} = (w - [b_0 .. b_j] h) / ||.||
-// (local Hessenberg column j of the block in hloc, on the device).
+// w = A b_j, h = B^H w, Q[k + j] = b_{j+1} = (w - B h) / ||.|| with B = [b_{l0}
+// .. b_j], the last min(j + 1, kBand) columns (local Hessenberg column j of
+// the block in hloc, on the device; its other entries zero).
 int s_products(SCtx& c, int k, int sb) {
     const size_t ldq = (size_t)c.ldq;
+    if (kBand < SL)
+        MARIE_HIP(hipMemsetAsync(c.w.hloc, 0, (size_t)SM * (SL + 1) * sizeof(c2<double>), c.s));
     for (int j = 0; j < sb; ++j) {
         const c2<float>* in = c.w.Q + (size_t)(k - 1 + j) * ldq;
         c2<float>* out = c.w.Q + (size_t)(k + j) * ldq;
         const int st = apply(c.h, OP_AN, c.transp, PREC_F32, in, c.w.wt, 1, c.s);
         if (st) return st;
-        c2<double>* hj = c.w.hloc + (size_t)j * (SL + 1);
+        // the product is orthogonalised against the block's last nl columns
+        // b_{l0} .. b_j (all of them while j + 1 <= kBand)
+        const int nl = std::min(j + 1, kBand), l0 = j + 1 - nl;
+        c2<double>* hj = c.w.hloc + (size_t)j * (SL + 1) + l0;
         // a quarter of the combination's blocks: each block's 19-row LDS sum
         // then covers four times the rows
         const int gd = std::max(1, c.G / 4);
-        hipLaunchKernelGGL(k_ldot, dim3(gd), dim3(ST), 0, c.s, c.w.Q, c.ldq, k - 1, j + 1, c.w.wt, c.n, c.w.lpart,
-                           gd);
-        hipLaunchKernelGGL(k_lsum, dim3(j + 2), dim3(ST), 0, c.s, c.w.lpart, gd, j + 1, hj, c.w.w2);
-        hipLaunchKernelGGL(k_lcomb, dim3(c.G), dim3(ST), 0, c.s, out, c.w.wt, c.w.Q, c.ldq, k - 1, j + 1, hj,
+        hipLaunchKernelGGL(k_ldot, dim3(gd), dim3(ST), 0, c.s, c.w.Q, c.ldq, k - 1 + l0, nl, c.w.wt, c.n,
+                           c.w.lpart, gd);
+        hipLaunchKernelGGL(k_lsum, dim3(nl + 1), dim3(ST), 0, c.s, c.w.lpart, gd, nl, hj, c.w.w2);
+        hipLaunchKernelGGL(k_lcomb, dim3(c.G), dim3(ST), 0, c.s, out, c.w.wt, c.w.Q, c.ldq, k - 1 + l0, nl, hj,
                            c.w.w2, c.n);
         MARIE_HIP(hipGetLastError());
     }
@@ -978,7 +1020,7 @@ int sstep_refine(Op* h, hipStream_t s, int transp, const c2<double>* b, c2<doubl
     if (!st) st = A((void**)&c.w.npart, SG * sizeof(double));
     if (!st) st = A((void**)&c.w.nout, sizeof(double));
     if (!st) st = A((void**)&c.w.wt, (size_t)c.ldq * sizeof(c2<float>));
-    if (!st) st = A((void**)&c.w.lpart, (size_t)(SL + 1) * SG * sizeof(c2<double>));
+    if (!st) st = A((void**)&c.w.lpart, (size_t)(kBand + 1) * SG * sizeof(c2<double>));
     if (!st) st = A((void**)&c.w.w2, sizeof(c2<double>));
     if (!st) st = A((void**)&c.w.hloc, (size_t)SM * (SL + 1) * sizeof(c2<double>));
     c.w.hp_bytes = ((size_t)ncolmax * SM + 4 * SM * SM + SM * (SL + 1)) * sizeof(c2<double>) + 2 * sizeof(int);
