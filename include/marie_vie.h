@@ -307,10 +307,20 @@ int marie_vie_material_gradient_device(marie_vie_op* h, const void* lambda_dev, 
  * marie_vie_gmres_device in `mode` (0-3; 3: block-orthogonalised complex64
  * cycles, the fastest per column at 128^3) with `restart` its cycle length
  * (then no nrhs limit); iter = {largest last-cycle count, cycles of all
- * columns}, resvec each column's final relres. */
+ * columns}, resvec each column's final relres.
+ * MARIE_BLOCK_DEVICE (modes 0-1, nrhs <= 32) runs every part of a block step
+ * on the device -- the product, the Gram-Schmidt pass, CholeskyQR (the s x s
+ * factor on the device, the third pass skipped on the device when the first
+ * factors unshifted) and the Householder least-squares update -- so a step
+ * is one stream-ordered sequence without host round trips; with
+ * MARIE_BLOCK_GRAPH the whole step is captured into a HIP graph once per step
+ * index and replayed in every cycle. The host reads one status record per
+ * step (the columns' least-squares residuals) and one per cycle (the true
+ * residual norms); the arithmetic and the stopping rule are those above. */
 #define MARIE_BLOCK_GRAPH 1
 #define MARIE_BLOCK_CGS2 2
 #define MARIE_BLOCK_COLUMNS 4
+#define MARIE_BLOCK_DEVICE 8
 int marie_vie_block_gmres_device(marie_vie_op* h, const void* B_dev, void* X_dev, int nrhs, int restart,
                                  double tol, int maxit, int mode, double inner_tol, int options,
                                  int* flag, double* relres, int* iter, double* resvec, int resvec_cap,
@@ -329,8 +339,13 @@ int marie_vie_block_gmres(marie_vie_op* h, const double* B, double* X, int nrhs,
  * the device. ops[i], B[i], X[i], nrhs[i] as the single-problem entry;
  * restart, tol, maxit, mode, inner_tol, options shared. flag and relres
  * hold sum(nrhs) entries in problem order, iter 2 per problem, status the
- * return code of each problem. The batch form of fVIEsolver_Assembly's
- * per-port solves over several bodies (VSIE_Solver.m:387-407 per body). */
+ * return code of each problem. With MARIE_BLOCK_DEVICE one host thread drives
+ * every problem instead: each problem's block steps (HIP graphs with
+ * MARIE_BLOCK_GRAPH) are launched on its stream, and whichever problem's step
+ * has completed -- its status event -- gets its next step, its cycle's end or
+ * its next cycle enqueued, so the problems' steps overlap on the device
+ * without host workers. The batch form of fVIEsolver_Assembly's per-port
+ * solves over several bodies (VSIE_Solver.m:387-407 per body). */
 int marie_vie_block_gmres_batch(marie_vie_op* const* ops, int nitems, const void* const* B_dev,
                                 void* const* X_dev, const int* nrhs, int restart, double tol, int maxit,
                                 int mode, double inner_tol, int options, int* flag, double* relres,

@@ -127,7 +127,8 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 template <int WPT>
 __global__ __launch_bounds__(BT) void k_vhw(const c2<double>* __restrict__ V, int64_t ldv, int P,
                                             const c2<double>* __restrict__ W, int64_t ldw, int Q, int64_t n,
-                                            c2<double>* __restrict__ part, int G) {
+                                            c2<double>* __restrict__ part, int G, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     constexpr int NSUB = 4 / WPT, TP = 16 * NSUB;
     __shared__ c2<double> Vs[KR][TP + 1];
     __shared__ c2<double> Ws[KR][VQ + 1];
@@ -212,7 +213,9 @@ __global__ __launch_bounds__(BT) void k_vhw(const c2<double>* __restrict__ V, in
 // C[q * ldc + p] = sum_rb part[(q P + p) G + rb]: one wave per output, lanes
 // over rb in a fixed order, then a fixed shuffle tree
 __global__ __launch_bounds__(BT) void k_vhw_reduce(const c2<double>* __restrict__ part, int P, int Q, int G,
-                                                   c2<double>* __restrict__ C, int64_t ldc) {
+                                                   c2<double>* __restrict__ C, int64_t ldc,
+                                                   const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int o = blockIdx.x * (BT / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (o >= P * Q) return;
@@ -239,7 +242,8 @@ __global__ __launch_bounds__(BT) void k_vhw_reduce(const c2<double>* __restrict_
 constexpr int CQ = 16;
 __global__ __launch_bounds__(BT) void k_vc(c2<double>* __restrict__ out, int64_t ldo, const c2<double>* __restrict__ V,
                                            int64_t ldv, int P, const c2<double>* __restrict__ C, int64_t ldc, int Q,
-                                           int64_t n, double alpha, double beta) {
+                                           int64_t n, double alpha, double beta, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int q0 = blockIdx.y * CQ;
     const int nq = Q - q0 < CQ ? Q - q0 : CQ;
     const int64_t e = blockIdx.x * (int64_t)BT + threadIdx.x;
@@ -316,8 +320,9 @@ int vhw_blocks(int tiles) {
     return g < 64 ? 64 : (g > 512 ? 512 : g);
 }
 
-// C (P x Q, ld ldc) = V^H W over n rows
-int vhw(BCtx& c, const c2<double>* V, int P, const c2<double>* W, int Q, c2<double>* C, int64_t ldc) {
+// C (P x Q, ld ldc) = V^H W over n rows (nothing when *skip is set on the device)
+int vhw(BCtx& c, const c2<double>* V, int P, const c2<double>* W, int Q, c2<double>* C, int64_t ldc,
+        const int* skip = nullptr) {
     // waves per 16-column subtile: a narrow V (the CholeskyQR Gram, early
     // block steps) splits the k-steps over the four waves instead
     const int wpt = P <= 16 ? 4 : (P <= 32 ? 2 : 1);
@@ -330,22 +335,22 @@ int vhw(BCtx& c, const c2<double>* V, int P, const c2<double>* W, int Q, c2<doub
     }
     const dim3 grid(ptiles, G, qtiles);
     if (wpt == 4)
-        hipLaunchKernelGGL(k_vhw<4>, grid, dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
+        hipLaunchKernelGGL(k_vhw<4>, grid, dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G, skip);
     else if (wpt == 2)
-        hipLaunchKernelGGL(k_vhw<2>, grid, dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
+        hipLaunchKernelGGL(k_vhw<2>, grid, dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G, skip);
     else
-        hipLaunchKernelGGL(k_vhw<1>, grid, dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G);
+        hipLaunchKernelGGL(k_vhw<1>, grid, dim3(BT), 0, c.s, V, c.n, P, W, c.n, Q, c.n, c.ws.vpart, G, skip);
     hipLaunchKernelGGL(k_vhw_reduce, dim3((P * Q + BT / 64 - 1) / (BT / 64)), dim3(BT), 0, c.s, c.ws.vpart, P, Q, G,
-                       C, ldc);
+                       C, ldc, skip);
     MARIE_HIP(hipGetLastError());
     return OK;
 }
 
 // out (n x Q) = beta * out + alpha * V (n x P) C (P x Q, ld ldc)
 int vc(BCtx& c, c2<double>* out, const c2<double>* V, int P, const c2<double>* C, int64_t ldc, int Q, double alpha,
-       double beta) {
+       double beta, const int* skip = nullptr) {
     const dim3 g((unsigned)((c.n + BT - 1) / BT), (unsigned)((Q + CQ - 1) / CQ));
-    hipLaunchKernelGGL(k_vc, g, dim3(BT), 0, c.s, out, c.n, V, c.n, P, C, ldc, Q, c.n, alpha, beta);
+    hipLaunchKernelGGL(k_vc, g, dim3(BT), 0, c.s, out, c.n, V, c.n, P, C, ldc, Q, c.n, alpha, beta, skip);
     MARIE_HIP(hipGetLastError());
     return OK;
 }
@@ -669,6 +674,772 @@ int block_cycle(BCtx& c, int m, const std::vector<double>& target, bool f32, boo
     return OK;
 }
 
+// ---- device-driven block steps -----------------------------------------------
+// Every part of a block step runs on the device: the product, the block
+// Gram-Schmidt pass, CholeskyQR (k_bchol: the s x s factor on the device, the
+// third pass skipped on the device when the first factored unshifted) and the
+// Householder update of the least-squares problem (k_bls). A step is then one
+// stream-ordered sequence with no host round trip, captured once per step
+// index into a HIP graph and replayed in every cycle; the host reads one
+// status record per step (the columns' least-squares residuals) and decides
+// whether to take another step or close the cycle (k_bsolve, V Y, the true
+// residual). One host thread drives any number of such solves by their
+// events (marie_vie_block_gmres_batch).
+
+constexpr int DS = 32;           // most right-hand sides of a device-driven block
+constexpr int DLS = 1024;        // threads of the least-squares update kernel
+
+struct DevStatus {
+    double res[DS];              // |LS residual| per column after the last block step
+    double rn2[DS];              // squared true residual norms (cycle end)
+    int skip3;                   // CholeskyQR: the first pass factored unshifted, the third is skipped
+    int brk;                     // CholeskyQR breakdown in the last block step
+    int brk0;                    // breakdown of the cycle's first CholeskyQR (the residual block)
+    int pad;
+};
+
+__device__ __forceinline__ double2 dmul(double2 a, double2 b) {
+    return double2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ double2 dcmul(double2 a, double2 b) {      // conj(a) b
+    return double2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double2 ld2d(const c2<double>* p) {
+    const c2<double> v = *p;
+    return double2{v.x, v.y};
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One CholeskyQR factor on the device (s <= DS): the Gram Gm (s x s,
+// column-major) equilibrated to unit diagonal and factored R^H R (R upper,
+// real positive diagonal; right-looking, one thread per trailing entry).
+// Pass 0 first tries the plain factor with pivots above 1e-10 of the largest
+// diagonal entry (then *skip3 = 1: two passes suffice) and otherwise factors
+// Gram + shift I (*skip3 = 0); later passes factor plainly (pivots above
+// 1e-28). R is scaled back (R D), Rinv = R^-1 (column-major, the k_vc
+// coefficient layout), Rf = R (pass 0) or R Rf. A zero, non-finite or
+// indefinite Gram sets *brk and zeroes Rinv and Rf; later passes of the same
+// factorisation do nothing once *brk is set (pass 0 clears it).
+__global__ __launch_bounds__(BT) void k_bchol(const c2<double>* __restrict__ Gm, int s, int pass, double shift,
+                                              c2<double>* __restrict__ Rinv, c2<double>* __restrict__ Rf,
+                                              int* __restrict__ skip3, int* __restrict__ brk,
+                                              const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    if (pass > 0 && *brk) return;
+    __shared__ double2 A[DS][DS + 1];     // the equilibrated Gram, then R (upper)
+    __shared__ double2 X[DS][DS + 1];     // R^-1
+    __shared__ double2 F[DS][DS + 1];     // the previous Rf
+    __shared__ double d[DS], rinv[DS];
+    __shared__ double amax;
+    __shared__ int ok, bad;
+    const int t = threadIdx.x;
+    if (t == 0) bad = 0;
+    __syncthreads();
+    if (t < s) {
+        const double dd = sqrt(Gm[(int64_t)t * s + t].x);
+        d[t] = dd;
+        if (!(dd > 0.0) || !isfinite(dd)) bad = 1;
+    }
+    if (pass > 0)
+        for (int e = t; e < DS * DS; e += BT) {
+            const int r = e % DS, cc = e / DS;
+            F[r][cc] = (r < s && cc < s) ? ld2d(Rf + (int64_t)cc * s + r) : double2{0.0, 0.0};
+        }
+    __syncthreads();
+    bool factored = false;
+    if (!bad) {
+        for (int attempt = 0;; ++attempt) {
+            const double minp = (pass == 0 && attempt == 0) ? 1e-10 : 1e-28;
+            for (int e = t; e < DS * DS; e += BT) {
+                const int r = e % DS, cc = e / DS;
+                double2 v = {0.0, 0.0};
+                if (r < s && cc < s) {
+                    const c2<double> g = Gm[(int64_t)cc * s + r];
+                    const double sc = 1.0 / (d[r] * d[cc]);
+                    v = double2{g.x * sc, g.y * sc};
+                    if (r == cc && attempt == 1) v.x += shift;
+                }
+                A[r][cc] = v;
+                X[r][cc] = double2{0.0, 0.0};
+            }
+            __syncthreads();
+            if (t == 0) {
+                double am = 0.0;
+                for (int i = 0; i < s; ++i) am = fmax(am, A[i][i].x);
+                amax = am;
+                ok = 0;
+            }
+            __syncthreads();
+            for (int jj = 0; jj < s; ++jj) {
+                if (t == 0) {
+                    const double piv = A[jj][jj].x;
+                    if (piv > minp * amax && isfinite(piv)) {
+                        const double rjj = sqrt(piv);
+                        A[jj][jj] = double2{rjj, 0.0};
+                        rinv[jj] = 1.0 / rjj;
+                        ok = jj + 1;
+                    }
+                }
+                __syncthreads();
+                if (ok != jj + 1) break;
+                if (t > jj && t < s) A[jj][t] = double2{A[jj][t].x * rinv[jj], A[jj][t].y * rinv[jj]};
+                __syncthreads();
+                for (int e = t; e < DS * DS; e += BT) {
+                    const int a = e / DS, b = e % DS;
+                    if (a > jj && b >= a && b < s) {
+                        const double2 p = dcmul(A[jj][a], A[jj][b]);
+                        A[a][b].x -= p.x;
+                        A[a][b].y -= p.y;
+                    }
+                }
+                __syncthreads();
+            }
+            if (ok == s) {
+                factored = true;
+                if (pass == 0 && t == 0) *skip3 = attempt == 0 ? 1 : 0;
+                break;
+            }
+            if (pass != 0 || attempt == 1) break;
+            __syncthreads();
+        }
+    }
+    if (!factored) {
+        for (int e = t; e < s * s; e += BT) {
+            Rinv[e] = c2<double>{0.0, 0.0};
+            Rf[e] = c2<double>{0.0, 0.0};
+        }
+        if (t == 0) {
+            *brk = 1;
+            if (pass == 0) *skip3 = 1;
+        }
+        return;
+    }
+    if (pass == 0 && t == 0) *brk = 0;
+    // R D, then R^-1 column by column
+    for (int e = t; e < DS * DS; e += BT) {
+        const int r = e % DS, cc = e / DS;
+        if (r <= cc && cc < s) A[r][cc] = double2{A[r][cc].x * d[cc], A[r][cc].y * d[cc]};
+    }
+    __syncthreads();
+    if (t < s) rinv[t] = 1.0 / A[t][t].x;
+    __syncthreads();
+    if (t < s) {
+        const int l = t;
+        X[l][l] = double2{rinv[l], 0.0};
+        for (int i = l - 1; i >= 0; --i) {
+            double2 v = {0.0, 0.0};
+            for (int k = i + 1; k <= l; ++k) {
+                const double2 p = dmul(A[i][k], X[k][l]);
+                v.x += p.x;
+                v.y += p.y;
+            }
+            X[i][l] = double2{-v.x * rinv[i], -v.y * rinv[i]};
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < s * s; e += BT) {
+        const int r = e % s, cc = e / s;
+        const double2 xi = r <= cc ? X[r][cc] : double2{0.0, 0.0};
+        Rinv[e] = c2<double>{xi.x, xi.y};
+        double2 f = {0.0, 0.0};
+        if (r <= cc) {
+            if (pass == 0) {
+                f = A[r][cc];
+            } else {
+                for (int k = r; k <= cc; ++k) {
+                    const double2 p = dmul(A[r][k], F[k][cc]);
+                    f.x += p.x;
+                    f.y += p.y;
+                }
+            }
+        }
+        Rf[e] = c2<double>{f.x, f.y};
+    }
+}
+
+// Least-squares update of block step j (one workgroup, s <= DS): the new block
+// column (columns js .. js+s-1 of Rq, ld rows, nonzero in rows 0 .. (j+2)s-1)
+// meets the reflectors of the earlier columns in order, then its own:
+// reflector c zeroes rows c+1 .. c+s of column c (v = x + phase(x_0) ||x|| e_0,
+// beta = 2 / v^H v, the identity when x is already zero below row c) and is
+// applied to the block's later columns and to the right-hand side G (ld x s).
+// res[q] = || G(rows (j+1)s .. (j+2)s-1, column q) ||.
+// Earlier reflectors: one wave per new column holds rows p .. p+s of its
+// column in lanes 0 .. s while p runs over the reflectors (staged in LDS 64 at
+// a time); row p is stored when it leaves the window and row p+s+1 comes from
+// a register chunk loaded one chunk ahead. The block's own reflectors work on
+// the 2s x s window of rows js .. js+2s-1 in LDS.
+__global__ __launch_bounds__(DLS) void k_bls(c2<double>* __restrict__ Rq, int64_t ld, c2<double>* __restrict__ G,
+                                             c2<double>* __restrict__ hv, double* __restrict__ hb, int s, int j,
+                                             double* __restrict__ res) {
+    constexpr int NW = DLS / 64;
+    static_assert(2 * NW >= DS, "at most two new columns per wave");
+    __shared__ double2 vs[64][DS + 1];
+    __shared__ double bs[64];
+    __shared__ double2 Wn[2 * DS][DS + 1];
+    __shared__ double2 Gw[2 * DS][DS + 1];
+    __shared__ double2 vcur[DS + 1];
+    __shared__ double bcur;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int js = j * s, L = s + 1;
+    const double2 z = {0.0, 0.0};
+    // ---- earlier reflectors p < js on the new columns
+    double2 x[2], inc[2], nxt[2];
+    int64_t cb[2];
+    bool on[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int q = wave + u * NW;
+        on[u] = q < s;
+        cb[u] = (int64_t)(js + (on[u] ? q : 0)) * ld;
+        x[u] = (on[u] && lane < L) ? ld2d(Rq + cb[u] + lane) : z;
+        // rows entering the window during the first chunk: s+1 .. s+64
+        const int64_t r0 = s + 1 + lane;
+        inc[u] = (on[u] && r0 < ld) ? ld2d(Rq + cb[u] + r0) : z;
+    }
+    for (int p0 = 0; p0 < js; p0 += 64) {
+        const int cnt = js - p0 < 64 ? js - p0 : 64;
+        __syncthreads();
+        for (int e = threadIdx.x; e < cnt * L; e += DLS) {
+            const int i = e / L, k = e % L;
+            vs[i][k] = ld2d(hv + (int64_t)(p0 + i) * L + k);
+        }
+        for (int i = threadIdx.x; i < cnt; i += DLS) bs[i] = hb[p0 + i];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int64_t r1 = (int64_t)p0 + 64 + s + 1 + lane;      // the next chunk's entering rows
+            nxt[u] = (on[u] && r1 < ld) ? ld2d(Rq + cb[u] + r1) : z;
+        }
+        __syncthreads();
+        for (int pl = 0; pl < cnt; ++pl) {
+            const double2 v = lane < L ? vs[pl][lane] : z;
+            const double beta = bs[pl];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (!on[u]) continue;
+                const double2 pr = dcmul(v, x[u]);
+                const double dx = wave_sum(pr.x) * beta, dy = wave_sum(pr.y) * beta;
+                const double2 dv = dmul(v, double2{dx, dy});
+                x[u].x -= dv.x;
+                x[u].y -= dv.y;
+                if (lane == 0) Rq[cb[u] + p0 + pl] = c2<double>{x[u].x, x[u].y};
+                const double2 dn = {__shfl_down(x[u].x, 1, 64), __shfl_down(x[u].y, 1, 64)};
+                const double2 in = {__shfl(inc[u].x, pl, 64), __shfl(inc[u].y, pl, 64)};
+                x[u] = lane == s ? in : (lane < s ? dn : z);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) inc[u] = nxt[u];
+    }
+    // ---- the block's own reflectors on rows js .. js+2s-1
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int q = wave + u * NW;
+        if (on[u] && lane < L) Wn[lane][q] = x[u];
+    }
+    for (int e = threadIdx.x; e < 2 * s * s; e += DLS) {
+        const int r = e % (2 * s), q = e / (2 * s);
+        if (r > s) Wn[r][q] = ld2d(Rq + (int64_t)(js + q) * ld + js + r);
+        Gw[r][q] = ld2d(G + (int64_t)q * ld + js + r);
+    }
+    __syncthreads();
+    for (int q = 0; q < s; ++q) {
+        const int c = js + q;
+        if (wave == 0) {
+            const double2 xv = lane < L ? Wn[q + lane][q] : z;
+            const double nx2 = wave_sum(xv.x * xv.x + xv.y * xv.y);
+            const double2 x0 = {__shfl(xv.x, 0, 64), __shfl(xv.y, 0, 64)};
+            const double nx = sqrt(nx2);
+            const double tail = nx2 - (x0.x * x0.x + x0.y * x0.y);
+            double beta = 0.0;
+            double2 v = z;
+            if (!(nx == 0.0 || tail <= 0.0)) {
+                const double a0 = sqrt(x0.x * x0.x + x0.y * x0.y);
+                const double2 ph = a0 > 0.0 ? double2{x0.x / a0, x0.y / a0} : double2{1.0, 0.0};
+                v = xv;
+                if (lane == 0) {
+                    v.x += ph.x * nx;
+                    v.y += ph.y * nx;
+                }
+                beta = 2.0 / wave_sum(v.x * v.x + v.y * v.y);
+                const double2 pr = dcmul(v, xv);
+                const double dx = wave_sum(pr.x) * beta, dy = wave_sum(pr.y) * beta;
+                const double2 dv = dmul(v, double2{dx, dy});
+                if (lane < L) Wn[q + lane][q] = lane == 0 ? double2{xv.x - dv.x, xv.y - dv.y} : z;
+            }
+            if (lane < L) {
+                vcur[lane] = v;
+                hv[(int64_t)c * L + lane] = c2<double>{v.x, v.y};
+            }
+            if (lane == 0) {
+                bcur = beta;
+                hb[c] = beta;
+            }
+        }
+        __syncthreads();
+        const double beta = bcur;
+        if (beta != 0.0) {
+            const int nw = s - q - 1;               // the block's later columns, then G's s columns
+            for (int k = wave; k < nw + s; k += NW) {
+                const bool inW = k < nw;
+                const int col = inW ? q + 1 + k : k - nw;
+                const double2 xv = lane < L ? (inW ? Wn[q + lane][col] : Gw[q + lane][col]) : z;
+                const double2 v = lane < L ? vcur[lane] : z;
+                const double2 pr = dcmul(v, xv);
+                const double dx = wave_sum(pr.x) * beta, dy = wave_sum(pr.y) * beta;
+                const double2 dv = dmul(v, double2{dx, dy});
+                if (lane < L) {
+                    const double2 o = {xv.x - dv.x, xv.y - dv.y};
+                    if (inW) Wn[q + lane][col] = o;
+                    else Gw[q + lane][col] = o;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < 2 * s * s; e += DLS) {
+        const int r = e % (2 * s), q = e / (2 * s);
+        Rq[(int64_t)(js + q) * ld + js + r] = c2<double>{Wn[r][q].x, Wn[r][q].y};
+        G[(int64_t)q * ld + js + r] = c2<double>{Gw[r][q].x, Gw[r][q].y};
+    }
+    if (threadIdx.x < s) {
+        const int q = threadIdx.x;
+        double r2 = 0.0;
+        for (int r = s; r < 2 * s; ++r) r2 += Gw[r][q].x * Gw[r][q].x + Gw[r][q].y * Gw[r][q].y;
+        res[q] = sqrt(r2);
+    }
+}
+
+// Y (K x s, column-major) = R^-1 G(0:K, :) with R the leading K x K upper
+// triangle of Rq: one workgroup per column of G, column-oriented back
+// substitution in LDS (y_k = g_k / R_kk, then g(0:k) -= R(0:k, k) y_k), the
+// next column of R loaded while the current one is applied.
+constexpr int BSR = 8;           // rows of R per thread: K <= BSR * BT
+__global__ __launch_bounds__(BT) void k_bsolve(const c2<double>* __restrict__ Rq, int64_t ld,
+                                               const c2<double>* __restrict__ G, int K, c2<double>* __restrict__ Y) {
+    extern __shared__ double2 sm_[];
+    double2* g = sm_;
+    double2* dg = sm_ + K;
+    const int q = blockIdx.x, t = threadIdx.x;
+    for (int i = t; i < K; i += BT) {
+        g[i] = ld2d(G + (int64_t)q * ld + i);
+        dg[i] = ld2d(Rq + (int64_t)i * ld + i);
+    }
+    double2 cur[BSR], nxt[BSR];
+#pragma unroll
+    for (int i = 0; i < BSR; ++i) {
+        const int r = t + BT * i;
+        cur[i] = r < K - 1 ? ld2d(Rq + (int64_t)(K - 1) * ld + r) : double2{0.0, 0.0};
+    }
+    __syncthreads();
+    for (int k = K - 1; k >= 0; --k) {
+#pragma unroll
+        for (int i = 0; i < BSR; ++i) {
+            const int r = t + BT * i;
+            nxt[i] = (k > 0 && r < k - 1) ? ld2d(Rq + (int64_t)(k - 1) * ld + r) : double2{0.0, 0.0};
+        }
+        const double2 gk = g[k], rk = dg[k];
+        const double den = rk.x * rk.x + rk.y * rk.y;
+        const double2 yk = {(gk.x * rk.x + gk.y * rk.y) / den, (gk.y * rk.x - gk.x * rk.y) / den};
+#pragma unroll
+        for (int i = 0; i < BSR; ++i) {
+            const int r = t + BT * i;
+            if (r < k) {
+                const double2 p = dmul(cur[i], yk);
+                g[r].x -= p.x;
+                g[r].y -= p.y;
+            }
+        }
+        if (t == 0) Y[(int64_t)q * K + k] = c2<double>{yk.x, yk.y};
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < BSR; ++i) cur[i] = nxt[i];
+    }
+}
+
+// Rq block column col0 .. col0+s-1: rows 0 .. P-1 = Hd (P x s, column-major),
+// or + Hd when acc; without acc the rows from P on are zeroed
+__global__ void k_hset(const c2<double>* __restrict__ Hd, int P, int s, c2<double>* __restrict__ Rq, int64_t ld,
+                       int col0, int acc) {
+    const int64_t rows = acc ? P : ld;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < rows * s;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e % rows;
+        const int q = (int)(e / rows);
+        c2<double>* o = Rq + (int64_t)(col0 + q) * ld + r;
+        if (r < P) {
+            const c2<double> h = Hd[(int64_t)q * P + r];
+            *o = acc ? c2<double>{o->x + h.x, o->y + h.y} : h;
+        } else {
+            *o = c2<double>{0.0, 0.0};
+        }
+    }
+}
+
+// rows P .. P+s-1 of the block column = Rf (upper, column-major s x s)
+__global__ void k_rset(const c2<double>* __restrict__ Rf, int s, c2<double>* __restrict__ Rq, int64_t ld, int col0,
+                       int P) {
+    const int e = threadIdx.x;
+    if (e >= s * s) return;
+    const int i = e % s, q = e / s;
+    Rq[(int64_t)(col0 + q) * ld + P + i] = i <= q ? Rf[(int64_t)q * s + i] : c2<double>{0.0, 0.0};
+}
+
+// G (ld x s) = [S0; 0] with S0 upper (column-major s x s)
+__global__ void k_ginit(const c2<double>* __restrict__ S0, int s, c2<double>* __restrict__ G, int64_t ld) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ld * s; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % ld;
+        const int q = (int)(e / ld);
+        G[e] = (i <= q) ? S0[(int64_t)q * s + i] : c2<double>{0.0, 0.0};
+    }
+}
+
+// dst = src, nothing when *skip
+__global__ void k_copy_skip(c2<double>* __restrict__ dst, const c2<double>* __restrict__ src, int64_t n,
+                            const int* __restrict__ skip) {
+    if (*skip) return;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = src[e];
+}
+
+struct DevWs {
+    c2<double>* Rq = nullptr;    // ld x m s transformed block Hessenberg
+    c2<double>* Gd = nullptr;    // ld x s transformed right-hand side
+    c2<double>* hv = nullptr;    // m s reflectors of length s + 1
+    double* hb = nullptr;        // m s reflector factors
+    c2<double>* Gm = nullptr;    // s x s CholeskyQR Gram
+    c2<double>* Ri = nullptr;    // 3 s x s inverse factors
+    c2<double>* Rf = nullptr;    // s x s accumulated factor
+    DevStatus* st = nullptr;     // device status
+    DevStatus* hs = nullptr;     // pinned mirror
+    hipEvent_t ev = nullptr;
+    std::vector<hipGraphExec_t> graphs;
+    std::vector<hipGraph_t> gsrc;
+    void free_all() {
+        void* p[] = {Rq, Gd, hv, hb, Gm, Ri, Rf, st};
+        for (void* q : p)
+            if (q) (void)hipFree(q);
+        if (hs) (void)hipHostFree(hs);
+        if (ev) (void)hipEventDestroy(ev);
+        for (auto g : graphs)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (auto g : gsrc)
+            if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+// One restarted block GMRES solve driven by events: every call of advance()
+// after its event completed reads the status and enqueues the next step, the
+// cycle's end, the next cycle or nothing (done). The arithmetic is that of
+// block_gmres/block_cycle (same CholeskyQR passes, Householder updates and
+// stopping rule); only where it runs differs.
+struct DevSolve {
+    Op* h = nullptr;
+    BCtx c;
+    DevWs d;
+    const c2<double>* B = nullptr;
+    c2<double>* X = nullptr;
+    int s = 0, m = 0, mode = 0, maxit = 0, ncgs = 1;
+    bool use_graph = false, f32 = false;
+    double tol = 0.0, inner_tol = 0.0;
+    std::vector<double> bn, rn, target, hist;
+    int cycles = 0, j = 0, steps = 0;
+    enum { IDLE, STEP, CYCLE_END, DONE } phase = IDLE;
+    int err = OK;
+    std::string msg;
+    bool started = false;        // init() ran: the stream and workspace are the handle's
+    int64_t ld() const { return (int64_t)(m + 1) * s; }
+
+    DevSolve() {
+        c.h = nullptr;
+        c.s = nullptr;
+        c.n = 0;
+        c.ns = 0;
+        c.G = 0;
+    }
+    ~DevSolve() {
+        if (started) {
+            if (c.s) (void)hipStreamSynchronize(c.s);
+            else (void)hipDeviceSynchronize();
+        }
+        d.free_all();
+        c.ws.free_all();
+    }
+    int fail(int code) {
+        err = code;
+        msg = get_error();
+        phase = DONE;
+        return code;
+    }
+    double worst_rel() const {
+        double w = 0.0;
+        for (int q = 0; q < s; ++q) w = std::max(w, rn[q] / bn[q]);
+        return w;
+    }
+    int alloc() {
+        int st = alloc_block(c, m, mode);
+        if (st) return st;
+        auto A = [&](void** p, size_t bytes) -> int {
+            if (hipMalloc(p, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                *p = nullptr;
+                set_error("out of device memory for the block Krylov workspace");
+                return ERR_ALLOC;
+            }
+            return OK;
+        };
+        const size_t L = (size_t)ld();
+        if (!st) st = A((void**)&d.Rq, L * m * s * sizeof(c2<double>));
+        if (!st) st = A((void**)&d.Gd, L * s * sizeof(c2<double>));
+        if (!st) st = A((void**)&d.hv, (size_t)m * s * (s + 1) * sizeof(c2<double>));
+        if (!st) st = A((void**)&d.hb, (size_t)m * s * sizeof(double));
+        if (!st) st = A((void**)&d.Gm, (size_t)s * s * sizeof(c2<double>));
+        if (!st) st = A((void**)&d.Ri, (size_t)3 * s * s * sizeof(c2<double>));
+        if (!st) st = A((void**)&d.Rf, (size_t)s * s * sizeof(c2<double>));
+        if (!st) st = A((void**)&d.st, sizeof(DevStatus));
+        if (st) return st;
+        MARIE_HIP(hipMemset(d.Ri, 0, (size_t)3 * s * s * sizeof(c2<double>)));
+        MARIE_HIP(hipMemset(d.st, 0, sizeof(DevStatus)));
+        MARIE_HIP(hipHostMalloc((void**)&d.hs, sizeof(DevStatus), hipHostMallocDefault));
+        MARIE_HIP(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming));
+        return OK;
+    }
+    // CholeskyQR of the n x s block W into out (orthonormal), factor in d.Rf;
+    // *brk (device) set on breakdown
+    int cholqr_dev(const c2<double>* W, c2<double>* out, int* brk) {
+        const double shift = 11.0 * ((double)c.n * s + (double)s * (s + 1)) * 1.1102230246251565e-16 * s;
+        c2<double>* T = c.ws.T;
+        c2<double>* Ri[3] = {d.Ri, d.Ri + s * s, d.Ri + 2 * s * s};
+        int* sk = &d.st->skip3;
+        int st = vhw(c, W, s, W, s, d.Gm, s);
+        if (st) return st;
+        hipLaunchKernelGGL(k_bchol, dim3(1), dim3(BT), 0, c.s, d.Gm, s, 0, shift, Ri[0], d.Rf, sk, brk,
+                           (const int*)nullptr);
+        st = vc(c, T, W, s, Ri[0], s, s, 1.0, 0.0);
+        if (!st) st = vhw(c, T, s, T, s, d.Gm, s);
+        if (st) return st;
+        hipLaunchKernelGGL(k_bchol, dim3(1), dim3(BT), 0, c.s, d.Gm, s, 1, shift, Ri[1], d.Rf, sk, brk,
+                           (const int*)nullptr);
+        st = vc(c, out, T, s, Ri[1], s, s, 1.0, 0.0);
+        // third pass only after a shifted first factor
+        if (!st) st = vhw(c, out, s, out, s, d.Gm, s, sk);
+        if (st) return st;
+        hipLaunchKernelGGL(k_bchol, dim3(1), dim3(BT), 0, c.s, d.Gm, s, 2, shift, Ri[2], d.Rf, sk, brk,
+                           (const int*)sk);
+        st = vc(c, T, out, s, Ri[2], s, s, 1.0, 0.0, sk);
+        if (st) return st;
+        hipLaunchKernelGGL(k_copy_skip, dim3(grid_of(c.n * s)), dim3(BT), 0, c.s, out, T, c.n * s, (const int*)sk);
+        MARIE_HIP(hipGetLastError());
+        return OK;
+    }
+    // block step jj: W = A V_jj, Gram-Schmidt, CholeskyQR into V_jj+1, LS update
+    int step_body(int jj) {
+        const int64_t n = c.n;
+        const int P = (jj + 1) * s;
+        int st = block_matvec(c, c.ws.V + (int64_t)jj * s * n, c.ws.W, f32, -1, false);
+        if (st) return st;
+        for (int pass = 0; pass < ncgs; ++pass) {
+            st = vhw(c, c.ws.V, P, c.ws.W, s, c.ws.Hd, P);
+            if (!st) st = vc(c, c.ws.W, c.ws.V, P, c.ws.Hd, P, s, -1.0, 1.0);
+            if (st) return st;
+            const int64_t rows = pass > 0 ? P : ld();
+            const int g = (int)std::min<int64_t>(BG, (rows * s + BT - 1) / BT);
+            hipLaunchKernelGGL(k_hset, dim3(g), dim3(BT), 0, c.s, c.ws.Hd, P, s, d.Rq, ld(), jj * s, pass > 0 ? 1 : 0);
+        }
+        st = cholqr_dev(c.ws.W, c.ws.V + (int64_t)(jj + 1) * s * n, &d.st->brk);
+        if (st) return st;
+        hipLaunchKernelGGL(k_rset, dim3(1), dim3(DS * DS), 0, c.s, d.Rf, s, d.Rq, ld(), jj * s, P);
+        hipLaunchKernelGGL(k_bls, dim3(1), dim3(DLS), 0, c.s, d.Rq, ld(), d.Gd, d.hv, d.hb, s, jj, d.st->res);
+        MARIE_HIP(hipGetLastError());
+        return OK;
+    }
+    int enqueue_status() {
+        MARIE_HIP(hipMemcpyAsync(d.hs, d.st, sizeof(DevStatus), hipMemcpyDeviceToHost, c.s));
+        MARIE_HIP(hipEventRecord(d.ev, c.s));
+        return OK;
+    }
+    int enqueue_step(int jj) {
+        j = jj;
+        phase = STEP;
+        if (!use_graph) {
+            const int st = step_body(jj);
+            return st ? st : enqueue_status();
+        }
+        if ((int)d.graphs.size() <= jj) {
+            d.graphs.resize(jj + 1, nullptr);
+            d.gsrc.resize(jj + 1, nullptr);
+        }
+        if (!d.graphs[jj]) {
+            MARIE_HIP(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+            const int st = step_body(jj);
+            hipGraph_t gr = nullptr;
+            const hipError_t e = hipStreamEndCapture(c.s, &gr);
+            if (st) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return st;
+            }
+            if (e != hipSuccess) {
+                set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                return ERR_HIP;
+            }
+            d.gsrc[jj] = gr;
+            MARIE_HIP(hipGraphInstantiate(&d.graphs[jj], gr, nullptr, nullptr, 0));
+        }
+        MARIE_HIP(hipGraphLaunch(d.graphs[jj], c.s));
+        return enqueue_status();
+    }
+    int start_cycle() {
+        ++cycles;
+        target.assign(s, 0.0);
+        for (int q = 0; q < s; ++q) {
+            double t = (mode == 1 ? 0.9 : 1.0) * tol * bn[q];
+            if (mode == 1) t = std::max(t, inner_tol * rn[q]);
+            target[q] = std::min(t, 0.5 * rn[q]);
+        }
+        int st = cholqr_dev(c.ws.R, c.ws.V, &d.st->brk0);
+        if (st) return st;
+        hipLaunchKernelGGL(k_ginit, dim3((unsigned)std::min<int64_t>(BG, (ld() * s + BT - 1) / BT)), dim3(BT), 0, c.s,
+                           d.Rf, s, d.Gd, ld());
+        MARIE_HIP(hipGetLastError());
+        return enqueue_step(0);
+    }
+    int enqueue_cycle_end(int J) {
+        steps = J;
+        phase = CYCLE_END;
+        const int K = J * s;
+        const int64_t ns = c.n * s;
+        const int g = grid_of(ns);
+        hipLaunchKernelGGL(k_bsolve, dim3(s), dim3(BT), (size_t)2 * K * sizeof(double2), c.s, d.Rq, ld(), d.Gd, K,
+                           c.ws.Sd);
+        int st = vc(c, c.ws.D, c.ws.V, K, c.ws.Sd, K, s, 1.0, 0.0);
+        if (st) return st;
+        hipLaunchKernelGGL(k_add, dim3(g), dim3(BT), 0, c.s, X, c.ws.D, ns);
+        st = apply(h, OP_AN, 0, PREC_F64, X, c.ws.T, s, c.s);
+        if (st) return st;
+        hipLaunchKernelGGL(k_sub, dim3(g), dim3(BT), 0, c.s, c.ws.R, B, c.ws.T, ns);
+        hipLaunchKernelGGL(k_colnorm2, dim3(c.G, s), dim3(BT), 0, c.s, c.ws.R, c.n, c.ws.part, c.G);
+        hipLaunchKernelGGL(k_colsum, dim3(s), dim3(BT), 0, c.s, c.ws.part, c.G, d.st->rn2);
+        MARIE_HIP(hipGetLastError());
+        return enqueue_status();
+    }
+    // set-up: the initial residual (synchronous), then the first cycle
+    int init() {
+        started = true;
+        c.h = h;
+        c.s = h->stream;
+        c.n = h->nidx;
+        c.ns = s;
+        c.G = grid_of(c.n);
+        f32 = mode == 1;
+        if (!c.s) use_graph = false;
+        int st = alloc();
+        if (!st) st = ensure_work(h, s);
+        if (!st) st = colnorms(c, B, bn);
+        if (st) return fail(st);
+        for (int q = 0; q < s; ++q)
+            if (bn[q] == 0.0) {
+                set_error("a right-hand side column is zero");
+                return fail(ERR_ARG);
+            }
+        const int64_t ns = c.n * s;
+        st = apply(h, OP_AN, 0, PREC_F64, X, c.ws.T, s, c.s);
+        if (st) return fail(st);
+        hipLaunchKernelGGL(k_sub, dim3(grid_of(ns)), dim3(BT), 0, c.s, c.ws.R, B, c.ws.T, ns);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("kernel launch failed");
+            return fail(ERR_HIP);
+        }
+        st = colnorms(c, c.ws.R, rn);
+        if (st) return fail(st);
+        hist.assign(1, worst_rel());
+        if (worst_rel() <= tol || maxit < 1) {
+            phase = DONE;
+            return OK;
+        }
+        st = start_cycle();
+        return st ? fail(st) : OK;
+    }
+    // the event of the enqueued work has completed
+    int advance() {
+        int st = OK;
+        const DevStatus& hs = *d.hs;
+        if (phase == STEP) {
+            if (hs.brk0) {
+                set_error("the block of residuals is rank deficient (dependent right-hand sides)");
+                return fail(ERR_ARG);
+            }
+            double worst = 0.0;
+            bool done = true;
+            for (int q = 0; q < s; ++q) {
+                worst = std::max(worst, hs.res[q] / bn[q]);
+                if (hs.res[q] > target[q]) done = false;
+            }
+            hist.push_back(worst);
+            const int J = j + 1;
+            if (done || hs.brk || J == m) st = enqueue_cycle_end(J);
+            else st = enqueue_step(J);
+        } else if (phase == CYCLE_END) {
+            for (int q = 0; q < s; ++q) rn[q] = std::sqrt(hs.rn2[q]);
+            hist.push_back(worst_rel());
+            if (worst_rel() <= tol || cycles >= maxit) phase = DONE;
+            else st = start_cycle();
+        }
+        return st ? fail(st) : OK;
+    }
+    void results(int* flags, double* relres, int* iter) const {
+        for (int q = 0; q < s; ++q) {
+            relres[q] = rn[q] / bn[q];
+            flags[q] = relres[q] <= tol ? 0 : 1;
+        }
+        iter[0] = steps;
+        iter[1] = cycles;
+    }
+};
+
+// Drive the solves to the end from one host thread: whichever solve's event
+// has completed advances and enqueues its next piece of work on its stream.
+void drive(std::vector<DevSolve*>& ps) {
+    for (DevSolve* p : ps)
+        if (p->phase != DevSolve::DONE) p->init();
+    for (;;) {
+        int active = 0;
+        bool moved = false;
+        for (DevSolve* p : ps) {
+            if (p->phase == DevSolve::DONE) continue;
+            ++active;
+            const hipError_t q = ps.size() == 1 ? hipEventSynchronize(p->d.ev) : hipEventQuery(p->d.ev);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) {
+                set_error(std::string("block step: ") + hipGetErrorString(q));
+                p->fail(ERR_HIP);
+                continue;
+            }
+            p->advance();
+            moved = true;
+        }
+        if (!active) break;
+        if (!moved) std::this_thread::yield();
+    }
+}
+
+void set_dev_solve(DevSolve& p, Op* h, const void* B, void* X, int nrhs, int restart, double tol, int maxit,
+                   int mode, double inner_tol, int options) {
+    p.h = h;
+    p.B = reinterpret_cast<const c2<double>*>(B);
+    p.X = reinterpret_cast<c2<double>*>(X);
+    p.s = nrhs;
+    p.m = restart;
+    p.mode = mode;
+    p.maxit = maxit;
+    p.tol = tol;
+    p.inner_tol = inner_tol;
+    p.ncgs = (options & MARIE_BLOCK_CGS2) ? 2 : 1;
+    p.use_graph = (options & MARIE_BLOCK_GRAPH) != 0;
+}
+
 }  // namespace
 
 int block_gmres(Op* h, const c2<double>* B, c2<double>* X, int s, int m, double tol, int maxit, int mode,
@@ -791,6 +1562,10 @@ extern "C" int marie_vie_block_gmres_device(marie_vie_op* hh, const void* B, voi
         set_error("invalid block gmres arguments (nrhs 1..64, (restart+1)*nrhs <= 2048, mode 0 or 1)");
         return MARIE_ERR_ARG;
     }
+    if ((options & MARIE_BLOCK_DEVICE) && nrhs > DS) {
+        set_error("device-driven block steps take at most 32 right-hand sides");
+        return MARIE_ERR_ARG;
+    }
     if (!(h->precisions & PREC_F64) || (mode == 1 && !(h->precisions & PREC_F32))) {
         set_error("block gmres needs an FP64 operator (and FP32 for mode 1)");
         return MARIE_ERR_STATE;
@@ -806,9 +1581,22 @@ extern "C" int marie_vie_block_gmres_device(marie_vie_op* hh, const void* B, voi
         return MARIE_OK;
     }
     std::vector<double> hist;
-    const int st = block_gmres(h, (const c2<double>*)B, (c2<double>*)X, nrhs, restart, tol, maxit, mode, inner_tol,
-                               options, flag, relres, iter, hist);
-    if (st) return st;
+    if (options & MARIE_BLOCK_DEVICE) {
+        DevSolve p;
+        set_dev_solve(p, h, B, X, nrhs, restart, tol, maxit, mode, inner_tol, options);
+        std::vector<DevSolve*> ps(1, &p);
+        drive(ps);
+        if (p.err) {
+            set_error(p.msg);
+            return p.err;
+        }
+        p.results(flag, relres, iter);
+        hist = p.hist;
+    } else {
+        const int st = block_gmres(h, (const c2<double>*)B, (c2<double>*)X, nrhs, restart, tol, maxit, mode,
+                                   inner_tol, options, flag, relres, iter, hist);
+        if (st) return st;
+    }
     if (resvec && resvec_cap > 0)
         for (int i = 0; i < (int)hist.size() && i < resvec_cap; ++i) resvec[i] = hist[i];
     if (nres) *nres = (int)hist.size();
@@ -871,6 +1659,55 @@ extern "C" int marie_vie_block_gmres_batch(marie_vie_op* const* ops, int nitems,
                 return MARIE_ERR_ARG;
             }
         off[i + 1] = off[i] + nrhs[i];
+    }
+    if (options & MARIE_BLOCK_DEVICE) {
+        // one host thread, every problem's steps on its own stream
+        for (int i = 0; i < nitems; ++i) {
+            const Op* hi = reinterpret_cast<const Op*>(ops[i]);
+            if (!B[i] || !X[i] || nrhs[i] > DS || restart < 1 || (int64_t)(restart + 1) * nrhs[i] > 2048 ||
+                maxit < 1 || !(tol > 0) || (mode != 0 && mode != 1)) {
+                set_error("batch problem " + std::to_string(i) +
+                          ": invalid arguments (nrhs <= 32 with device-driven steps, (restart+1)*nrhs <= 2048, "
+                          "mode 0 or 1)");
+                return MARIE_ERR_ARG;
+            }
+            if (!(hi->precisions & PREC_F64) || (mode == 1 && !(hi->precisions & PREC_F32)) || !hi->have_fN ||
+                !hi->have_mat || !hi->have_idx) {
+                set_error("batch problem " + std::to_string(i) +
+                          ": operator, index or material not set, or a precision missing");
+                return MARIE_ERR_STATE;
+            }
+        }
+        std::vector<DevSolve> ps(nitems);
+        std::vector<DevSolve*> pp(nitems);
+        for (int i = 0; i < nitems; ++i) {
+            Op* hi = reinterpret_cast<Op*>(ops[i]);
+            MARIE_HIP(hipSetDevice(hi->device));
+            set_dev_solve(ps[i], hi, B[i], X[i], nrhs[i], restart, tol, maxit, mode, inner_tol, options);
+            if (hi->nidx == 0) {
+                // no DOFs: converged with zero iterations, as the single-problem entry
+                ps[i].phase = DevSolve::DONE;
+                ps[i].bn.assign(nrhs[i], 1.0);
+                ps[i].rn.assign(nrhs[i], 0.0);
+            }
+            pp[i] = &ps[i];
+        }
+        drive(pp);
+        int first = MARIE_OK;
+        std::string fmsg;
+        for (int i = 0; i < nitems; ++i) {
+            status[i] = ps[i].err;
+            if (ps[i].err) {
+                if (first == MARIE_OK) {
+                    first = ps[i].err;
+                    fmsg = "batch problem " + std::to_string(i) + ": " + ps[i].msg;
+                }
+                continue;
+            }
+            ps[i].results(flag + off[i], relres + off[i], iter + 2 * i);
+        }
+        if (first != MARIE_OK) set_error(fmsg);
+        return first;
     }
     // largest problems first, so the small ones fill in around them
     std::vector<int> order(nitems);

@@ -171,22 +171,25 @@ def dgf_fields(seg_center, seg_J, port_start, points, ko, omega_mu, device=0):
 BLOCK_GRAPH = 1
 BLOCK_CGS2 = 2
 BLOCK_COLUMNS = 4
+BLOCK_DEVICE = 8
 
 
-def _block_options(use_graph, cgs_passes, columns=False):
+def _block_options(use_graph, cgs_passes, columns=False, device_steps=False):
     if cgs_passes not in (1, 2):
         raise ValueError('cgs_passes must be 1 or 2')
     return ((BLOCK_GRAPH if use_graph else 0) | (BLOCK_CGS2 if cgs_passes == 2 else 0) |
-            (BLOCK_COLUMNS if columns else 0))
+            (BLOCK_COLUMNS if columns else 0) | (BLOCK_DEVICE if device_steps else 0))
 
 
 def block_gmres_batch(ops, B_ptrs, X_ptrs, nrhs, restart=20, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4,
-                      use_graph=False, cgs_passes=1, columns=False):
+                      use_graph=False, cgs_passes=1, columns=False, device_steps=False):
     """Block GMRES of several problems at once (one handle each, device
     buffers B_ptrs[i], X_ptrs[i] of nrhs[i] stacked columns); columns=True
     solves each problem's columns one at a time with GMRES in `mode` (0-3)
-    instead. Returns (flags per problem, relres per problem, [steps, cycles]
-    per problem)."""
+    instead; device_steps=True runs every block step on the device and drives
+    all problems from one host thread (with use_graph: each step a HIP graph).
+    Returns (flags per problem, relres per problem, [steps, cycles] per
+    problem)."""
     n = len(ops)
     tot = int(sum(nrhs))
     P = ctypes.c_void_p
@@ -199,8 +202,8 @@ def block_gmres_batch(ops, B_ptrs, X_ptrs, nrhs, restart=20, tol=1e-6, maxit=200
     it = (ctypes.c_int * (2 * n))()
     status = (ctypes.c_int * n)()
     check(load().marie_vie_block_gmres_batch(opa, n, ba, xa, na, restart, tol, maxit, mode, inner_tol,
-                                             _block_options(use_graph, cgs_passes, columns), flag, relres, it,
-                                             status))
+                                             _block_options(use_graph, cgs_passes, columns, device_steps), flag,
+                                             relres, it, status))
     fl, rr, its, o = [], [], [], 0
     for i in range(n):
         fl.append(np.array(flag[o:o + nrhs[i]]))
@@ -398,9 +401,11 @@ class Operator:
                                                         ctypes.c_void_p(grad_sigma_ptr)))
 
     def block_gmres_device(self, B_ptr, X_ptr, nrhs, restart=20, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4,
-                           use_graph=False, cgs_passes=1, resvec_cap=20000, columns=False):
+                           use_graph=False, cgs_passes=1, resvec_cap=20000, columns=False, device_steps=False):
         """Block GMRES on device buffers (columns=True: the columns one at a
-        time with GMRES in `mode` 0-3); returns (flags, relres, [steps, cycles], resvec)."""
+        time with GMRES in `mode` 0-3; device_steps=True: every block step on
+        the device, a HIP graph per step with use_graph); returns (flags,
+        relres, [steps, cycles], resvec)."""
         flag = (ctypes.c_int * nrhs)()
         relres = (ctypes.c_double * nrhs)()
         it = (ctypes.c_int * 2)()
@@ -408,14 +413,14 @@ class Operator:
         nres = ctypes.c_int()
         check(load().marie_vie_block_gmres_device(self._h, ctypes.c_void_p(B_ptr), ctypes.c_void_p(X_ptr), nrhs,
                                                   restart, tol, maxit, mode, inner_tol,
-                                                  _block_options(use_graph, cgs_passes, columns), flag,
-                                                  relres, it, resvec.ctypes.data_as(_DP), resvec_cap,
+                                                  _block_options(use_graph, cgs_passes, columns, device_steps),
+                                                  flag, relres, it, resvec.ctypes.data_as(_DP), resvec_cap,
                                                   ctypes.byref(nres)))
         return (np.array(flag[:]), np.array(relres[:]), [it[0], it[1]],
                 resvec[:min(nres.value, resvec_cap)].copy())
 
     def block_gmres(self, B, X0=None, restart=20, tol=1e-6, maxit=200, mode=0, inner_tol=1e-4, use_graph=False,
-                    cgs_passes=1, resvec_cap=20000):
+                    cgs_passes=1, resvec_cap=20000, device_steps=False):
         """Host-buffer block GMRES; B is nidx x nrhs. Returns (X, flags, relres, [steps, cycles], resvec)."""
         B = np.asarray(B, np.complex128)
         if B.ndim == 1:
@@ -434,7 +439,8 @@ class Operator:
         resvec = np.zeros(resvec_cap)
         nres = ctypes.c_int()
         check(load().marie_vie_block_gmres(self._h, _ptr(Bc), _ptr(Xc), nrhs, restart, tol, maxit, mode, inner_tol,
-                                           _block_options(use_graph, cgs_passes), flag, relres, it,
+                                           _block_options(use_graph, cgs_passes, False, device_steps), flag,
+                                           relres, it,
                                            resvec.ctypes.data_as(_DP),
                                            resvec_cap, ctypes.byref(nres)))
         return (Xc.T.copy(), np.array(flag[:]), np.array(relres[:]), [it[0], it[1]],
