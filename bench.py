@@ -460,8 +460,11 @@ def main():
 
     if not args.no_solve and not args.no_block:
         # 16 excitations of the same body (plane waves from seeded directions and
-        # polarisations): one block GMRES vs 16 single solves, all to 1e-6
+        # polarisations): one block GMRES vs 16 single solves, all to 1e-6; on a
+        # stream of its own (HIP graphs cannot be captured on the null stream)
         s_blk = 16
+        bstream = torch.cuda.Stream(device=dev)
+        op.set_stream(bstream.cuda_stream)
         rng = np.random.default_rng(7)
         cols = []
         for q in range(s_blk):
@@ -475,6 +478,7 @@ def main():
         torch.cuda.synchronize()
         t = time.perf_counter()
         Xseq = torch.zeros_like(Bd)
+        torch.cuda.synchronize()      # the solves run on non-blocking streams
         for q in range(s_blk):
             op.gmres_device(Bd[q].data_ptr(), Xseq[q].data_ptr(), restart=50, tol=1e-6, maxit=200)
         torch.cuda.synchronize()
@@ -494,9 +498,24 @@ def main():
                         'flags_max': int(fl.max()), 'max_rel_diff_vs_single_solves': diff}
             log('block gmres mode %d: %.3f s for %d excitations (sequential FP64 %.3f s), cycles %d, max relres %.2e'
                 % (mode, tb, s_blk, t_seq, it[1], rr.max()))
+            # the same solve with every block step on the device, one HIP graph
+            # per step index replayed in every cycle (MARIE_BLOCK_DEVICE)
+            X = torch.zeros_like(Bd)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fl, rr, it, rv = op.block_gmres_device(Bd.data_ptr(), X.data_ptr(), s_blk, restart=40, tol=1e-6,
+                                                   maxit=100, mode=mode, inner_tol=1e-7, use_graph=True,
+                                                   device_steps=True)
+            torch.cuda.synchronize()
+            td = time.perf_counter() - t
+            blk[key]['device_driven_graph_steps'] = {
+                's': td, 'block_steps_last_cycle_and_cycles': it, 'max_relres': float(rr.max()),
+                'flags_max': int(fl.max())}
+            log('  device-driven graph steps: %.3f s, cycles %d, max relres %.2e' % (td, it[1], rr.max()))
         # the same 16 columns one at a time with the refined solver (GMRES mode
         # 3, cycles of 128): fewer bytes per product than any block variant
         X = torch.zeros_like(Bd)
+        torch.cuda.synchronize()      # the solves run on non-blocking streams
         op.block_gmres_device(Bd[:1].data_ptr(), X[:1].data_ptr(), 1, restart=128, tol=1e-6, maxit=200, mode=3,
                               inner_tol=1e-7, columns=True)       # warm: workspace allocation
         X = torch.zeros_like(Bd)
@@ -513,6 +532,8 @@ def main():
         log('16 columns one at a time, refined mode 3: %.3f s (%.2fx the sequential FP64 solves), max relres %.2e'
             % (tc, t_seq / tc, rr.max()))
         out['batching_16_excitations'] = blk
+        torch.cuda.synchronize()
+        op.set_stream(stream.cuda_stream)
 
     if not args.no_solve and not args.no_batch and n1 == 128:
         # several uneven bodies and frequencies: 64^3, 80^3, 96^3, 128^3 phantoms
@@ -539,21 +560,24 @@ def main():
                 cols = [fac * Ep.ravel(order='F')[idxb - 1] for Ep in Es]
                 assert np.allclose(cols[0], v0, rtol=1e-12, atol=0)
                 opb = lib.Operator(nb, nb, nb, precisions=lib.PREC_F64 | lib.PREC_F32)
+                stb = torch.cuda.Stream(device=dev)       # each problem on its own stream
+                opb.set_stream(stb.cuda_stream)
                 opb.assemble_N(dxb, emb['ko'])
                 opb.set_material(Mrb, Mcb, Gb)
                 opb.set_index(idxb, one_based=True)
                 opb.autotune(lib.PREC_F64, iters=3)
                 opb.autotune(lib.PREC_F32, iters=3)
                 Bb = torch.from_numpy(np.ascontiguousarray(np.stack(cols))).to(dev)
-                probs.append((nb, fb, opb, Bb))
+                probs.append((nb, fb, opb, Bb, stb))
         torch.cuda.synchronize()
         t_setup = time.perf_counter() - t
         kw = dict(restart=20, tol=1e-6, maxit=100, mode=0, inner_tol=1e-7, use_graph=True)
         seq = []
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for nb, fb, opb, Bb in probs:
+        for nb, fb, opb, Bb, _ in probs:
             Xb = torch.zeros_like(Bb)
+            torch.cuda.synchronize()      # the solves run on non-blocking streams
             t1 = time.perf_counter()
             fl, rr, it, rv = opb.block_gmres_device(Bb.data_ptr(), Xb.data_ptr(), 16, **kw)
             torch.cuda.synchronize()
@@ -567,11 +591,27 @@ def main():
                                               [x.data_ptr() for x in Xs], [16] * len(probs), **kw)
         torch.cuda.synchronize()
         t_bat = time.perf_counter() - t
+        # one host thread driving every problem's device-driven block steps
+        # (one HIP graph per step index, FP64 and complex64 products)
+        dev_batch = {}
+        for mode in (0, 1):
+            Xd = [torch.zeros_like(p[3]) for p in probs]
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fld, rrd, itd = lib.block_gmres_batch([p[2] for p in probs], [p[3].data_ptr() for p in probs],
+                                                  [x.data_ptr() for x in Xd], [16] * len(probs),
+                                                  **dict(kw, mode=mode, device_steps=True))
+            torch.cuda.synchronize()
+            dev_batch['fp64' if mode == 0 else 'fp32_products'] = {
+                's': time.perf_counter() - t, 'max_relres': float(max(r.max() for r in rrd)),
+                'flags_max': int(max(f.max() for f in fld)), 'steps_and_cycles': itd}
+            del Xd
         # the reference's loop: every port of every problem one FP64 GMRES(50) after another
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for nb, fb, opb, Bb in probs:
+        for nb, fb, opb, Bb, _ in probs:
             Xb = torch.zeros_like(Bb)
+            torch.cuda.synchronize()      # the solves run on non-blocking streams
             for q in range(Bb.shape[0]):
                 opb.gmres_device(Bb[q].data_ptr(), Xb[q].data_ptr(), restart=50, tol=1e-6, maxit=200)
         torch.cuda.synchronize()
@@ -581,8 +621,9 @@ def main():
         kwc = dict(restart=128, tol=1e-6, maxit=200, mode=3, inner_tol=1e-7, columns=True)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for nb, fb, opb, Bb in probs:
+        for nb, fb, opb, Bb, _ in probs:
             Xb = torch.zeros_like(Bb)
+            torch.cuda.synchronize()      # the solves run on non-blocking streams
             opb.block_gmres_device(Bb.data_ptr(), Xb.data_ptr(), 16, **kwc)
         torch.cuda.synchronize()
         t_seq3 = time.perf_counter() - t
@@ -603,11 +644,17 @@ def main():
                                            'speedup_vs_per_port_fp64': t_ports / t_bat3,
                                            'max_relres': float(max(r.max() for r in rr3)),
                                            'flags_max': int(max(f.max() for f in fl3))},
-            'speedup_block_batch_vs_per_port_fp64': t_ports / t_bat}
+            'speedup_block_batch_vs_per_port_fp64': t_ports / t_bat,
+            'device_driven_graph_batch': dict(dev_batch, method=(
+                'MARIE_BLOCK_DEVICE | MARIE_BLOCK_GRAPH: every block step (product, block Gram-Schmidt, CholeskyQR, '
+                'Householder least-squares update) on the device, one HIP graph per step index, all problems '
+                'driven from one host thread by their status events'))}
         log('batch of %d problems x 16 ports: block GMRES one after the other %.3f s, concurrent %.3f s (max relres '
             '%.2e; setup %.1f s); per-port FP64 GMRES %.3f s; refined mode-3 columns one after the other %.3f s, '
             'concurrent %.3f s (max relres %.2e)' % (len(probs), t_seq, t_bat, max(r.max() for r in rrs), t_setup,
                                                      t_ports, t_seq3, t_bat3, max(r.max() for r in rr3)))
+        log('  device-driven graph batch: FP64 %.3f s, complex64 products %.3f s'
+            % (dev_batch['fp64']['s'], dev_batch['fp32_products']['s']))
         for p in probs:
             p[2].close()
         del probs, Xs

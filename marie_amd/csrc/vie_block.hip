@@ -867,84 +867,107 @@ __global__ __launch_bounds__(BT) void k_bchol(const c2<double>* __restrict__ Gm,
 // beta = 2 / v^H v, the identity when x is already zero below row c) and is
 // applied to the block's later columns and to the right-hand side G (ld x s).
 // res[q] = || G(rows (j+1)s .. (j+2)s-1, column q) ||.
-// Earlier reflectors: one wave per new column holds rows p .. p+s of its
-// column in lanes 0 .. s while p runs over the reflectors (staged in LDS 64 at
-// a time); row p is stored when it leaves the window and row p+s+1 comes from
-// a register chunk loaded one chunk ahead. The block's own reflectors work on
-// the 2s x s window of rows js .. js+2s-1 in LDS.
+// The s reflectors of block b act on rows bs .. bs+2s-1 only; their product
+// H_0 .. H_{s-1} = I - V T V^H (V: 2s x s, column i holding reflector bs+i at
+// rows i .. i+s; T upper triangular, T_ii = beta_i, T(0:i, i) = -beta_i
+// T(0:i, 0:i) V(:, 0:i)^H v_i) is kept per block, so the earlier blocks reach
+// the new columns as one 2s-row window update each, W -= V (T^H (V^H W)),
+// three small products in LDS; rows bs .. bs+s-1 are then final and the
+// window moves down by s rows. Block j's own reflectors work on the window
+// rows js .. js+2s-1 one after the other, and block j's T is formed beside
+// them.
 __global__ __launch_bounds__(DLS) void k_bls(c2<double>* __restrict__ Rq, int64_t ld, c2<double>* __restrict__ G,
-                                             c2<double>* __restrict__ hv, double* __restrict__ hb, int s, int j,
-                                             double* __restrict__ res) {
+                                             c2<double>* __restrict__ hv, double* __restrict__ hb,
+                                             c2<double>* __restrict__ Tb, int s, int j, double* __restrict__ res) {
     constexpr int NW = DLS / 64;
-    static_assert(2 * NW >= DS, "at most two new columns per wave");
-    __shared__ double2 vs[64][DS + 1];
-    __shared__ double bs[64];
-    __shared__ double2 Wn[2 * DS][DS + 1];
-    __shared__ double2 Gw[2 * DS][DS + 1];
+    __shared__ double2 Wn[2 * DS][DS + 1];   // the window of the new columns
+    __shared__ double2 Gw[2 * DS][DS + 1];   // G's window (phase B); V^H W (phase A)
+    __shared__ double2 Vs[2 * DS][DS + 1];   // a block's reflectors as V
+    __shared__ double2 Ts[DS][DS + 1];       // a block's T
+    __shared__ double2 C2[DS][DS + 1];       // T^H V^H W
     __shared__ double2 vcur[DS + 1];
     __shared__ double bcur;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int js = j * s, L = s + 1;
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+    const int js = j * s, L = s + 1, s2 = 2 * s;
     const double2 z = {0.0, 0.0};
-    // ---- earlier reflectors p < js on the new columns
-    double2 x[2], inc[2], nxt[2];
-    int64_t cb[2];
-    bool on[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int q = wave + u * NW;
-        on[u] = q < s;
-        cb[u] = (int64_t)(js + (on[u] ? q : 0)) * ld;
-        x[u] = (on[u] && lane < L) ? ld2d(Rq + cb[u] + lane) : z;
-        // rows entering the window during the first chunk: s+1 .. s+64
-        const int64_t r0 = s + 1 + lane;
-        inc[u] = (on[u] && r0 < ld) ? ld2d(Rq + cb[u] + r0) : z;
+    auto rq = [&](int row, int q) -> c2<double>* { return Rq + (int64_t)(js + q) * ld + row; };
+    // the first window: rows 0 .. 2s-1 of the new columns
+    for (int e = t; e < s2 * s; e += DLS) {
+        const int r = e % s2, q = e / s2;
+        Wn[r][q] = ld2d(rq(r, q));
     }
-    for (int p0 = 0; p0 < js; p0 += 64) {
-        const int cnt = js - p0 < 64 ? js - p0 : 64;
+    // ---- earlier blocks b < j
+    for (int b = 0; b < j; ++b) {
+        const int bs = b * s;
         __syncthreads();
-        for (int e = threadIdx.x; e < cnt * L; e += DLS) {
-            const int i = e / L, k = e % L;
-            vs[i][k] = ld2d(hv + (int64_t)(p0 + i) * L + k);
+        for (int e = t; e < s2 * s; e += DLS) {
+            const int r = e % s2, i = e / s2;
+            const int k = r - i;
+            Vs[r][i] = (k >= 0 && k <= s) ? ld2d(hv + (int64_t)(bs + i) * L + k) : z;
         }
-        for (int i = threadIdx.x; i < cnt; i += DLS) bs[i] = hb[p0 + i];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int64_t r1 = (int64_t)p0 + 64 + s + 1 + lane;      // the next chunk's entering rows
-            nxt[u] = (on[u] && r1 < ld) ? ld2d(Rq + cb[u] + r1) : z;
+        for (int e = t; e < s * s; e += DLS) {
+            const int k = e % s, i = e / s;
+            Ts[k][i] = ld2d(Tb + ((int64_t)b * s + i) * s + k);          // T(k, i), column-major per block
         }
         __syncthreads();
-        for (int pl = 0; pl < cnt; ++pl) {
-            const double2 v = lane < L ? vs[pl][lane] : z;
-            const double beta = bs[pl];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                if (!on[u]) continue;
-                const double2 pr = dcmul(v, x[u]);
-                const double dx = wave_sum(pr.x) * beta, dy = wave_sum(pr.y) * beta;
-                const double2 dv = dmul(v, double2{dx, dy});
-                x[u].x -= dv.x;
-                x[u].y -= dv.y;
-                if (lane == 0) Rq[cb[u] + p0 + pl] = c2<double>{x[u].x, x[u].y};
-                const double2 dn = {__shfl_down(x[u].x, 1, 64), __shfl_down(x[u].y, 1, 64)};
-                const double2 in = {__shfl(inc[u].x, pl, 64), __shfl(inc[u].y, pl, 64)};
-                x[u] = lane == s ? in : (lane < s ? dn : z);
+        // V^H W
+        for (int e = t; e < s * s; e += DLS) {
+            const int i = e % s, q = e / s;
+            double2 a = z;
+            for (int r = i; r <= i + s; ++r) {
+                const double2 p = dcmul(Vs[r][i], Wn[r][q]);
+                a.x += p.x;
+                a.y += p.y;
             }
+            Gw[i][q] = a;
         }
-#pragma unroll
-        for (int u = 0; u < 2; ++u) inc[u] = nxt[u];
+        __syncthreads();
+        // T^H (V^H W)
+        for (int e = t; e < s * s; e += DLS) {
+            const int i = e % s, q = e / s;
+            double2 a = z;
+            for (int k = 0; k <= i; ++k) {
+                const double2 p = dcmul(Ts[k][i], Gw[k][q]);
+                a.x += p.x;
+                a.y += p.y;
+            }
+            C2[i][q] = a;
+        }
+        __syncthreads();
+        // W -= V C2
+        for (int e = t; e < s2 * s; e += DLS) {
+            const int r = e % s2, q = e / s2;
+            double2 a = Wn[r][q];
+            const int i0 = r - s > 0 ? r - s : 0, i1 = r < s - 1 ? r : s - 1;
+            for (int i = i0; i <= i1; ++i) {
+                const double2 p = dmul(Vs[r][i], C2[i][q]);
+                a.x -= p.x;
+                a.y -= p.y;
+            }
+            Wn[r][q] = a;
+        }
+        __syncthreads();
+        // rows bs .. bs+s-1 are final; the window moves down by s rows
+        for (int e = t; e < s * s; e += DLS) {
+            const int r = e % s, q = e / s;
+            const double2 v = Wn[r][q];
+            *rq(bs + r, q) = c2<double>{v.x, v.y};
+        }
+        __syncthreads();
+        for (int e = t; e < s * s; e += DLS) {
+            const int r = e % s, q = e / s;
+            Wn[r][q] = Wn[s + r][q];
+            Wn[s + r][q] = ld2d(rq(bs + s2 + r, q));
+        }
     }
     // ---- the block's own reflectors on rows js .. js+2s-1
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int q = wave + u * NW;
-        if (on[u] && lane < L) Wn[lane][q] = x[u];
-    }
-    for (int e = threadIdx.x; e < 2 * s * s; e += DLS) {
-        const int r = e % (2 * s), q = e / (2 * s);
-        if (r > s) Wn[r][q] = ld2d(Rq + (int64_t)(js + q) * ld + js + r);
+    for (int e = t; e < s2 * s; e += DLS) {
+        const int r = e % s2, q = e / s2;
         Gw[r][q] = ld2d(G + (int64_t)q * ld + js + r);
+        Vs[r][q] = z;
     }
+    for (int e = t; e < s * s; e += DLS) Ts[e % s][e / s] = z;
     __syncthreads();
     for (int q = 0; q < s; ++q) {
         const int c = js + q;
@@ -972,11 +995,13 @@ __global__ __launch_bounds__(DLS) void k_bls(c2<double>* __restrict__ Rq, int64_
             }
             if (lane < L) {
                 vcur[lane] = v;
+                Vs[q + lane][q] = v;
                 hv[(int64_t)c * L + lane] = c2<double>{v.x, v.y};
             }
             if (lane == 0) {
                 bcur = beta;
                 hb[c] = beta;
+                Ts[q][q] = double2{beta, 0.0};
             }
         }
         __syncthreads();
@@ -997,19 +1022,42 @@ __global__ __launch_bounds__(DLS) void k_bls(c2<double>* __restrict__ Rq, int64_
                     else Gw[q + lane][col] = o;
                 }
             }
+            // column q of T: C2[i][0] = v_i^H v_q (i < q), then T(0:q, q)
+            if (t < q) {
+                double2 a = z;
+                for (int r = q; r <= t + s; ++r) {
+                    const double2 p = dcmul(Vs[r][t], Vs[r][q]);
+                    a.x += p.x;
+                    a.y += p.y;
+                }
+                C2[t][0] = a;
+            }
+        }
+        __syncthreads();
+        if (beta != 0.0 && t < q) {
+            double2 a = z;
+            for (int k = t; k < q; ++k) {
+                const double2 p = dmul(Ts[t][k], C2[k][0]);
+                a.x += p.x;
+                a.y += p.y;
+            }
+            Ts[t][q] = double2{-beta * a.x, -beta * a.y};
         }
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < 2 * s * s; e += DLS) {
-        const int r = e % (2 * s), q = e / (2 * s);
-        Rq[(int64_t)(js + q) * ld + js + r] = c2<double>{Wn[r][q].x, Wn[r][q].y};
+    for (int e = t; e < s2 * s; e += DLS) {
+        const int r = e % s2, q = e / s2;
+        *rq(js + r, q) = c2<double>{Wn[r][q].x, Wn[r][q].y};
         G[(int64_t)q * ld + js + r] = c2<double>{Gw[r][q].x, Gw[r][q].y};
     }
-    if (threadIdx.x < s) {
-        const int q = threadIdx.x;
+    for (int e = t; e < s * s; e += DLS) {
+        const int k = e % s, i = e / s;
+        Tb[((int64_t)j * s + i) * s + k] = c2<double>{Ts[k][i].x, Ts[k][i].y};
+    }
+    if (t < s) {
         double r2 = 0.0;
-        for (int r = s; r < 2 * s; ++r) r2 += Gw[r][q].x * Gw[r][q].x + Gw[r][q].y * Gw[r][q].y;
-        res[q] = sqrt(r2);
+        for (int r = s; r < s2; ++r) r2 += Gw[r][t].x * Gw[r][t].x + Gw[r][t].y * Gw[r][t].y;
+        res[t] = sqrt(r2);
     }
 }
 
@@ -1110,6 +1158,7 @@ struct DevWs {
     c2<double>* Gd = nullptr;    // ld x s transformed right-hand side
     c2<double>* hv = nullptr;    // m s reflectors of length s + 1
     double* hb = nullptr;        // m s reflector factors
+    c2<double>* Tb = nullptr;    // m blocks of s x s: each block's reflectors as I - V T V^H
     c2<double>* Gm = nullptr;    // s x s CholeskyQR Gram
     c2<double>* Ri = nullptr;    // 3 s x s inverse factors
     c2<double>* Rf = nullptr;    // s x s accumulated factor
@@ -1119,7 +1168,7 @@ struct DevWs {
     std::vector<hipGraphExec_t> graphs;
     std::vector<hipGraph_t> gsrc;
     void free_all() {
-        void* p[] = {Rq, Gd, hv, hb, Gm, Ri, Rf, st};
+        void* p[] = {Rq, Gd, hv, hb, Tb, Gm, Ri, Rf, st};
         for (void* q : p)
             if (q) (void)hipFree(q);
         if (hs) (void)hipHostFree(hs);
@@ -1196,6 +1245,7 @@ struct DevSolve {
         if (!st) st = A((void**)&d.Gd, L * s * sizeof(c2<double>));
         if (!st) st = A((void**)&d.hv, (size_t)m * s * (s + 1) * sizeof(c2<double>));
         if (!st) st = A((void**)&d.hb, (size_t)m * s * sizeof(double));
+        if (!st) st = A((void**)&d.Tb, (size_t)m * s * s * sizeof(c2<double>));
         if (!st) st = A((void**)&d.Gm, (size_t)s * s * sizeof(c2<double>));
         if (!st) st = A((void**)&d.Ri, (size_t)3 * s * s * sizeof(c2<double>));
         if (!st) st = A((void**)&d.Rf, (size_t)s * s * sizeof(c2<double>));
@@ -1252,7 +1302,8 @@ struct DevSolve {
         st = cholqr_dev(c.ws.W, c.ws.V + (int64_t)(jj + 1) * s * n, &d.st->brk);
         if (st) return st;
         hipLaunchKernelGGL(k_rset, dim3(1), dim3(DS * DS), 0, c.s, d.Rf, s, d.Rq, ld(), jj * s, P);
-        hipLaunchKernelGGL(k_bls, dim3(1), dim3(DLS), 0, c.s, d.Rq, ld(), d.Gd, d.hv, d.hb, s, jj, d.st->res);
+        hipLaunchKernelGGL(k_bls, dim3(1), dim3(DLS), 0, c.s, d.Rq, ld(), d.Gd, d.hv, d.hb, d.Tb, s, jj,
+                           d.st->res);
         MARIE_HIP(hipGetLastError());
         return OK;
     }
