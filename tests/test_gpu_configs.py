@@ -9,9 +9,11 @@
   1e-6 solve's own error level.
 * batch: seeded phantoms of 64^3, 80^3, 96^3 and 128^3 voxels over a 0.192 m
   domain at 128 and 298 MHz, each driven by the 16 ports of a cylinder loop
-  coil, solved together by marie_vie_block_gmres_batch; every column's true
-  residual is recomputed, and two ports per problem are compared with single
-  GMRES solves (the per-port loops of VSIE_Solver.m:387-407 the batch replaces).
+  coil, solved together by marie_vie_block_gmres_batch, host-driven and with
+  device-driven block steps (each step a HIP graph, one host thread for all
+  problems); every column's true residual is recomputed, and two ports per
+  problem are compared with single GMRES solves (the per-port loops of
+  VSIE_Solver.m:387-407 the batch replaces).
 * differentiation: the 128^3 phantom, the 8 ports of a two-ring coil in
   circular polarisation, objective = coefficient of variation of |B1+| over a
   central ellipsoid + L2 misfit to a seeded target; the autograd gradient
@@ -74,7 +76,7 @@ def test_config4_batch_bodies_frequencies_ports():
     print("case batch: 64/80/96/128^3 phantoms x 128/298 MHz x 16 ports", flush=True)
     dev = torch.device('cuda:0')
     ports = coil.cylinder_loop_array(16, 0.15, 0.1, 0.05, seg_per_side=8, rings=2, length=0.2)
-    probs = []
+    probs, streams = [], []
     for nb, seed in ((64, 1), (80, 2), (96, 3), (128, 4)):
         dxb = 0.192 / nb
         for fb in (128e6, 298e6):
@@ -85,6 +87,9 @@ def test_config4_batch_bodies_frequencies_ports():
             cols = np.stack([fac * Ep.ravel(order='F')[idxb - 1] for Ep in Es])
             assert _rel(cols[0], v0) <= 1e-12
             opb = lib.Operator(nb, nb, nb)
+            st = torch.cuda.Stream(device=dev)          # each problem on its own stream
+            opb.set_stream(st.cuda_stream)
+            streams.append(st)
             opb.assemble_N(dxb, emb['ko'])
             opb.set_material(Mrb, Mcb, Gb)
             opb.set_index(idxb, one_based=True)
@@ -95,6 +100,23 @@ def test_config4_batch_bodies_frequencies_ports():
                                           [x.data_ptr() for x in Xs], [16] * len(probs), restart=20, tol=1e-6,
                                           maxit=100, mode=0, inner_tol=1e-7, use_graph=True)
     torch.cuda.synchronize()
+    # the same batch with device-driven block steps (each step a HIP graph),
+    # all problems driven from one host thread
+    Xd = [torch.zeros_like(p[3]) for p in probs]
+    torch.cuda.synchronize()
+    fld, rrd, itd = lib.block_gmres_batch([p[2] for p in probs], [p[3].data_ptr() for p in probs],
+                                          [x.data_ptr() for x in Xd], [16] * len(probs), restart=20, tol=1e-6,
+                                          maxit=100, mode=0, inner_tol=1e-7, use_graph=True, device_steps=True)
+    torch.cuda.synchronize()
+    for (nb, fb, opb, Bb), X, fl, rr, Xdv, fd, rd in zip(probs, Xs, fls, rrs, Xd, fld, rrd):
+        assert int(fd.max()) == 0 and float(rd.max()) <= 1e-6, (nb, fb)
+        Bh, Xh = Bb.cpu().numpy(), Xdv.cpu().numpy()
+        AX = opb.apply(Xh.T)
+        worst = max(_rel(AX[:, q], Bh[q]) for q in range(16))
+        assert worst <= 1e-6, (nb, fb, worst)
+        assert _rel(Xh, X.cpu().numpy()) <= 1e-5, (nb, fb)
+        print("  device-driven %d^3 %.0f MHz: max relres %.2e (recomputed %.2e)" % (nb, fb / 1e6, float(rd.max()),
+                                                                                  worst), flush=True)
     for (nb, fb, opb, Bb), X, fl, rr in zip(probs, Xs, fls, rrs):
         assert int(fl.max()) == 0 and float(rr.max()) <= 1e-6, (nb, fb)
         # every column's true residual, one 16-column product
