@@ -370,8 +370,11 @@ int marie_vie_dgf_fields_device(const double* seg_center_dev, const void* seg_J_
                                 void* E_dev, void* H_dev, void* stream);
 
 /* Tile autotuner: times every compiled variant of the five matvec kernels
- * with HIP events on the handle's stream, `iters` products each, and keeps
- * the fastest per kernel for precision `prec`. A variant is cfg6[0..4], one
+ * with HIP events on the handle's stream, `iters` products each, picks the
+ * fastest per kernel for precision `prec`, then revisits each kernel with
+ * the others at their picks and keeps the variant that gives the fastest
+ * whole product (a kernel's time depends on what its neighbours leave in the
+ * caches). A variant is cfg6[0..4], one
  * value per kernel (fwd_x, fwd_y, z, inv_y, inv_x): v > 0 selects the
  * LDS-stage FFT with v elements per thread, v < 0 the register-resident FFT
  * with -v points per thread (power-of-two sides 32-512 with 4, 8 or 16
@@ -379,14 +382,17 @@ int marie_vie_dgf_fields_device(const double* seg_center_dev, const void* seg_J_
  * with 10 or 20 -- z: 12, 10, and 14 in complex64); for the z kernel,
  * v = -(100*TX + E) selects the variant with one thread group per field
  * component (TX columns per workgroup, E points per thread, the three
- * spectra exchanged through LDS for the tensor product); cfg6[5] is reserved and
+ * spectra exchanged through LDS for the tensor product); for the y kernels,
+ * v = -(200 + E) selects two lines per thread on power-of-two sides
+ * (complex64: adjacent column pairs, one 16-byte access per point; FP64:
+ * columns TX apart, each access a contiguous run); cfg6[5] is reserved and
  * returned as 0. cfg6 / best_ms5 (optional) receive the choice and its mean
  * kernel times. No counterpart in the reference (MATLAB's fftn plans
  * internally). */
 int marie_vie_autotune(marie_vie_op* h, int prec, int iters, int* cfg6, double* best_ms5);
 
 /* Set the kernel variants explicitly (cfg6 as above). A new handle starts at
- * {-4, -16, -8, -8, -4} for FP64 and {-8, -16, -16, -16, -8} for FP32 (the
+ * {-4, -16, -8, -208, -4} for FP64 and {-8, -16, -16, -16, -8} for FP32 (the
  * autotuner's usual choice at 256-point sides); on a mixed-radix side a
  * register code takes that side's own plan (12, 14 or 20 points; the z
  * kernel at 224 one thread group per component, -414), and a kernel whose

@@ -440,7 +440,7 @@ extern "C" int marie_vie_set_tiles(marie_vie_op* h, int prec, const int* cfg6) {
     // a value between compiled variants runs the next larger one), the
     // register engine's points per thread (negative), for the z pass its
     // one-group-per-component codes -(100 * TX + E) and for the y passes the
-    // complex64 column-pair codes -(200 + E)
+    // two-lines-per-thread codes -(200 + E)
     static const int kReg[] = {4, 8, 16, 10, 12, 14, 20, 24, 28};
     static const int kZ3[] = {404, 408, 416, 804, 808, 816, 414};
     static const int kPair[] = {204, 208, 216};

@@ -88,7 +88,7 @@ struct Op {
     // as keep their x-pass buffers within kXYGroupBytes; MARIE_XY_GROUP at
     // creation sets it)
     int xy_group = 0;
-    int emax64[5] = {-4, -16, -8, -8, -4};
+    int emax64[5] = {-4, -16, -8, -208, -4};
     int emax32[5] = {-8, -16, -16, -16, -8};
 
     // optional per-kernel timing with HIP events on the handle's stream: each

@@ -709,9 +709,10 @@ static int apply_t(Op* h, int op, int transp, const c2<T>* x, c2<T>* y, int nrhs
         reg[k] = em[k] < 0 && (k == 2   ? reg_z_supported<T>(Pp[k], -em[k])
                                : (k & 1) ? reg_y_supported<T>(Pp[k], -em[k])
                                          : reg_supported<T>(Pp[k], -em[k]));
-    // column pairs need an even x side (a pair never straddles two rows)
+    // complex64 column pairs need an even x side (a pair never straddles two
+    // rows); the FP64 two-line tiles take any side
     for (int k = 1; k < 5; k += 2)
-        if (reg[k] && -em[k] > 200 && (Px & 1)) reg[k] = false;
+        if (reg[k] && -em[k] > 200 && (Px & 1) && !f64) reg[k] = false;
     // register-engine points per thread of each pass; a pass whose
     // mixed-radix side (96, 112, 160, 192, 224) has no plan for the requested
     // count takes that side's own (12, 14, 20 points; z: 12 and 10 where its
@@ -1035,8 +1036,10 @@ int apply_noalloc(Op* h, int op, int transp, int prec, const void* x, void* y, i
 }
 
 // Times every compiled tile variant of each pass with HIP events on the
-// handle's stream (real buffers, `iters` products each) and keeps the fastest
-// per pass. Results are left in h->emax64 / h->emax32.
+// handle's stream (real buffers, `iters` products each) and picks the fastest
+// per pass, then revisits each pass with the others at their picks and keeps
+// the code that gives the fastest whole product. Results are left in
+// h->emax64 / h->emax32; best_ms5 gets the final configuration's pass times.
 int autotune(Op* h, int prec, int iters, double* best_ms5) {
     const bool f64 = prec == PREC_F64;
     int* em = f64 ? h->emax64 : h->emax32;
@@ -1081,10 +1084,16 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
         for (int k = 0; k < 5; ++k) cands[nc][k] = -mixE[v];
         ++nc;
     }
-    // complex64 y passes on column pairs (code 200 + E), x and z passes at 16 points
+    // y passes on two lines per thread (code 200 + E): complex64 column pairs
+    // with x and z passes at 16 points; FP64 columns TX apart at 8 points
     if (!f64 && reg_y_supported<float>(h->Py, 216) && nc < 16) {
         for (int k = 0; k < 5; ++k) cands[nc][k] = -16;
         cands[nc][1] = cands[nc][3] = -216;
+        ++nc;
+    }
+    if (f64 && reg_y_supported<double>(h->Py, 208) && nc < 16) {
+        for (int k = 0; k < 5; ++k) cands[nc][k] = -8;
+        cands[nc][1] = cands[nc][3] = -208;
         ++nc;
     }
     // z pass with one thread group per component (code 100 * TX + E)
@@ -1119,6 +1128,58 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
             fprintf(stderr, "[marie tune] prec %d cfg [%d %d %d %d %d] : %.4f %.4f %.4f %.4f %.4f ms\n",
                     prec, em[0], em[1], em[2], em[3], em[4], h->kernel_ms[0] / iters, h->kernel_ms[1] / iters,
                     h->kernel_ms[2] / iters, h->kernel_ms[3] / iters, h->kernel_ms[4] / iters);
+    }
+    // refinement: a pass's time depends on what its neighbours left in the
+    // caches (after the slow one-group-per-component z candidates the y
+    // passes measure faster than beside the z pass they will run with), so
+    // each pass is revisited with the others at their picks, every code that
+    // pass had among the candidates, and the whole product's time decides
+    auto product_ms = [&](double* per) -> int {
+        h->timing = false;
+        int s2 = apply(h, OP_AN, 0, prec, x, y, 1, h->stream);
+        if (s2) return s2;
+        h->timing = true;
+        for (int k = 0; k < 8; ++k) h->kernel_ms[k] = 0.0;
+        h->kernel_calls = 0;
+        for (int it = 0; it < iters && s2 == OK; ++it) s2 = apply(h, OP_AN, 0, prec, x, y, 1, h->stream);
+        if (s2 == OK) s2 = timing_flush(h);
+        for (int k = 0; k < 5; ++k) per[k] = h->kernel_ms[k] / (h->kernel_calls ? h->kernel_calls : 1);
+        return s2;
+    };
+    if (st == OK) {
+        double per[5];
+        for (int k = 0; k < 5; ++k) em[k] = pick[k];
+        st = product_ms(per);
+        double total = 0.0;
+        for (int k = 0; k < 5; ++k) {
+            total += per[k];
+            best[k] = per[k];
+        }
+        for (int k = 0; k < 5 && st == OK; ++k) {
+            int seen[16], ns = 0;
+            for (int ci = 0; ci < nc; ++ci) {
+                bool dup = cands[ci][k] == pick[k];
+                for (int q = 0; q < ns; ++q) dup = dup || seen[q] == cands[ci][k];
+                if (!dup) seen[ns++] = cands[ci][k];
+            }
+            for (int q = 0; q < ns && st == OK; ++q) {
+                for (int k2 = 0; k2 < 5; ++k2) em[k2] = pick[k2];
+                em[k] = seen[q];
+                double p2[5];
+                st = product_ms(p2);
+                if (st) break;
+                double t2 = 0.0;
+                for (int k2 = 0; k2 < 5; ++k2) t2 += p2[k2];
+                if (verbose)
+                    fprintf(stderr, "[marie tune] refine pass %d code %d: %.4f ms (best %.4f)\n", k, seen[q], t2,
+                            total);
+                if (t2 < total) {
+                    total = t2;
+                    pick[k] = seen[q];
+                    for (int k2 = 0; k2 < 5; ++k2) best[k2] = p2[k2];
+                }
+            }
+        }
     }
     for (int k = 0; k < 5; ++k) em[k] = pick[k];
     h->timing = was;

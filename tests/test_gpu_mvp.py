@@ -349,6 +349,30 @@ def test_column_pair_y_pass(shape):
                 assert _rel(op.apply(x, L_.OP_AN, True), yt) <= TOL32, (code, ex)
 
 
+@pytest.mark.parametrize("shape", [(16, 16, 32), (20, 64, 12), (24, 128, 8)])
+def test_two_line_y_pass_fp64(shape):
+    """FP64 y passes on two lines per thread (codes 200 + E: columns TX apart,
+    each load a contiguous run) give the oracle's products to 1e-12: both
+    directions, two right-hand sides; x sides 32, 40, 48 leave the last
+    2 TX-column tile partly empty."""
+    from marie_amd import lib as L_
+    L, M, N = shape
+    print("case fp64 two-line y", shape, flush=True)
+    rng = np.random.default_rng(45)
+    fN, Mr, Mc, idx, x = _case(L, M, N, seed=46)
+    x2 = np.stack([x, rng.standard_normal(x.size) + 1j * rng.standard_normal(x.size)], axis=1)
+    op = _op(L, M, N, None, fN, Mr, Mc, 0.8, idx, prec=L_.PREC_F64)
+    yr = [mvp.mv_AN(x2[:, q], fN, Mr, Mc, 0.8, 'notransp', idx) for q in range(2)]
+    yt = mvp.mv_AN(x, fN, Mr, Mc, 0.8, 'transp', idx)
+    for code in (204, 208, 216):
+        for fwd, inv in ((-code, -8), (-8, -code), (-code, -code)):
+            op.set_tiles([-8, fwd, -8, inv, -4, 0], L_.PREC_F64)
+            y = op.apply(x2)
+            for q in range(2):
+                assert _rel(y[:, q], yr[q]) <= TOL64, (code, fwd, inv, q)
+            assert _rel(op.apply(x, L_.OP_AN, True), yt) <= TOL64, (code, fwd, inv)
+
+
 MIXED = [
     ((112, 96, 10), ([-14, -12, 8, -12, -14], [-28, -24, 8, -24, -28], [-28, -12, 8, -24, -14])),  # 224, 192
     ((48, 56, 10), ([-12, -14, 8, -14, -12], [-24, -28, 8, -28, -24], [-24, -14, 8, -28, -12])),    # 96, 112
