@@ -83,16 +83,53 @@ struct Cache {
     int precisions = MARIE_PREC_F64;     // MARIE_PREC_* the cached operator is created with
     marie_vie_op* op = nullptr;
     int dims[6] = {0, 0, 0, 0, 0, 0};
-    Identity fN, Mr, Mc, idx;
+    Identity fN, Mr, Mc, idx;            // fN: the operator array (fN, or fK for mv_K)
+    int ncomp = 0;                       // 6: fN, 3: fK
     double gram = 0.0;
-    bool have_mat = false, have_idx = false;
+    bool have_mat = false, have_idx = false, unit_mat = false;
     int64_t nidx = 0;
 
     void clear() {
         if (op) marie_vie_destroy(op);
         op = nullptr;
-        have_mat = have_idx = false;
+        have_mat = have_idx = unit_mat = false;
+        ncomp = 0;
         fN = Mr = Mc = idx = Identity();
+    }
+
+    // fN (nc = 6) or fK (nc = 3), Px x Py x Pz x nc, on the grid of r
+    // (L x M x N x 3): the operator and domain arguments of mv_N.m, mv_L.m
+    // and mv_K.m (their sizes, mv_N.m:35-38)
+    void operator_on_grid(const mxArray* fa, const mxArray* ra, int nc) {
+        const mwSize* fd = mxGetDimensions(fa);
+        const mwSize* rd = mxGetDimensions(ra);
+        if (mxGetNumberOfDimensions(fa) != 4 || fd[3] != (mwSize)nc)
+            mexErrMsgIdAndTxt("MARIE:hip:fN", nc == 6 ? "fN must be Px x Py x Pz x 6." : "fK must be Px x Py x Pz x 3.");
+        if (mxGetNumberOfDimensions(ra) != 4 || rd[3] != 3)
+            mexErrMsgIdAndTxt("MARIE:hip:r", "r must be L x M x N x 3.");
+        const int d[6] = {(int)rd[0], (int)rd[1], (int)rd[2], (int)fd[0], (int)fd[1], (int)fd[2]};
+        const Identity idf = identity(fa);
+        if (op && idf == fN && ncomp == nc && std::memcmp(d, dims, sizeof d) == 0) return;
+        clear();
+        fail(marie_vie_create(&op, d[0], d[1], d[2], d[3], d[4], d[5], precisions, 0));
+        std::vector<double> buf;
+        interleave(fa, buf);
+        fail(nc == 6 ? marie_vie_set_fN(op, buf.data()) : marie_vie_set_fK(op, buf.data()));
+        fN = idf;
+        ncomp = nc;
+        std::memcpy(dims, d, sizeof d);
+    }
+
+    // Gram alone (mv_L.m: N x - Gram x): the material Mr = 1, Mc = 0
+    void gram_only(double g) {
+        if (have_mat && unit_mat && g == gram) return;
+        const size_t nv = (size_t)dims[0] * dims[1] * dims[2];
+        std::vector<double> one(2 * nv, 0.0), zero(2 * nv, 0.0);
+        for (size_t i = 0; i < nv; ++i) one[2 * i] = 1.0;
+        fail(marie_vie_set_material(op, one.data(), zero.data(), g));
+        Mr = Mc = Identity();
+        gram = g;
+        have_mat = unit_mat = true;
     }
 
     // fN: Px x Py x Pz x 6; Mr: the L x M x N voxel grid
@@ -105,13 +142,14 @@ struct Cache {
         const int d[6] = {(int)md[0], nd > 1 ? (int)md[1] : 1, nd > 2 ? (int)md[2] : 1, (int)fd[0], (int)fd[1],
                           (int)fd[2]};
         const Identity idf = identity(fNa);
-        if (op && idf == fN && std::memcmp(d, dims, sizeof d) == 0) return;
+        if (op && idf == fN && ncomp == 6 && std::memcmp(d, dims, sizeof d) == 0) return;
         clear();
         fail(marie_vie_create(&op, d[0], d[1], d[2], d[3], d[4], d[5], precisions, 0));
         std::vector<double> buf;
         interleave(fNa, buf);
         fail(marie_vie_set_fN(op, buf.data()));
         fN = idf;
+        ncomp = 6;
         std::memcpy(dims, d, sizeof d);
     }
 
@@ -120,7 +158,7 @@ struct Cache {
         if (mxGetNumberOfElements(Mra) != nv || mxGetNumberOfElements(Mca) != nv)
             mexErrMsgIdAndTxt("MARIE:hip:material", "Mr and Mc must be L x M x N.");
         const Identity a = identity(Mra), b = identity(Mca);
-        if (have_mat && a == Mr && b == Mc && g == gram) return;
+        if (have_mat && !unit_mat && a == Mr && b == Mc && g == gram) return;
         std::vector<double> r, c;
         interleave(Mra, r);
         interleave(Mca, c);
@@ -129,6 +167,7 @@ struct Cache {
         Mc = b;
         gram = g;
         have_mat = true;
+        unit_mat = false;
     }
 
     // MATLAB's 1-based linear indices into the L x M x N x 3 grid (idxS3)

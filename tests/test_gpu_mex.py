@@ -4,7 +4,10 @@ MEX API emulation of tests/mex, against the oracle.
 mv_AN_hip(JIn0, fN, Mr, Mc, Gram, transp_flag, idx) must return mv_AN.m's
 column for both flags, keep its device operator across calls with the same
 arrays and follow a changed material; jvie_gmres_hip must return pgmres'
-[x, flag, relres, iter, resvec] for the JVIE system (pgmres.m:70-181).
+[x, flag, relres, iter, resvec] for the JVIE system (pgmres.m:70-181);
+mv_N_hip(JIn0, fN, r, idx), mv_L_hip(JIn0, fN, r, Gram, idx) and
+mv_K_hip(JIn0, fK, r, idx) must return the columns of mv_N.m, mv_L.m and
+mv_K.m.
 """
 import os
 import sys
@@ -90,3 +93,37 @@ def test_jvie_gmres_gateway():
     with pytest.raises(emul.MexError, match='MARIE:hip:precision'):
         g(b, fN, Mr, Mc, Gram, (idx + 1).astype(float), 40, 1e-6, 20, np.zeros(0), 'fp16')
     g('clear', nargout=0)
+
+
+def test_mv_N_L_K_gateways():
+    import emul
+    print("case MEX gateways mv_N_hip, mv_L_hip, mv_K_hip", flush=True)
+    n, dx, f = 9, 6.25e-3, 298.2e6
+    x1 = (np.arange(n) - (n - 1) / 2) * dx
+    X, Y, Z = np.meshgrid(x1, x1, x1, indexing='ij')
+    r = np.stack([X, Y, Z], axis=-1)
+    em = fields.em_constants(f)
+    fN = oop.get_operators(n, n, n, dx, f, 'N')
+    fK = oop.fft_operator(oop.circulant_kop(oop.assembly_kop(n, n, n, em['ko'], dx)))
+    rng = np.random.default_rng(5)
+    # E_field_Nop.m:69 / H_field_Kop.m:69 use every DOF; a subset checks the scatter too
+    for idx in (np.arange(3 * n ** 3), np.sort(rng.choice(3 * n ** 3, 1500, replace=False))):
+        idx1 = (idx + 1).astype(float)
+        x = rng.standard_normal(idx.size) + 1j * rng.standard_normal(idx.size)
+        gN, gL, gK = emul.Gateway('mv_N_hip'), emul.Gateway('mv_L_hip'), emul.Gateway('mv_K_hip')
+        y = gN(x, fN, r, idx1)
+        assert y.shape == (idx.size, 1)
+        assert _rel(y[:, 0], mvp.mv_N(x, fN, n, n, n, idx)) <= 1e-12
+        for gram in (dx ** 3, 2 * dx ** 3):              # a changed Gram is applied
+            y = gL(x, fN, r, gram, idx1)
+            assert _rel(y[:, 0], mvp.mv_L(x, fN, n, n, n, gram, idx)) <= 1e-12
+        y = gK(x, fK, r, idx1)
+        assert _rel(y[:, 0], mvp.mv_K(x, fK, n, n, n, idx)) <= 1e-12
+        with pytest.raises(emul.MexError, match='MARIE:hip:r'):
+            gN(x, fN, r[..., 0], idx1)
+        with pytest.raises(emul.MexError, match='MARIE:hip:fN'):
+            gK(x, fN, r, idx1)
+        with pytest.raises(emul.MexError, match='MARIE:hip:size'):
+            gL(x[:-1], fN, r, dx ** 3, idx1)
+        for g in (gN, gL, gK):
+            g('clear', nargout=0)
