@@ -36,6 +36,7 @@ class Gateway:
         so.emul_im.restype = _P
         so.emul_im.argtypes = [_P]
         so.emul_dim.argtypes = [_P, ctypes.c_int]
+        so.emul_ndim.argtypes = [_P]
         so.emul_call.argtypes = [ctypes.c_int, _P, ctypes.c_int, _P, ctypes.c_char_p, ctypes.c_int]
         self.so = so
 
@@ -56,7 +57,7 @@ class Gateway:
 
     def _out(self, p):
         n = self.so.emul_numel(p)
-        shape = (self.so.emul_dim(p, 0), self.so.emul_dim(p, 1))
+        shape = tuple(self.so.emul_dim(p, d) for d in range(max(2, self.so.emul_ndim(p))))
         re = np.ctypeslib.as_array((ctypes.c_double * n).from_address(self.so.emul_re(p))).copy() if n else \
             np.zeros(0)
         if self.so.emul_is_complex(p):

@@ -9,6 +9,7 @@
 typedef size_t mwSize;
 typedef struct mxArray_tag mxArray;
 typedef enum { mxREAL = 0, mxCOMPLEX = 1 } mxComplexity;
+typedef enum { mxDOUBLE_CLASS = 6 } mxClassID;
 
 #ifdef __cplusplus
 extern "C" {
@@ -27,6 +28,7 @@ double mxGetScalar(const mxArray* a);
 int mxGetString(const mxArray* a, char* buf, mwSize len);
 mxArray* mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity c);
 mxArray* mxCreateDoubleScalar(double v);
+mxArray* mxCreateNumericArray(mwSize ndim, const mwSize* dims, mxClassID classid, mxComplexity c);
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,19 @@ mxArray* mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity c) {
     if (a->cplx) a->im.assign(m * n, 0.0);
     return a;
 }
+mxArray* mxCreateNumericArray(mwSize ndim, const mwSize* dims, mxClassID classid, mxComplexity c) {
+    if (classid != mxDOUBLE_CLASS) throw MexError("emulation: double arrays only");
+    auto* a = new mxArray_tag();
+    size_t n = 1;
+    for (mwSize d = 0; d < ndim; ++d) {
+        a->dims.push_back(dims[d]);
+        n *= dims[d];
+    }
+    a->re.assign(n, 0.0);
+    a->cplx = c == mxCOMPLEX;
+    if (a->cplx) a->im.assign(n, 0.0);
+    return a;
+}
 mxArray* mxCreateDoubleScalar(double v) {
     mxArray* a = mxCreateDoubleMatrix(1, 1, mxREAL);
     a->re[0] = v;
@@ -99,6 +112,7 @@ int emul_is_complex(const mxArray* a) { return a->cplx ? 1 : 0; }
 const double* emul_re(const mxArray* a) { return a->re.data(); }
 const double* emul_im(const mxArray* a) { return a->cplx ? a->im.data() : nullptr; }
 int emul_dim(const mxArray* a, int d) { return d < (int)a->dims.size() ? (int)a->dims[(size_t)d] : 1; }
+int emul_ndim(const mxArray* a) { return (int)a->dims.size(); }
 int emul_call(int nlhs, mxArray** plhs, int nrhs, mxArray** prhs, char* err, int errlen) {
     try {
         mexFunction(nlhs, plhs, nrhs, const_cast<const mxArray**>(prhs));

@@ -105,7 +105,8 @@ def test_mex_gateways_build_and_check_arguments_without_gpu():
     subprocess.run(['make', '-s', '-C', os.path.join(ROOT, 'tests', 'mex')], check=True)
     sys.path.insert(0, os.path.join(ROOT, 'tests', 'mex'))
     import emul
-    for name, nargs in (('mv_AN_hip', 3), ('jvie_gmres_hip', 4), ('mv_N_hip', 3), ('mv_L_hip', 4), ('mv_K_hip', 2)):
+    for name, nargs in (('mv_AN_hip', 3), ('jvie_gmres_hip', 4), ('mv_N_hip', 3), ('mv_L_hip', 4), ('mv_K_hip', 2),
+                        ('jvie_bicgstab_hip', 5), ('getOPERATORS_hip', 1)):
         g = emul.Gateway(name)
         with pytest.raises(emul.MexError, match='MARIE:hip:nargin'):
             g(*([np.zeros(2)] * nargs))
