@@ -19,6 +19,7 @@
 #include "marie_mex.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <string>
 
 namespace {
@@ -81,13 +82,23 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     const double ko = 2.0 * pi / 299792458.0 * mxGetScalar(prhs[1]);
     const int P[3] = {circulant_side(L), circulant_side(M), circulant_side(N)};
     Op o;
-    fail(marie_vie_create(&o.h, L, M, N, P[0], P[1], P[2], MARIE_PREC_F64, 0));
+    // mexErrMsgIdAndTxt does not return and may skip destructors: release the
+    // handle before raising
+    auto check = [&o](int st) {
+        if (!st) return;
+        char msg[512];
+        std::snprintf(msg, sizeof msg, "%s", marie_vie_last_error());
+        if (o.h) marie_vie_destroy(o.h);
+        o.h = nullptr;
+        mexErrMsgIdAndTxt("MARIE:hip:status", "libmarie_vie: %s", msg);
+    };
+    check(marie_vie_create(&o.h, L, M, N, P[0], P[1], P[2], MARIE_PREC_F64, 0));
     const bool isN = op == "N";
     const int nc = isN ? 6 : 3;
     double sv[10];
     const double* singular = nullptr;
     if (isN && sing == "DIRECTFN") {
-        fail(marie_vie_singular_terms_directfn(dx, ko, 20, sv));
+        check(marie_vie_singular_terms_directfn(dx, ko, 20, sv));
         singular = sv;
     }
     mwSize dims[4];
@@ -101,14 +112,14 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     std::vector<double> out(2 * count);
     if (type == "FFTofC") {
         if (isN) {
-            fail(marie_vie_assemble_N(o.h, dx, ko, singular));
-            fail(marie_vie_get_fN(o.h, out.data()));
+            check(marie_vie_assemble_N(o.h, dx, ko, singular));
+            check(marie_vie_get_fN(o.h, out.data()));
         } else {
-            fail(marie_vie_assemble_K(o.h, dx, ko));
-            fail(marie_vie_get_fK(o.h, out.data()));
+            check(marie_vie_assemble_K(o.h, dx, ko));
+            check(marie_vie_get_fK(o.h, out.data()));
         }
     } else {
-        fail(marie_vie_assemble_type(o.h, isN ? MARIE_OP_N : MARIE_OP_K, dx, ko, singular,
+        check(marie_vie_assemble_type(o.h, isN ? MARIE_OP_N : MARIE_OP_K, dx, ko, singular,
                                      type == "T" ? MARIE_TYPE_T : MARIE_TYPE_C, out.data()));
     }
     plhs[0] = mxCreateNumericArray(4, dims, mxDOUBLE_CLASS, mxCOMPLEX);
