@@ -1049,9 +1049,14 @@ int autotune(Op* h, int prec, int iters, double* best_ms5) {
     const size_t es = f64 ? sizeof(c2<double>) : sizeof(c2<float>);
     void *x = nullptr, *y = nullptr;
     const size_t n = (size_t)(h->nidx > 0 ? h->nidx : 1);
-    MARIE_HIP(hipMalloc(&x, n * es));
-    MARIE_HIP(hipMalloc(&y, n * es));
-    MARIE_HIP(hipMemsetAsync(x, 0, n * es, h->stream));
+    if (hipMalloc(&x, n * es) != hipSuccess || hipMalloc(&y, n * es) != hipSuccess ||
+        hipMemsetAsync(x, 0, n * es, h->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        if (x) (void)hipFree(x);
+        if (y) (void)hipFree(y);
+        set_error("out of device memory for the autotuner's vectors");
+        return ERR_ALLOC;
+    }
     double best[5] = {1e30, 1e30, 1e30, 1e30, 1e30};
     int pick[5] = {em[0], em[1], em[2], em[3], em[4]};
     const bool was = h->timing;
