@@ -364,21 +364,23 @@ __global__ __launch_bounds__(ST) void k_supdate(c2<float>* __restrict__ Q, int64
     if (skip && *skip) return;
     const int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x;
     if (e >= n) return;
+    // the block's columns: all SM loads issued before the first is used
+    // (columns past sb read column k and are zeroed)
     double ax[SM], ay[SM];
+    c2<float> bq[SM];
+#pragma unroll
+    for (int c = 0; c < SM; ++c) bq[c] = Q[(int64_t)(k + (c < sb ? c : 0)) * ldq + e];
 #pragma unroll
     for (int c = 0; c < SM; ++c) {
-        if (c < sb) {
-            const double2 v = ld2(Q + (int64_t)(k + c) * ldq + e);
-            ax[c] = v.x;
-            ay[c] = v.y;
-        } else {
-            ax[c] = 0.0;
-            ay[c] = 0.0;
-        }
+        ax[c] = c < sb ? (double)bq[c].x : 0.0;
+        ay[c] = c < sb ? (double)bq[c].y : 0.0;
     }
     if (proj) {
-        for (int i = 0; i < k; ++i) {
-            const c2<float> qf = ldf_nt(Q + (int64_t)i * ldq + e);
+        // basis rows QU at a time, their loads in flight together (one load
+        // per wave in flight left the projection latency bound); the sums
+        // run over i in the same order as one row at a time
+        constexpr int QU = 4;
+        auto project = [&](const c2<float> qf, int i) {
             const double2 q = double2{(double)qf.x, (double)qf.y};
             const c2<double>* ci = C + (int64_t)i * SM;
 #pragma unroll
@@ -387,7 +389,16 @@ __global__ __launch_bounds__(ST) void k_supdate(c2<float>* __restrict__ Q, int64
                 ax[c] -= q.x * cc.x - q.y * cc.y;
                 ay[c] -= q.x * cc.y + q.y * cc.x;
             }
+        };
+        int i = 0;
+        for (; i + QU <= k; i += QU) {
+            c2<float> qf[QU];
+#pragma unroll
+            for (int u = 0; u < QU; ++u) qf[u] = ldf_nt(Q + (int64_t)(i + u) * ldq + e);
+#pragma unroll
+            for (int u = 0; u < QU; ++u) project(qf[u], i + u);
         }
+        for (; i < k; ++i) project(ldf_nt(Q + (int64_t)i * ldq + e), i);
     }
 #pragma unroll
     for (int d = SM - 1; d >= 0; --d) {
