@@ -582,14 +582,25 @@ __global__ __launch_bounds__(ST) void k_scomb(c2<double>* __restrict__ x, const 
     __shared__ c2<double> cs[kMaxBasis];
     for (int i = threadIdx.x; i < k; i += ST) cs[i] = y[i];
     __syncthreads();
+    // basis rows QU at a time, their (non-temporal) loads in flight together;
+    // the sum runs over i in order
+    constexpr int QU = 4;
     for (int64_t e = blockIdx.x * (int64_t)ST + threadIdx.x; e < n; e += (int64_t)gridDim.x * ST) {
         double2 acc = ld2(x + e);
-        for (int i = 0; i < k; ++i) {
-            const double2 a = ld2(Q + (int64_t)i * ldq + e);
+        auto add = [&](const c2<float> a, int i) {
             const c2<double> c = cs[i];
-            acc.x += a.x * c.x - a.y * c.y;
-            acc.y += a.x * c.y + a.y * c.x;
+            acc.x += (double)a.x * c.x - (double)a.y * c.y;
+            acc.y += (double)a.x * c.y + (double)a.y * c.x;
+        };
+        int i = 0;
+        for (; i + QU <= k; i += QU) {
+            c2<float> a[QU];
+#pragma unroll
+            for (int u = 0; u < QU; ++u) a[u] = ldf_nt(Q + (int64_t)(i + u) * ldq + e);
+#pragma unroll
+            for (int u = 0; u < QU; ++u) add(a[u], i + u);
         }
+        for (; i < k; ++i) add(ldf_nt(Q + (int64_t)i * ldq + e), i);
         x[e] = c2<double>{acc.x, acc.y};
     }
 }
