@@ -1453,8 +1453,12 @@ struct DevSolve {
 // Drive the solves to the end from one host thread: whichever solve's event
 // has completed advances and enqueues its next piece of work on its stream.
 void drive(std::vector<DevSolve*>& ps) {
+    // each problem's work is enqueued with its handle's device current
     for (DevSolve* p : ps)
-        if (p->phase != DevSolve::DONE) p->init();
+        if (p->phase != DevSolve::DONE) {
+            (void)hipSetDevice(p->h->device);
+            p->init();
+        }
     for (;;) {
         int active = 0;
         bool moved = false;
@@ -1468,6 +1472,7 @@ void drive(std::vector<DevSolve*>& ps) {
                 p->fail(ERR_HIP);
                 continue;
             }
+            (void)hipSetDevice(p->h->device);
             p->advance();
             moved = true;
         }
