@@ -208,22 +208,27 @@ __global__ __launch_bounds__(KT) void k_reduce_real(const double* __restrict__ p
     if (threadIdx.x == 0) out[0] = acc[0];
 }
 
-// *out = {sqrt(sum_b p[b]), 0}: the subdiagonal entry H(k+1,k) = ||w||
-__global__ __launch_bounds__(KT) void k_norm_entry(const double* __restrict__ p, int G, c2<double>* __restrict__ out) {
+// v = w / ||w|| with ||w||^2 = sum of the G partials p (zero when the norm is
+// zero: breakdown). Every block adds the partials in the same fixed order,
+// so all use the same norm; block 0 stores it in *out (H(k+1,k)). One launch
+// for the norm and the scaling.
+template <typename TV>
+__global__ __launch_bounds__(KT) void k_scale_by_norm(c2<TV>* __restrict__ v, const c2<TV>* __restrict__ w,
+                                                      const double* __restrict__ p, int G,
+                                                      c2<double>* __restrict__ out, int64_t n) {
     __shared__ double sh[KT / 64];
+    __shared__ double nrm;
     double acc[1] = {0.0};
     for (int b = threadIdx.x; b < G; b += KT) acc[0] += p[b];
     block_reduce<1>(acc, sh);
-    if (threadIdx.x == 0) *out = c2<double>{sqrt(acc[0]), 0.0};
-}
-
-// v = w / nrm->x (zero when the norm is zero: breakdown)
-template <typename TV>
-__global__ void k_scale_by_entry(c2<TV>* __restrict__ v, const c2<TV>* __restrict__ w,
-                                 const c2<double>* __restrict__ nrm, int64_t n) {
-    const double d = nrm->x;
+    if (threadIdx.x == 0) {
+        nrm = sqrt(acc[0]);
+        if (blockIdx.x == 0) *out = c2<double>{nrm, 0.0};
+    }
+    __syncthreads();
+    const double d = nrm;
     const double a = d > 0.0 ? 1.0 / d : 0.0;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t e = blockIdx.x * (int64_t)KT + threadIdx.x; e < n; e += (int64_t)gridDim.x * KT) {
         const double2 x = ldc(w + e);
         v[e] = c2<TV>{(TV)(a * x.x), (TV)(a * x.y)};
     }
@@ -376,9 +381,9 @@ static int gmres_cycle(Ctx& c, c2<double>* x, c2<TV>* r, int m, double abstol, b
         // w = w - V h with ||w||^2 partials (pgmres_iter.m:107, :109)
         hipLaunchKernelGGL((k_vcomb<TV, TV>), dim3(c.G), dim3(KT), 0, c.s, w, 1.0, V, c.ldv, k, hc, -1.0, n,
                            c.ws.npart, c.G);
-        hipLaunchKernelGGL(k_norm_entry, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, hc + k);
-        // V(:,k+1) = w / H(k+1,k)  (pgmres_iter.m:110)
-        hipLaunchKernelGGL(k_scale_by_entry<TV>, dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)k * c.ldv, w, hc + k, n);
+        // H(k+1,k) = ||w||, V(:,k+1) = w / H(k+1,k)  (pgmres_iter.m:109-110)
+        hipLaunchKernelGGL(k_scale_by_norm<TV>, dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)k * c.ldv, w, c.ws.npart,
+                           c.G, hc + k, n);
         MARIE_HIP(hipGetLastError());
         MARIE_HIP(hipMemcpyAsync(c.ws.hp + (size_t)(k - 1) * ld, hc, (k + 1) * sizeof(c2<double>),
                                  hipMemcpyDeviceToHost, c.s));
@@ -805,9 +810,8 @@ static int dr_cycle(Ctx& c, c2<double>* Cb, int k, c2<double>* V, const c2<doubl
         hipLaunchKernelGGL(k_reduce, dim3(nc), dim3(KT), 0, c.s, c.ws.part, c.G, c.ws.h);
         hipLaunchKernelGGL((k_vcomb<double, double>), dim3(c.G), dim3(KT), 0, c.s, w, 1.0, Cb, n, nc, c.ws.h, -1.0,
                            n, c.ws.npart, c.G);
-        hipLaunchKernelGGL(k_norm_entry, dim3(1), dim3(KT), 0, c.s, c.ws.npart, c.G, c.ws.h + nc);
-        hipLaunchKernelGGL(k_scale_by_entry<double>, dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)j * n, w,
-                           c.ws.h + nc, n);
+        hipLaunchKernelGGL(k_scale_by_norm<double>, dim3(c.G), dim3(KT), 0, c.s, V + (int64_t)j * n, w, c.ws.npart,
+                           c.G, c.ws.h + nc, n);
         MARIE_HIP(hipGetLastError());
         MARIE_HIP(hipMemcpyAsync(c.ws.hp, c.ws.h, (nc + 1) * sizeof(c2<double>), hipMemcpyDeviceToHost, c.s));
         MARIE_HIP(hipStreamSynchronize(c.s));
